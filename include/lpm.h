@@ -1,0 +1,119 @@
+/* C ABI of liblpm.so: the MI355X (gfx950) kernels of the LPM elastic solve path.
+ *
+ * Every buffer is a device pointer owned by the caller (PyTorch tensors' data_ptr(), or hipMalloc from another
+ * host).  `stream` is a hipStream_t (the caller's current stream).  Every entry point returns an int status
+ * (LPM_OK = 0); lpm_last_error() describes the last failure of the calling thread.  No entry point allocates
+ * device memory: solvers take a caller-owned workspace whose size the *_workspace query returns.
+ *
+ * Lattices are full boxes, particle id p = (z * ny + y) * nx + x; vectors are (systems, N, dim) interleaved,
+ * directed coefficients (batch, N, NN) with NN = 18 (3D) or 8 (2D) in the reference slot order; packed
+ * coefficients are (batch, NN/2, N) forward-slot planes (valid for symmetric parameters).  A batch of systems is
+ * (nb, nl) = (specimens, load cases); every operand carries element strides (sb, sl) for the two batch indices,
+ * 0 meaning broadcast, which is the reference's (B,1,N,18) against (1|B,L,N,3) convention
+ * (coarse3d/torch_lpm3d.py:212-227).
+ */
+#ifndef LPM_H_
+#define LPM_H_
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LPM_OK = 0,
+  LPM_E_BADARG = 1,      /* bad shape, dtype or pointer */
+  LPM_E_HIP = 2,         /* HIP runtime error */
+  LPM_E_NOT_SPD = 3,     /* preconditioner block or curvature not positive */
+  LPM_E_NONFINITE = 4,   /* NaN / Inf met in the iteration */
+  LPM_E_UNSUPPORTED = 5  /* valid request this build does not implement */
+};
+enum { LPM_F32 = 0, LPM_F64 = 1 };
+enum { LPM_COEF_DIRECTED = 0, LPM_COEF_PACKED = 1 };
+
+typedef struct {
+  int32_t dim;        /* 2 or 3 */
+  int32_t nx, ny, nz; /* nz = 1 in 2D */
+  double h;           /* lattice spacing */
+} lpm_grid;
+
+typedef struct {
+  int64_t sb, sl; /* element strides of the (b, l) batch indices; 0 = broadcast */
+} lpm_bcast;
+
+typedef struct {
+  int32_t precond; /* 0 = none (plain CG), 1 = particle block Jacobi */
+  int32_t maxit;   /* iteration cap per system (inner iterations in refined mode) */
+  int32_t check;   /* host polls the convergence flags every `check` iterations */
+  int32_t graph;   /* 1 = replay the `check` iterations as a captured hipGraph */
+  double tol;      /* stop a system at ||r|| <= tol ||b|| */
+  double inner_tol;/* refined mode: relative tolerance of every FP32 inner solve */
+  int32_t max_outer;/* refined mode: cap on FP64 refinement steps */
+  int32_t pad_;
+} lpm_cg_opts;
+
+const char* lpm_last_error(void);
+int lpm_version(void);
+
+/* out (nb, nl, N, dim) = K u.  Replaces SCT.apply (coarse3d/torch_lpm3d.py:81-83), lin3d.k_apply
+ * (coarse3d/lin3d.py:126-149) and the 2D _kmv (timing/t2d/solver2d.py:269-305). */
+int lpm_apply(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* u,
+              lpm_bcast bu, const void* kn, const void* tv, lpm_bcast bc, void* out, void* stream);
+
+/* Directed (nbatch, N, NN) -> packed (nbatch, NN/2, N) forward planes, converting dtype_in -> dtype_out.
+ * asym (device, nbatch uint32, zeroed by the caller) counts bonds whose two directed values differ by more than
+ * rtol.  Replaces the symmetric duplication of the reference's Scatter (coarse3d/train3d.py:77-97). */
+int lpm_pack_coef(const lpm_grid* g, int32_t nbatch, int32_t dtype_in, int32_t dtype_out, const void* kn,
+                  const void* tv, void* kn_fwd, void* tv_fwd, double rtol, uint32_t* asym, void* stream);
+
+/* Block-Jacobi preconditioner out (nb, nl, N, 6|3) symmetric (xx yy zz yz xz xy | xx yy xy), inverse of the
+ * free-DOF diagonal block, zero rows/cols on fixed DOFs; raw = 1 writes the diagonal blocks D (nb, nl, N, dim, dim)
+ * instead.  mask (per DOF, 1 = free) may be null.  Replaces SCT.block_diag / precond (coarse3d/torch_lpm3d.py:94-106),
+ * lin3d.k_block_diag / k_block_jacobi (coarse3d/lin3d.py:239-301) and _diag_inv (timing/t2d/solver2d.py:308-346). */
+int lpm_block_jacobi(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype_c, int32_t dtype_out, int32_t layout,
+                     const void* kn, const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm, void* out,
+                     int32_t raw, void* stream);
+
+/* Per-particle outputs from u (any of the output pointers may be null): ku (nb,nl,N,dim), stress (nb,nl,N,6|3),
+ * energy (nb,nl,N), dl / force (nb,nl,N,NN).  work: nb*nl*N*4 values of dtype.  Replaces SCT.forces / stress /
+ * energy (coarse3d/torch_lpm3d.py:76-92), lin3d.k_bond_forces / k_stress / k_energy (coarse3d/lin3d.py:152-236),
+ * Lattice2D.bond_forces / stress / energy_per_particle (coarse2d/lin2d.py:121-163). */
+int lpm_post(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* u, lpm_bcast bu,
+             const void* kn, const void* tv, lpm_bcast bc, void* work, void* ku, void* stress, void* energy, void* dl,
+             void* force, void* stream);
+
+/* Adjoint bond-parameter gradient: gkn, gtv (ncb, N, NN) = scale * d/d(Kn, Tv) sum_s lam_s . K u_s, with systems
+ * s = cb * spc + j (j < spc) contiguous in u and lam (ncb*spc, N, dim).  Coefficients are only read for the shell
+ * sums' layout check (kn / tv directed or packed, batch ncb).  work: 2 * ncb*spc*N*4 values.  Replaces the
+ * autograd part of LinSolve.backward (coarse3d/torch_lpm3d.py:165-175). */
+int lpm_bond_grad(const lpm_grid* g, int32_t ncb, int32_t spc, int32_t dtype, int32_t layout, const void* kn,
+                  const void* tv, const void* u, const void* lam, double scale, void* work, void* gkn, void* gtv,
+                  void* stream);
+
+/* Batched (P)CG: K_ff x_f = b_f for nb*nl systems, b and x (nb*nl, N, dim) contiguous, x0 = x when use_x0 else 0.
+ * minv (symmetric blocks, lpm_block_jacobi layout) is required when opts->precond = 1.  iters / relres / conv are
+ * host arrays of nb*nl entries (conv = 1 when ||r|| <= tol ||b||).  Replaces SCT.pcg (coarse3d/torch_lpm3d.py:109-140),
+ * lin3d._pcg (coarse3d/lin3d.py:464-518), cgplain.cg3 (timing/t3d/cgplain.py:11-55) and cg2plain.cg_mf
+ * (timing/t2d/cg2plain.py:11-49). */
+size_t lpm_pcg_workspace(const lpm_grid* g, int32_t nsys, int32_t dtype);
+int lpm_pcg_solve(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* kn,
+                  const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm, const void* minv, lpm_bcast bminv,
+                  const void* b, void* x, int32_t use_x0, const lpm_cg_opts* opts, void* work, int32_t* iters,
+                  double* relres, int32_t* conv, void* stream);
+
+/* Mixed precision: FP32 (P)CG inner solves inside an FP64 iterative-refinement loop; the true residual
+ * r = mask * (b - K x) and the update x += d are FP64.  kn64/tv64 (layout64) define the FP64 operator, kn32/tv32
+ * (layout32) the FP32 one; minv32 is the FP32 preconditioner.  relres is the true FP64 relative residual, iters the
+ * total inner iterations, outer the refinement steps.  Same problem as lpm_pcg_solve. */
+size_t lpm_refined_workspace(const lpm_grid* g, int32_t nsys);
+int lpm_pcg_refined(const lpm_grid* g, int32_t nb, int32_t nl, int32_t layout64, const void* kn64, const void* tv64,
+                    lpm_bcast bc64, int32_t layout32, const void* kn32, const void* tv32, lpm_bcast bc32,
+                    const uint8_t* mask, lpm_bcast bm, const void* minv32, lpm_bcast bminv, const void* b64,
+                    void* x64, int32_t use_x0, const lpm_cg_opts* opts, void* work, int32_t* iters, double* relres,
+                    int32_t* outer, int32_t* conv, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LPM_H_ */

@@ -1,0 +1,259 @@
+// Vector kernels of the batched (P)CG loop.  Every system s = b * nl + l has its own scalars, so systems converge
+// and freeze independently (the reference freezes converged systems of a batch, coarse3d/torch_lpm3d.py:123-133, and
+// tests ||r|| <= tol ||b|| every iteration in coarse3d/lin3d.py:491-507).  Per-block partial sums are reduced by a
+// finalize kernel in a fixed order, so a system's result does not depend on which other systems share the launch.
+#pragma once
+#include "lpm_common.hpp"
+
+namespace lpm {
+
+// per-system scalars (type T) and integer state
+enum CgScal : int { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_BN, S_TOL, S_REL, NSCAL };
+enum CgInt : int { I_IT = 0, I_ACTIVE, I_MAXIT, I_CONV, NINT };
+
+template <typename T>
+struct CgVec {
+  int64_t N;  // particles
+  int nl;
+  T* x;
+  T* r;
+  T* z;  // null for unpreconditioned CG (z = r)
+  T* p;
+  T* ap;
+  const T* minv;  // symmetric blocks (N, 6 | 3) per system, null = identity
+  Bcast bminv;
+  const uint8_t* mask;  // per DOF, null = all free
+  Bcast bm;
+  T* partial;  // (systems, gridDim.x, 3)
+  T* scal;     // (systems, NSCAL)
+  int* ist;    // (systems, NINT)
+};
+
+template <int D, typename T>
+__device__ __forceinline__ void apply_minv(const T* __restrict__ m, const T* r, T* z) {
+  if constexpr (D == 3) {
+    z[0] = m[0] * r[0] + m[5] * r[1] + m[4] * r[2];
+    z[1] = m[5] * r[0] + m[1] * r[1] + m[3] * r[2];
+    z[2] = m[4] * r[0] + m[3] * r[1] + m[2] * r[2];
+  } else {
+    z[0] = m[0] * r[0] + m[2] * r[1];
+    z[1] = m[2] * r[0] + m[1] * r[1];
+  }
+}
+
+// r = mask * b (or r given by the residual kernel when `r_ready`), z = M r, p = z; partials (r.z, r.r, b.b).
+template <int D, typename T>
+__global__ __launch_bounds__(256) void cg_init_kernel(CgVec<T> v, const T* __restrict__ bvec, int r_ready) {
+  __shared__ T red[4];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = blockIdx.y, b = blockIdx.z;
+  const int64_t s = (int64_t)b * v.nl + l;
+  T rz = T(0), rr = T(0), bb = T(0);
+  if (p < v.N) {
+    const int64_t o = (s * v.N + p) * D;
+    T r[3], z[3];
+    for (int a = 0; a < D; ++a) {
+      const T m = v.mask ? (v.mask[v.bm.at(b, l) + D * p + a] ? T(1) : T(0)) : T(1);
+      const T bv = m * bvec[o + a];
+      bb += bv * bv;
+      r[a] = r_ready ? v.r[o + a] : bv;
+    }
+    if (v.minv) {
+      constexpr int NS = D == 3 ? 6 : 3;
+      apply_minv<D, T>(v.minv + v.bminv.at(b, l) + NS * p, r, z);
+    } else {
+      for (int a = 0; a < D; ++a) z[a] = r[a];
+    }
+    for (int a = 0; a < D; ++a) {
+      if (!r_ready) v.r[o + a] = r[a];
+      if (v.z) v.z[o + a] = z[a];
+      v.p[o + a] = z[a];
+      if (!r_ready) v.x[o + a] = T(0);
+      rz += r[a] * z[a];
+      rr += r[a] * r[a];
+    }
+  }
+  const T s0 = block_sum(rz, red);
+  const T s1 = block_sum(rr, red);
+  const T s2 = block_sum(bb, red);
+  if (threadIdx.x == 0) {
+    T* pp = v.partial + (s * gridDim.x + blockIdx.x) * 3;
+    pp[0] = s0;
+    pp[1] = s1;
+    pp[2] = s2;
+  }
+}
+
+// sum of column c of the (nblk, stride) partials of one system, fixed order
+template <typename T>
+__device__ __forceinline__ T sum_partials(const T* __restrict__ pp, int nblk, int stride, int c, T* red) {
+  T acc = T(0);
+  for (int j = threadIdx.x; j < nblk; j += blockDim.x) acc += pp[(int64_t)j * stride + c];
+  return block_sum(acc, red);
+}
+
+// after cg_init: rz, rr, bn; a system with b = 0 is done at iteration 0 (coarse3d/lin3d.py:479-480)
+template <typename T>
+__global__ __launch_bounds__(256) void cg_fin_init_kernel(T* partial, int nblk, T* scal, int* ist) {
+  __shared__ T red[4];
+  const int s = blockIdx.x;
+  const T* pp = partial + (int64_t)s * nblk * 3;
+  const T rz = sum_partials(pp, nblk, 3, 0, red);
+  const T rr = sum_partials(pp, nblk, 3, 1, red);
+  const T bb = sum_partials(pp, nblk, 3, 2, red);
+  if (threadIdx.x == 0) {
+    T* sc = scal + (int64_t)s * NSCAL;
+    int* is = ist + (int64_t)s * NINT;
+    const T bn = sqrt(bb);
+    sc[S_RZ] = rz;
+    sc[S_RR] = rr;
+    sc[S_BN] = bn;
+    sc[S_ALPHA] = T(0);
+    sc[S_BETA] = T(0);
+    is[I_IT] = 0;
+    const bool done = !(bn > T(0)) || sqrt(rr) <= sc[S_TOL] * bn;
+    sc[S_REL] = bn > T(0) ? sqrt(rr) / bn : T(0);
+    is[I_ACTIVE] = done ? 0 : 1;
+    is[I_CONV] = done ? 1 : 0;
+  }
+}
+
+// alpha = rz / pAp from the partials of the masked K p kernel (nblk per system)
+template <typename T>
+__global__ __launch_bounds__(256) void cg_fin_alpha_kernel(const T* partial, int nblk, T* scal, const int* ist) {
+  __shared__ T red[4];
+  const int s = blockIdx.x;
+  if (!ist[(int64_t)s * NINT + I_ACTIVE]) return;
+  const T pap = sum_partials(partial + (int64_t)s * nblk, nblk, 1, 0, red);
+  if (threadIdx.x == 0) {
+    T* sc = scal + (int64_t)s * NSCAL;
+    sc[S_PAP] = pap;
+    sc[S_ALPHA] = pap != T(0) ? sc[S_RZ] / pap : T(0);
+  }
+}
+
+// x += alpha p, r -= alpha Ap, z = M r; partials (r.z, r.r)
+template <int D, typename T>
+__global__ __launch_bounds__(256) void cg_update_kernel(CgVec<T> v) {
+  __shared__ T red[4];
+  const int l = blockIdx.y, b = blockIdx.z;
+  const int64_t s = (int64_t)b * v.nl + l;
+  if (!v.ist[s * NINT + I_ACTIVE]) return;
+  const T alpha = v.scal[s * NSCAL + S_ALPHA];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  T rz = T(0), rr = T(0);
+  if (p < v.N) {
+    const int64_t o = (s * v.N + p) * D;
+    T r[3], z[3];
+    for (int a = 0; a < D; ++a) {
+      v.x[o + a] += alpha * v.p[o + a];
+      r[a] = v.r[o + a] - alpha * v.ap[o + a];
+      v.r[o + a] = r[a];
+    }
+    if (v.minv) {
+      constexpr int NS = D == 3 ? 6 : 3;
+      apply_minv<D, T>(v.minv + v.bminv.at(b, l) + NS * p, r, z);
+      for (int a = 0; a < D; ++a) v.z[o + a] = z[a];
+    } else {
+      for (int a = 0; a < D; ++a) z[a] = r[a];
+    }
+    for (int a = 0; a < D; ++a) {
+      rz += r[a] * z[a];
+      rr += r[a] * r[a];
+    }
+  }
+  const T s0 = block_sum(rz, red);
+  const T s1 = block_sum(rr, red);
+  if (threadIdx.x == 0) {
+    T* pp = v.partial + (s * gridDim.x + blockIdx.x) * 3;
+    pp[0] = s0;
+    pp[1] = s1;
+  }
+}
+
+// convergence test and beta
+template <typename T>
+__global__ __launch_bounds__(256) void cg_fin_beta_kernel(const T* partial, int nblk, T* scal, int* ist) {
+  __shared__ T red[4];
+  const int s = blockIdx.x;
+  int* is = ist + (int64_t)s * NINT;
+  if (!is[I_ACTIVE]) return;
+  const T* pp = partial + (int64_t)s * nblk * 3;
+  const T rzn = sum_partials(pp, nblk, 3, 0, red);
+  const T rr = sum_partials(pp, nblk, 3, 1, red);
+  if (threadIdx.x == 0) {
+    T* sc = scal + (int64_t)s * NSCAL;
+    const int it = is[I_IT] + 1;
+    is[I_IT] = it;
+    sc[S_RR] = rr;
+    const T rn = sqrt(rr);
+    sc[S_REL] = rn / sc[S_BN];
+    if (rn <= sc[S_TOL] * sc[S_BN]) {
+      is[I_ACTIVE] = 0;
+      is[I_CONV] = 1;
+    } else if (it >= is[I_MAXIT]) {
+      is[I_ACTIVE] = 0;
+    } else {
+      sc[S_BETA] = sc[S_RZ] != T(0) ? rzn / sc[S_RZ] : T(0);
+      sc[S_RZ] = rzn;
+    }
+  }
+}
+
+// p = z + beta p
+template <int D, typename T>
+__global__ __launch_bounds__(256) void cg_pupdate_kernel(CgVec<T> v) {
+  const int l = blockIdx.y, b = blockIdx.z;
+  const int64_t s = (int64_t)b * v.nl + l;
+  if (!v.ist[s * NINT + I_ACTIVE]) return;
+  const T beta = v.scal[s * NSCAL + S_BETA];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= v.N) return;
+  const int64_t o = (s * v.N + p) * D;
+  const T* z = v.z ? v.z : v.r;
+  for (int a = 0; a < D; ++a) v.p[o + a] = z[o + a] + beta * v.p[o + a];
+}
+
+// ------------------------------------------------------------------------------------------ refinement helpers
+// lo = (TL) hi elementwise (n values)
+template <typename TH, typename TL>
+__global__ __launch_bounds__(256) void cast_kernel(const TH* __restrict__ hi, TL* __restrict__ lo, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lo[i] = (TL)hi[i];
+}
+// hi += (TH) lo
+template <typename TH, typename TL>
+__global__ __launch_bounds__(256) void add_cast_kernel(TH* __restrict__ hi, const TL* __restrict__ lo, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) hi[i] += (TH)lo[i];
+}
+// sum of squares per system of partials written by the residual kernel (nblk per system) -> out[s]
+template <typename T>
+__global__ __launch_bounds__(256) void sumsq_fin_kernel(const T* partial, int nblk, double* out) {
+  __shared__ T red[4];
+  const int s = blockIdx.x;
+  const T v = sum_partials(partial + (int64_t)s * nblk, nblk, 1, 0, red);
+  if (threadIdx.x == 0) out[s] = (double)v;
+}
+// ||mask * b||^2 per system (grid (ceil(N/256), nl, nb)); partial layout (systems, gridDim.x)
+template <int D, typename T>
+__global__ __launch_bounds__(256) void masked_norm2_kernel(const T* __restrict__ bvec, int64_t N, int nl,
+                                                           const uint8_t* __restrict__ mask, Bcast bm,
+                                                           T* __restrict__ partial) {
+  __shared__ T red[4];
+  const int l = blockIdx.y, b = blockIdx.z;
+  const int64_t s = (int64_t)b * nl + l;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  T acc = T(0);
+  if (p < N) {
+    for (int a = 0; a < D; ++a) {
+      const T m = mask ? (mask[bm.at(b, l) + D * p + a] ? T(1) : T(0)) : T(1);
+      const T v = m * bvec[(s * N + p) * D + a];
+      acc += v * v;
+    }
+  }
+  const T t = block_sum(acc, red);
+  if (threadIdx.x == 0) partial[s * gridDim.x + blockIdx.x] = t;
+}
+
+}  // namespace lpm

@@ -1,0 +1,722 @@
+// C ABI of liblpm.so (declarations and reference citations in include/lpm.h).
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lpm.h"
+#include "cg.hpp"
+#include "ku2d.hpp"
+#include "ku3d.hpp"
+#include "post.hpp"
+
+using namespace lpm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(LPM_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK()                                                                           \
+  do {                                                                                        \
+    hipError_t e_ = hipGetLastError();                                                        \
+    if (e_ != hipSuccess) return fail(LPM_E_HIP, "kernel launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+inline Bcast bc_of(lpm_bcast b) { return Bcast{b.sb, b.sl}; }
+inline int64_t nsites(const lpm_grid* g) { return (int64_t)g->nx * g->ny * (g->dim == 3 ? g->nz : 1); }
+inline Box box_of(const lpm_grid* g) { return Box{g->nx, g->ny, g->dim == 3 ? g->nz : 1}; }
+
+int check_grid(const lpm_grid* g) {
+  if (!g) return fail(LPM_E_BADARG, "grid is null");
+  if (g->dim != 2 && g->dim != 3) return fail(LPM_E_BADARG, "dim must be 2 or 3 (got %d)", g->dim);
+  if (g->nx < 1 || g->ny < 1 || (g->dim == 3 && g->nz < 1))
+    return fail(LPM_E_BADARG, "bad lattice %d x %d x %d", g->nx, g->ny, g->nz);
+  if (nsites(g) >= (int64_t)1 << 31) return fail(LPM_E_BADARG, "lattice too large");
+  return LPM_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// ------------------------------------------------------------------------------------------------ K u launch
+struct KuGeom {
+  dim3 grid, block;
+  int ntx, nty, nzc, zc, tx, ty;
+};
+
+// tile shapes: 3D float 64x4 or 32x8, double 32x4 or 16x8; 2D float 32x8, double 32x4
+template <int DIM, typename T>
+KuGeom ku_geom(const lpm_grid* g, int nb, int nl) {
+  KuGeom k{};
+  if constexpr (DIM == 3) {
+    const bool wide = g->nx >= 48;
+    if (sizeof(T) == 4) {
+      k.tx = wide ? 64 : 32;
+      k.ty = wide ? 4 : 8;
+    } else {
+      k.tx = wide ? 32 : 16;
+      k.ty = wide ? 4 : 8;
+    }
+    k.ntx = (g->nx + k.tx - 1) / k.tx;
+    k.nty = (g->ny + k.ty - 1) / k.ty;
+    const int64_t xy = (int64_t)k.ntx * k.nty * nb * nl;
+    const int64_t want = std::max<int64_t>(1, (4096 + xy - 1) / xy);
+    k.zc = std::max<int>(4, (int)((g->nz + want - 1) / want));
+    k.nzc = (g->nz + k.zc - 1) / k.zc;
+    k.grid = dim3(k.ntx * k.nty * k.nzc, nl, nb);
+  } else {
+    k.tx = 32;
+    k.ty = sizeof(T) == 4 ? 8 : 4;
+    k.ntx = (g->nx + k.tx - 1) / k.tx;
+    k.nty = (g->ny + k.ty - 1) / k.ty;
+    k.nzc = 1;
+    k.zc = 1;
+    k.grid = dim3(k.ntx * k.nty, nl, nb);
+  }
+  k.block = dim3(k.tx * k.ty);
+  return k;
+}
+
+template <typename T, int MODE, int EPI>
+int launch_ku3(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const Ku3Args<T, T>& a0, hipStream_t st) {
+  Ku3Args<T, T> a = a0;
+  a.nx = g->nx;
+  a.ny = g->ny;
+  a.nz = g->nz;
+  a.nl = nl;
+  a.zc = kg.zc;
+  a.ntx = kg.ntx;
+  a.nty = kg.nty;
+  a.nzc = kg.nzc;
+  if constexpr (sizeof(T) == 4) {
+    if (kg.tx == 64)
+      hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 64, 4, EPI>), kg.grid, kg.block, 0, st, a);
+    else
+      hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 32, 8, EPI>), kg.grid, kg.block, 0, st, a);
+  } else {
+    if (kg.tx == 32)
+      hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 32, 4, EPI>), kg.grid, kg.block, 0, st, a);
+    else
+      hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 16, 8, EPI>), kg.grid, kg.block, 0, st, a);
+  }
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+template <typename T, int MODE, int EPI>
+int launch_ku2(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const Ku2Args<T, T>& a0, hipStream_t st) {
+  Ku2Args<T, T> a = a0;
+  a.nx = g->nx;
+  a.ny = g->ny;
+  a.nl = nl;
+  a.ntx = kg.ntx;
+  a.nty = kg.nty;
+  if constexpr (sizeof(T) == 4)
+    hipLaunchKernelGGL((ku2d_kernel<T, T, MODE, 32, 8, EPI>), kg.grid, kg.block, 0, st, a);
+  else
+    hipLaunchKernelGGL((ku2d_kernel<T, T, MODE, 32, 4, EPI>), kg.grid, kg.block, 0, st, a);
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+// one K u launch with epilogue EPI for either dimension / layout
+struct KuCall {
+  const void* u;
+  Bcast bu;
+  const void* kn;
+  const void* tv;
+  Bcast bc;
+  const uint8_t* mask;
+  Bcast bm;
+  const void* rhs;
+  Bcast brhs;
+  void* out;
+  Bcast bo;
+  void* partial;
+  const int* active;  // per-system flags with stride NINT (CG state), or null
+};
+
+template <int DIM, typename T, int EPI>
+int ku_call(const lpm_grid* g, int nb, int nl, int layout, const KuGeom& kg, const KuCall& c, hipStream_t st) {
+  if constexpr (DIM == 3) {
+    Ku3Args<T, T> a{};
+    a.u = (const T*)c.u;
+    a.bu = c.bu;
+    a.kn = (const T*)c.kn;
+    a.tv = (const T*)c.tv;
+    a.bc = c.bc;
+    a.mask = c.mask;
+    a.bm = c.bm;
+    a.rhs = (const T*)c.rhs;
+    a.brhs = c.brhs;
+    a.out = (T*)c.out;
+    a.bo = c.bo;
+    a.partial = (T*)c.partial;
+    a.active = c.active;
+    a.act_stride = NINT;
+    return layout == LPM_COEF_PACKED ? launch_ku3<T, COEF_PACKED, EPI>(g, nb, nl, kg, a, st)
+                                     : launch_ku3<T, COEF_DIRECTED, EPI>(g, nb, nl, kg, a, st);
+  } else {
+    Ku2Args<T, T> a{};
+    a.u = (const T*)c.u;
+    a.bu = c.bu;
+    a.kn = (const T*)c.kn;
+    a.tv = (const T*)c.tv;
+    a.bc = c.bc;
+    a.mask = c.mask;
+    a.bm = c.bm;
+    a.rhs = (const T*)c.rhs;
+    a.brhs = c.brhs;
+    a.out = (T*)c.out;
+    a.bo = c.bo;
+    a.partial = (T*)c.partial;
+    a.active = c.active;
+    a.act_stride = NINT;
+    return layout == LPM_COEF_PACKED ? launch_ku2<T, COEF_PACKED, EPI>(g, nb, nl, kg, a, st)
+                                     : launch_ku2<T, COEF_DIRECTED, EPI>(g, nb, nl, kg, a, st);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ PCG
+template <typename T>
+__global__ void cg_setup_kernel(T* scal, int* ist, int nsys, T tol, int maxit) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  scal[(int64_t)s * NSCAL + S_TOL] = tol;
+  ist[(int64_t)s * NINT + I_MAXIT] = maxit;
+  ist[(int64_t)s * NINT + I_ACTIVE] = 0;
+  ist[(int64_t)s * NINT + I_IT] = 0;
+}
+// per-system tolerance from a device array (refinement inner solves)
+template <typename T>
+__global__ void cg_settol_kernel(T* scal, const double* tol, int nsys) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsys) scal[(int64_t)s * NSCAL + S_TOL] = (T)tol[s];
+}
+
+struct PcgLayout {
+  size_t r, z, p, ap, partial, scal, ist, total;
+  int maxblk;
+};
+
+template <int DIM, typename T>
+PcgLayout pcg_layout(const lpm_grid* g, int nsys) {
+  constexpr int D = DIM;
+  const int64_t N = nsites(g);
+  const KuGeom kg = ku_geom<DIM, T>(g, 1, nsys);
+  const int vblk = (int)((N + 255) / 256);
+  PcgLayout L{};
+  L.maxblk = std::max<int>((int)kg.grid.x, vblk);
+  size_t o = 0;
+  const size_t vec = align256(sizeof(T) * (size_t)nsys * N * D);
+  L.r = o;
+  o += vec;
+  L.z = o;
+  o += vec;
+  L.p = o;
+  o += vec;
+  L.ap = o;
+  o += vec;
+  L.partial = o;
+  o += align256(sizeof(T) * (size_t)nsys * L.maxblk * 3);
+  L.scal = o;
+  o += align256(sizeof(T) * (size_t)nsys * NSCAL);
+  L.ist = o;
+  o += align256(sizeof(int) * (size_t)nsys * NINT);
+  L.total = o;
+  return L;
+}
+
+// Core batched (P)CG on device buffers.  When tol_dev is non-null it gives a per-system tolerance.
+template <int DIM, typename T>
+int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, const void* tv, Bcast bc,
+             const uint8_t* mask, Bcast bm, const T* minv, Bcast bminv, const T* b, T* x, int use_x0,
+             const lpm_cg_opts* o, char* work, const double* tol_dev, int32_t* iters, double* relres, int32_t* conv,
+             hipStream_t st) {
+  constexpr int D = DIM;
+  const int nsys = nb * nl;
+  const int64_t N = nsites(g);
+  const PcgLayout L = pcg_layout<DIM, T>(g, nsys);
+  const KuGeom kg = ku_geom<DIM, T>(g, nb, nl);
+  T* r = (T*)(work + L.r);
+  T* z = o->precond ? (T*)(work + L.z) : nullptr;
+  T* p = (T*)(work + L.p);
+  T* ap = (T*)(work + L.ap);
+  T* partial = (T*)(work + L.partial);
+  T* scal = (T*)(work + L.scal);
+  int* ist = (int*)(work + L.ist);
+  const Bcast bvec{(int64_t)nl * N * D, N * D};
+  const dim3 vgrid((unsigned)((N + 255) / 256), nl, nb);
+  const int vblk = (int)vgrid.x;
+
+  CgVec<T> v{};
+  v.N = N;
+  v.nl = nl;
+  v.x = x;
+  v.r = r;
+  v.z = z;
+  v.p = p;
+  v.ap = ap;
+  v.minv = o->precond ? minv : nullptr;
+  v.bminv = bminv;
+  v.mask = mask;
+  v.bm = bm;
+  v.partial = partial;
+  v.scal = scal;
+  v.ist = ist;
+
+  hipLaunchKernelGGL((cg_setup_kernel<T>), dim3((nsys + 255) / 256), dim3(256), 0, st, scal, ist, nsys, (T)o->tol,
+                     o->maxit);
+  if (tol_dev) hipLaunchKernelGGL((cg_settol_kernel<T>), dim3((nsys + 255) / 256), dim3(256), 0, st, scal, tol_dev, nsys);
+  LAUNCHCHK();
+  if (use_x0) {
+    KuCall c{x, bvec, kn, tv, bc, mask, bm, b, bvec, r, bvec, partial, nullptr};
+    int rc = ku_call<DIM, T, EPI_RESID>(g, nb, nl, layout, kg, c, st);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL((cg_init_kernel<D, T>), vgrid, dim3(256), 0, st, v, b, use_x0);
+  hipLaunchKernelGGL((cg_fin_init_kernel<T>), dim3(nsys), dim3(256), 0, st, partial, vblk, scal, ist);
+  LAUNCHCHK();
+
+  const int check = std::max(1, o->check);
+  std::vector<int> host_ist((size_t)nsys * NINT);
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  auto body = [&]() -> int {
+    KuCall c{p, bvec, kn, tv, bc, mask, bm, nullptr, bvec, ap, bvec, partial, ist + I_ACTIVE};
+    int rc = ku_call<DIM, T, EPI_MASKDOT>(g, nb, nl, layout, kg, c, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL((cg_fin_alpha_kernel<T>), dim3(nsys), dim3(256), 0, st, partial, (int)kg.grid.x, scal, ist);
+    hipLaunchKernelGGL((cg_update_kernel<D, T>), vgrid, dim3(256), 0, st, v);
+    hipLaunchKernelGGL((cg_fin_beta_kernel<T>), dim3(nsys), dim3(256), 0, st, partial, vblk, scal, ist);
+    hipLaunchKernelGGL((cg_pupdate_kernel<D, T>), vgrid, dim3(256), 0, st, v);
+    LAUNCHCHK();
+    return LPM_OK;
+  };
+  int done_it = 0;
+  int rc = LPM_OK;
+  while (done_it < o->maxit) {
+    const int nrep = std::min(check, o->maxit - done_it);
+    if (o->graph && nrep == check) {
+      if (!gexec) {
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < check && rc == LPM_OK; ++i) rc = body();
+        hipError_t ec = hipStreamEndCapture(st, &graph);
+        if (rc) return rc;
+        if (ec != hipSuccess) return fail(LPM_E_HIP, "graph capture: %s", hipGetErrorString(ec));
+        HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+      }
+      HIPCHK(hipGraphLaunch(gexec, st));
+    } else {
+      for (int i = 0; i < nrep && rc == LPM_OK; ++i) rc = body();
+      if (rc) return rc;
+    }
+    done_it += nrep;
+    HIPCHK(hipMemcpyAsync(host_ist.data(), ist, sizeof(int) * nsys * NINT, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    bool any = false;
+    for (int s = 0; s < nsys; ++s) any |= host_ist[(size_t)s * NINT + I_ACTIVE] != 0;
+    if (!any) break;
+  }
+  if (gexec) {
+    (void)hipGraphExecDestroy(gexec);
+    (void)hipGraphDestroy(graph);
+  }
+  std::vector<T> host_scal((size_t)nsys * NSCAL);
+  HIPCHK(hipMemcpyAsync(host_scal.data(), scal, sizeof(T) * nsys * NSCAL, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(host_ist.data(), ist, sizeof(int) * nsys * NINT, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  bool bad = false;
+  for (int s = 0; s < nsys; ++s) {
+    if (iters) iters[s] = host_ist[(size_t)s * NINT + I_IT];
+    if (relres) relres[s] = (double)host_scal[(size_t)s * NSCAL + S_REL];
+    if (conv) conv[s] = host_ist[(size_t)s * NINT + I_CONV];
+    if (!std::isfinite((double)host_scal[(size_t)s * NSCAL + S_REL])) bad = true;
+  }
+  if (bad) return fail(LPM_E_NONFINITE, "non-finite residual in the (P)CG iteration");
+  return LPM_OK;
+}
+
+}  // namespace
+
+// ================================================================================================ C ABI
+extern "C" {
+
+const char* lpm_last_error(void) { return g_err.c_str(); }
+int lpm_version(void) { return 1; }
+
+int lpm_apply(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* u, lpm_bcast bu,
+              const void* kn, const void* tv, lpm_bcast bc, void* out, void* stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!u || !kn || !tv || !out || nb < 1 || nl < 1) return fail(LPM_E_BADARG, "lpm_apply: null pointer or empty batch");
+  if (nb > 65535 || nl > 65535) return fail(LPM_E_BADARG, "lpm_apply: batch too large");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t N = nsites(g);
+  const int D = g->dim;
+  KuCall c{u, bc_of(bu), kn, tv, bc_of(bc), nullptr, {0, 0}, nullptr, {0, 0}, out,
+           {(int64_t)nl * N * D, N * D}, nullptr, nullptr};
+  if (g->dim == 3) {
+    if (dtype == LPM_F32) return ku_call<3, float, EPI_PLAIN>(g, nb, nl, layout, ku_geom<3, float>(g, nb, nl), c, st);
+    if (dtype == LPM_F64) return ku_call<3, double, EPI_PLAIN>(g, nb, nl, layout, ku_geom<3, double>(g, nb, nl), c, st);
+  } else {
+    if (dtype == LPM_F32) return ku_call<2, float, EPI_PLAIN>(g, nb, nl, layout, ku_geom<2, float>(g, nb, nl), c, st);
+    if (dtype == LPM_F64) return ku_call<2, double, EPI_PLAIN>(g, nb, nl, layout, ku_geom<2, double>(g, nb, nl), c, st);
+  }
+  return fail(LPM_E_BADARG, "lpm_apply: bad dtype %d", dtype);
+}
+
+int lpm_pack_coef(const lpm_grid* g, int32_t nbatch, int32_t dtype_in, int32_t dtype_out, const void* kn,
+                  const void* tv, void* kn_fwd, void* tv_fwd, double rtol, uint32_t* asym, void* stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!kn || !tv || !kn_fwd || !tv_fwd || !asym || nbatch < 1 || nbatch > 65535)
+    return fail(LPM_E_BADARG, "lpm_pack_coef: null pointer or bad batch");
+  hipStream_t st = (hipStream_t)stream;
+  const Box box = box_of(g);
+  const dim3 grid((unsigned)((nsites(g) + 255) / 256), nbatch);
+#define PACK(DIM, CI, CO)                                                                                      \
+  hipLaunchKernelGGL((pack_coef_kernel<DIM, CI, CO>), grid, dim3(256), 0, st, box, (const CI*)kn, (const CI*)tv, \
+                     (CO*)kn_fwd, (CO*)tv_fwd, rtol, asym)
+  const int key = (g->dim == 3 ? 100 : 0) + dtype_in * 10 + dtype_out;
+  switch (key) {
+    case 100: PACK(3, float, float); break;
+    case 101: PACK(3, float, double); break;
+    case 110: PACK(3, double, float); break;
+    case 111: PACK(3, double, double); break;
+    case 0: PACK(2, float, float); break;
+    case 1: PACK(2, float, double); break;
+    case 10: PACK(2, double, float); break;
+    case 11: PACK(2, double, double); break;
+    default: return fail(LPM_E_BADARG, "lpm_pack_coef: bad dtype");
+  }
+#undef PACK
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+int lpm_block_jacobi(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype_c, int32_t dtype_out, int32_t layout,
+                     const void* kn, const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm, void* out,
+                     int32_t raw, void* stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!kn || !tv || !out || nb < 1 || nl < 1 || nb > 65535 || nl > 65535)
+    return fail(LPM_E_BADARG, "lpm_block_jacobi: null pointer or bad batch");
+  hipStream_t st = (hipStream_t)stream;
+  const Box box = box_of(g);
+  const int64_t N = nsites(g);
+  const int D = g->dim;
+  const int64_t per = raw ? (int64_t)D * D : (D == 3 ? 6 : 3);
+  const Bcast bo{(int64_t)nl * N * per, N * per};
+  const dim3 grid((unsigned)((N + 255) / 256), nl, nb);
+#define BJ(DIM, C, T, MODE)                                                                                        \
+  hipLaunchKernelGGL((block_jacobi_kernel<DIM, T, C, MODE>), grid, dim3(256), 0, st, box, nl, (const C*)kn,       \
+                     (const C*)tv, bc_of(bc), mask, bc_of(bm), (T*)out, bo, raw)
+  const int key = (g->dim == 3 ? 1000 : 0) + dtype_c * 100 + dtype_out * 10 + (layout == LPM_COEF_PACKED);
+  switch (key) {
+    case 1000: BJ(3, float, float, COEF_DIRECTED); break;
+    case 1001: BJ(3, float, float, COEF_PACKED); break;
+    case 1010: BJ(3, float, double, COEF_DIRECTED); break;
+    case 1011: BJ(3, float, double, COEF_PACKED); break;
+    case 1100: BJ(3, double, float, COEF_DIRECTED); break;
+    case 1101: BJ(3, double, float, COEF_PACKED); break;
+    case 1110: BJ(3, double, double, COEF_DIRECTED); break;
+    case 1111: BJ(3, double, double, COEF_PACKED); break;
+    case 0: BJ(2, float, float, COEF_DIRECTED); break;
+    case 1: BJ(2, float, float, COEF_PACKED); break;
+    case 10: BJ(2, float, double, COEF_DIRECTED); break;
+    case 11: BJ(2, float, double, COEF_PACKED); break;
+    case 100: BJ(2, double, float, COEF_DIRECTED); break;
+    case 101: BJ(2, double, float, COEF_PACKED); break;
+    case 110: BJ(2, double, double, COEF_DIRECTED); break;
+    case 111: BJ(2, double, double, COEF_PACKED); break;
+    default: return fail(LPM_E_BADARG, "lpm_block_jacobi: bad dtype");
+  }
+#undef BJ
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+int lpm_post(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* u, lpm_bcast bu,
+             const void* kn, const void* tv, lpm_bcast bc, void* work, void* ku, void* stress, void* energy, void* dl,
+             void* force, void* stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!u || !kn || !tv || !work || nb < 1 || nl < 1 || nb > 65535 || nl > 65535)
+    return fail(LPM_E_BADARG, "lpm_post: null pointer or bad batch");
+  hipStream_t st = (hipStream_t)stream;
+  const Box box = box_of(g);
+  const int64_t N = nsites(g);
+  const dim3 grid((unsigned)((N + 255) / 256), nl, nb);
+  PostOut o{ku, stress, energy, dl, force};
+#define POST(DIM, T, MODE)                                                                                        \
+  do {                                                                                                            \
+    hipLaunchKernelGGL((shell_sums_kernel<DIM, T, T, MODE>), grid, dim3(256), 0, st, box, nl, (const T*)u,       \
+                       bc_of(bu), (const T*)tv, bc_of(bc), (T*)work);                                            \
+    hipLaunchKernelGGL((site_post_kernel<DIM, T, T, MODE>), grid, dim3(256), 0, st, box, nl, g->h, (const T*)u,  \
+                       bc_of(bu), (const T*)kn, (const T*)tv, bc_of(bc), (const T*)work, o);                     \
+  } while (0)
+  const int key = (g->dim == 3 ? 100 : 0) + dtype * 10 + (layout == LPM_COEF_PACKED);
+  switch (key) {
+    case 100: POST(3, float, COEF_DIRECTED); break;
+    case 101: POST(3, float, COEF_PACKED); break;
+    case 110: POST(3, double, COEF_DIRECTED); break;
+    case 111: POST(3, double, COEF_PACKED); break;
+    case 0: POST(2, float, COEF_DIRECTED); break;
+    case 1: POST(2, float, COEF_PACKED); break;
+    case 10: POST(2, double, COEF_DIRECTED); break;
+    case 11: POST(2, double, COEF_PACKED); break;
+    default: return fail(LPM_E_BADARG, "lpm_post: bad dtype");
+  }
+#undef POST
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+int lpm_bond_grad(const lpm_grid* g, int32_t ncb, int32_t spc, int32_t dtype, int32_t layout, const void* kn,
+                  const void* tv, const void* u, const void* lam, double scale, void* work, void* gkn, void* gtv,
+                  void* stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!kn || !tv || !u || !lam || !work || !gkn || !gtv || ncb < 1 || spc < 1 || ncb > 65535 || spc > 65535)
+    return fail(LPM_E_BADARG, "lpm_bond_grad: null pointer or bad batch");
+  hipStream_t st = (hipStream_t)stream;
+  const Box box = box_of(g);
+  const int64_t N = nsites(g);
+  const int D = g->dim, NN = D == 3 ? 18 : 8;
+  const int64_t cstride = layout == LPM_COEF_PACKED ? N * NN / 2 : N * NN;
+  const Bcast bu{(int64_t)spc * N * D, N * D};
+  const Bcast bc{cstride, 0};
+  const dim3 grid((unsigned)((N + 255) / 256), spc, ncb);
+  const dim3 ggrid((unsigned)((N + 255) / 256), ncb);
+  const size_t half = (size_t)ncb * spc * N * 4;
+#define GRAD(DIM, T, MODE)                                                                                          \
+  do {                                                                                                              \
+    T* wu = (T*)work;                                                                                               \
+    T* wl = wu + half;                                                                                              \
+    hipLaunchKernelGGL((shell_sums_kernel<DIM, T, T, MODE>), grid, dim3(256), 0, st, box, spc, (const T*)u, bu,    \
+                       (const T*)tv, bc, wu);                                                                       \
+    hipLaunchKernelGGL((shell_sums_kernel<DIM, T, T, MODE>), grid, dim3(256), 0, st, box, spc, (const T*)lam, bu,  \
+                       (const T*)tv, bc, wl);                                                                       \
+    hipLaunchKernelGGL((bond_grad_kernel<DIM, T>), ggrid, dim3(256), 0, st, box, spc, (const T*)u, (const T*)lam,  \
+                       (const T*)wu, (const T*)wl, (T)scale, (T*)gkn, (T*)gtv);                                    \
+  } while (0)
+  const int key = (g->dim == 3 ? 100 : 0) + dtype * 10 + (layout == LPM_COEF_PACKED);
+  switch (key) {
+    case 100: GRAD(3, float, COEF_DIRECTED); break;
+    case 101: GRAD(3, float, COEF_PACKED); break;
+    case 110: GRAD(3, double, COEF_DIRECTED); break;
+    case 111: GRAD(3, double, COEF_PACKED); break;
+    case 0: GRAD(2, float, COEF_DIRECTED); break;
+    case 1: GRAD(2, float, COEF_PACKED); break;
+    case 10: GRAD(2, double, COEF_DIRECTED); break;
+    case 11: GRAD(2, double, COEF_PACKED); break;
+    default: return fail(LPM_E_BADARG, "lpm_bond_grad: bad dtype");
+  }
+#undef GRAD
+  LAUNCHCHK();
+  (void)NN;
+  return LPM_OK;
+}
+
+size_t lpm_pcg_workspace(const lpm_grid* g, int32_t nsys, int32_t dtype) {
+  if (check_grid(g) || nsys < 1) return 0;
+  if (g->dim == 3) return dtype == LPM_F32 ? pcg_layout<3, float>(g, nsys).total : pcg_layout<3, double>(g, nsys).total;
+  return dtype == LPM_F32 ? pcg_layout<2, float>(g, nsys).total : pcg_layout<2, double>(g, nsys).total;
+}
+
+static int check_solve_args(const lpm_grid* g, int nb, int nl, const lpm_cg_opts* o, const void* minv) {
+  if (int rc = check_grid(g)) return rc;
+  if (!o) return fail(LPM_E_BADARG, "options are null");
+  if (nb < 1 || nl < 1 || nb > 65535 || nl > 65535) return fail(LPM_E_BADARG, "bad batch %d x %d", nb, nl);
+  if (o->precond && !minv) return fail(LPM_E_BADARG, "block-Jacobi PCG needs the preconditioner blocks");
+  if (o->maxit < 0) return fail(LPM_E_BADARG, "maxit < 0");
+  if (!(o->tol >= 0.0)) return fail(LPM_E_BADARG, "tol must be >= 0");
+  return LPM_OK;
+}
+
+int lpm_pcg_solve(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* kn,
+                  const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm, const void* minv, lpm_bcast bminv,
+                  const void* b, void* x, int32_t use_x0, const lpm_cg_opts* opts, void* work, int32_t* iters,
+                  double* relres, int32_t* conv, void* stream) {
+  if (int rc = check_solve_args(g, nb, nl, opts, minv)) return rc;
+  if (!kn || !tv || !b || !x || !work) return fail(LPM_E_BADARG, "lpm_pcg_solve: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const int key = (g->dim == 3 ? 10 : 0) + dtype;
+  switch (key) {
+    case 10:
+      return pcg_core<3, float>(g, nb, nl, layout, kn, tv, bc_of(bc), mask, bc_of(bm), (const float*)minv,
+                                bc_of(bminv), (const float*)b, (float*)x, use_x0, opts, (char*)work, nullptr, iters,
+                                relres, conv, st);
+    case 11:
+      return pcg_core<3, double>(g, nb, nl, layout, kn, tv, bc_of(bc), mask, bc_of(bm), (const double*)minv,
+                                 bc_of(bminv), (const double*)b, (double*)x, use_x0, opts, (char*)work, nullptr, iters,
+                                 relres, conv, st);
+    case 0:
+      return pcg_core<2, float>(g, nb, nl, layout, kn, tv, bc_of(bc), mask, bc_of(bm), (const float*)minv,
+                                bc_of(bminv), (const float*)b, (float*)x, use_x0, opts, (char*)work, nullptr, iters,
+                                relres, conv, st);
+    case 1:
+      return pcg_core<2, double>(g, nb, nl, layout, kn, tv, bc_of(bc), mask, bc_of(bm), (const double*)minv,
+                                 bc_of(bminv), (const double*)b, (double*)x, use_x0, opts, (char*)work, nullptr, iters,
+                                 relres, conv, st);
+    default:
+      return fail(LPM_E_BADARG, "lpm_pcg_solve: bad dtype %d", dtype);
+  }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------ refinement
+namespace {
+
+__global__ void zero_kernel(double* x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+struct RefLayout {
+  size_t inner, r64, r32, d32, part, bn2, rr2, tin, total;
+};
+
+template <int DIM>
+RefLayout ref_layout(const lpm_grid* g, int nsys) {
+  const int64_t N = nsites(g);
+  RefLayout L{};
+  size_t o = 0;
+  L.inner = o;
+  o += align256(pcg_layout<DIM, float>(g, nsys).total);
+  L.r64 = o;
+  o += align256(sizeof(double) * (size_t)nsys * N * DIM);
+  L.r32 = o;
+  o += align256(sizeof(float) * (size_t)nsys * N * DIM);
+  L.d32 = o;
+  o += align256(sizeof(float) * (size_t)nsys * N * DIM);
+  const KuGeom kg = ku_geom<DIM, double>(g, 1, nsys);
+  const int64_t nblk = std::max<int64_t>(kg.grid.x, (N + 255) / 256);
+  L.part = o;
+  o += align256(sizeof(double) * (size_t)nsys * nblk);
+  L.bn2 = o;
+  o += align256(sizeof(double) * nsys);
+  L.rr2 = o;
+  o += align256(sizeof(double) * nsys);
+  L.tin = o;
+  o += align256(sizeof(double) * nsys);
+  L.total = o;
+  return L;
+}
+
+template <int DIM>
+int refined_core(const lpm_grid* g, int nb, int nl, int layout64, const void* kn64, const void* tv64, Bcast bc64,
+                 int layout32, const void* kn32, const void* tv32, Bcast bc32, const uint8_t* mask, Bcast bm,
+                 const float* minv32, Bcast bminv, const double* b64, double* x64, int use_x0, const lpm_cg_opts* o,
+                 char* work, int32_t* iters, double* relres, int32_t* outer, int32_t* conv, hipStream_t st) {
+  constexpr int D = DIM;
+  const int nsys = nb * nl;
+  const int64_t N = nsites(g);
+  const int64_t nvec = (int64_t)nsys * N * D;
+  const RefLayout L = ref_layout<DIM>(g, nsys);
+  double* r64 = (double*)(work + L.r64);
+  float* r32 = (float*)(work + L.r32);
+  float* d32 = (float*)(work + L.d32);
+  double* part = (double*)(work + L.part);
+  double* bn2 = (double*)(work + L.bn2);
+  double* rr2 = (double*)(work + L.rr2);
+  double* tin = (double*)(work + L.tin);
+  const Bcast bvec{(int64_t)nl * N * D, N * D};
+  const dim3 vgrid((unsigned)((N + 255) / 256), nl, nb);
+  const unsigned egrid = (unsigned)((nvec + 255) / 256);
+  const KuGeom kg = ku_geom<DIM, double>(g, nb, nl);
+
+  hipLaunchKernelGGL((masked_norm2_kernel<D, double>), vgrid, dim3(256), 0, st, b64, N, nl, mask, bm, part);
+  hipLaunchKernelGGL((sumsq_fin_kernel<double>), dim3(nsys), dim3(256), 0, st, part, (int)vgrid.x, bn2);
+  if (!use_x0) hipLaunchKernelGGL(zero_kernel, dim3(egrid), dim3(256), 0, st, x64, nvec);
+  LAUNCHCHK();
+  std::vector<double> hb(nsys), hr(nsys), ht(nsys);
+  std::vector<int32_t> it_tot(nsys, 0), it_in(nsys), cv_in(nsys);
+  std::vector<double> rel_in(nsys);
+  std::vector<int32_t> cv(nsys, 0);
+  int nout = 0;
+  lpm_cg_opts oi = *o;
+  oi.tol = 1.0;
+  for (;;) {
+    KuCall c{x64, bvec, kn64, tv64, bc64, mask, bm, b64, bvec, r64, bvec, part, nullptr};
+    if (int rc = ku_call<DIM, double, EPI_RESID>(g, nb, nl, layout64, kg, c, st)) return rc;
+    hipLaunchKernelGGL((sumsq_fin_kernel<double>), dim3(nsys), dim3(256), 0, st, part, (int)kg.grid.x, rr2);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(hb.data(), bn2, sizeof(double) * nsys, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hr.data(), rr2, sizeof(double) * nsys, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    bool any = false;
+    for (int s = 0; s < nsys; ++s) {
+      const double bn = std::sqrt(hb[s]), rn = std::sqrt(hr[s]);
+      if (!std::isfinite(rn)) return fail(LPM_E_NONFINITE, "non-finite FP64 residual");
+      relres[s] = bn > 0 ? rn / bn : 0.0;
+      cv[s] = (bn == 0.0 || rn <= o->tol * bn) ? 1 : 0;
+      // inner tolerance: enough to reach the target in this step, never looser than 0.5
+      ht[s] = cv[s] ? 2.0 : std::min(0.5, std::max(o->inner_tol, 0.5 * o->tol / relres[s]));
+      any |= !cv[s];
+    }
+    if (!any || nout >= o->max_outer) break;
+    HIPCHK(hipMemcpyAsync(tin, ht.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((cast_kernel<double, float>), dim3(egrid), dim3(256), 0, st, r64, r32, nvec);
+    LAUNCHCHK();
+    char* iw = work + L.inner;
+    if (int rc = pcg_core<DIM, float>(g, nb, nl, layout32, kn32, tv32, bc32, mask, bm, minv32, bminv, r32, d32, 0,
+                                      &oi, iw, tin, it_in.data(), rel_in.data(), cv_in.data(), st))
+      return rc;
+    hipLaunchKernelGGL((add_cast_kernel<double, float>), dim3(egrid), dim3(256), 0, st, x64, d32, nvec);
+    LAUNCHCHK();
+    for (int s = 0; s < nsys; ++s) it_tot[s] += it_in[s];
+    ++nout;
+  }
+  for (int s = 0; s < nsys; ++s) {
+    if (iters) iters[s] = it_tot[s];
+    if (conv) conv[s] = cv[s];
+  }
+  if (outer) *outer = nout;
+  return LPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t lpm_refined_workspace(const lpm_grid* g, int32_t nsys) {
+  if (check_grid(g) || nsys < 1) return 0;
+  return g->dim == 3 ? ref_layout<3>(g, nsys).total : ref_layout<2>(g, nsys).total;
+}
+
+int lpm_pcg_refined(const lpm_grid* g, int32_t nb, int32_t nl, int32_t layout64, const void* kn64, const void* tv64,
+                    lpm_bcast bc64, int32_t layout32, const void* kn32, const void* tv32, lpm_bcast bc32,
+                    const uint8_t* mask, lpm_bcast bm, const void* minv32, lpm_bcast bminv, const void* b64,
+                    void* x64, int32_t use_x0, const lpm_cg_opts* opts, void* work, int32_t* iters, double* relres,
+                    int32_t* outer, int32_t* conv, void* stream) {
+  if (int rc = check_solve_args(g, nb, nl, opts, minv32)) return rc;
+  if (!kn64 || !tv64 || !kn32 || !tv32 || !b64 || !x64 || !work || !relres)
+    return fail(LPM_E_BADARG, "lpm_pcg_refined: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (g->dim == 3)
+    return refined_core<3>(g, nb, nl, layout64, kn64, tv64, bc_of(bc64), layout32, kn32, tv32, bc_of(bc32), mask,
+                           bc_of(bm), (const float*)minv32, bc_of(bminv), (const double*)b64, (double*)x64, use_x0,
+                           opts, (char*)work, iters, relres, outer, conv, st);
+  return refined_core<2>(g, nb, nl, layout64, kn64, tv64, bc_of(bc64), layout32, kn32, tv32, bc_of(bc32), mask,
+                         bc_of(bm), (const float*)minv32, bc_of(bminv), (const double*)b64, (double*)x64, use_x0, opts,
+                         (char*)work, iters, relres, outer, conv, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,358 @@
+// Per-particle kernels outside the CG loop: shell sums, computeStress / particle energy / bond forces, adjoint
+// bond-parameter gradient, block-Jacobi preconditioner, coefficient packing.  They run once per solve, so they use
+// a plain two-pass layout (shell sums written to a workspace, then read back for the neighbours).
+#pragma once
+#include "lpm_common.hpp"
+
+namespace lpm {
+
+template <int DIM>
+struct Lat;
+template <>
+struct Lat<3> {
+  static constexpr int D = 3, NN = 18;
+  __host__ __device__ static constexpr int off(int k, int a) { return off3(k, a); }
+  __host__ __device__ static constexpr int shell(int k) { return shell3(k); }
+  template <typename T>
+  __host__ __device__ static constexpr T nd(int k, int a) { return nd3<T>(k, a); }
+};
+template <>
+struct Lat<2> {
+  static constexpr int D = 2, NN = 8;
+  __host__ __device__ static constexpr int off(int k, int a) { return a < 2 ? off2(k, a) : 0; }
+  __host__ __device__ static constexpr int shell(int k) { return shell2(k); }
+  template <typename T>
+  __host__ __device__ static constexpr T nd(int k, int a) { return a < 2 ? nd2<T>(k, a) : T(0); }
+};
+
+struct Box {
+  int nx, ny, nz;  // nz = 1 in 2D
+  __device__ __forceinline__ void coords(int64_t p, int& x, int& y, int& z) const {
+    x = (int)(p % nx);
+    const int64_t r = p / nx;
+    y = (int)(r % ny);
+    z = (int)(r / ny);
+  }
+  // neighbour of (x, y, z) in slot k, or -1 outside the box
+  template <int DIM>
+  __device__ __forceinline__ int64_t nbr(int x, int y, int z, int k) const {
+    const int qx = x + Lat<DIM>::off(k, 0), qy = y + Lat<DIM>::off(k, 1), qz = z + Lat<DIM>::off(k, 2);
+    if (qx < 0 || qx >= nx || qy < 0 || qy >= ny || qz < 0 || qz >= nz) return -1;
+    return ((int64_t)qz * ny + qy) * nx + qx;
+  }
+};
+
+// ----------------------------------------------------------------------------------------- shell sums
+// ws (systems, N, 4) = S0, S1, TS0, TS1.  Grid (ceil(N/256), nl, nb).
+template <int DIM, typename T, typename C, int MODE>
+__global__ __launch_bounds__(256) void shell_sums_kernel(Box box, int nl, const T* __restrict__ u, Bcast bu,
+                                                         const C* __restrict__ tv, Bcast bc, T* __restrict__ ws) {
+  using L = Lat<DIM>;
+  const int64_t N = (int64_t)box.nx * box.ny * box.nz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int l = blockIdx.y, b = blockIdx.z;
+  const T* uu = u + bu.at(b, l);
+  const C* tt = tv + bc.at(b, l);
+  int x, y, z;
+  box.coords(p, x, y, z);
+  T s[2] = {T(0), T(0)}, ts[2] = {T(0), T(0)};
+#pragma unroll
+  for (int k = 0; k < L::NN; ++k) {
+    const int64_t q = box.nbr<DIM>(x, y, z, k);
+    if (q < 0) continue;
+    T dl = T(0);
+#pragma unroll
+    for (int a = 0; a < L::D; ++a)
+      if (L::off(k, a)) dl += L::template nd<T>(k, a) * (uu[L::D * p + a] - uu[L::D * q + a]);
+    const T t = (T)coef_at<C, MODE, L::NN>(tt, N, p, q, k);
+    s[L::shell(k)] += dl;
+    ts[L::shell(k)] += t * dl;
+  }
+  T* w = ws + (((int64_t)b * nl + l) * N + p) * 4;
+  w[0] = s[0];
+  w[1] = s[1];
+  w[2] = ts[0];
+  w[3] = ts[1];
+}
+
+// ----------------------------------------------------------------------------------------- site outputs
+// From u and the shell sums: any of K u (N, D), stress (N, 3|6), energy (N,), dL and F per directed slot (N, NN).
+// computeStress (coarse3d/lin3d.py:170-210, coarse2d/lin2d.py:143-154): sigma_i = sum_k of_ik / V * l0_k * F_ik
+// n_k n_k^T, of = 1/2 when the particle has all bonds or the opposite bond exists, else 1; Voigt order xx yy zz yz
+// xz xy (3D) or xx yy xy (2D).  Energy (coarse3d/lin3d.py:213-236): U_i = 1/2 sum Kn dL^2 + 1/2 sum_c S^c TS^c.
+struct PostOut {
+  void* ku;
+  void* stress;
+  void* energy;
+  void* dl;
+  void* force;
+};
+
+template <int DIM, typename T, typename C, int MODE>
+__global__ __launch_bounds__(256) void site_post_kernel(Box box, int nl, double h, const T* __restrict__ u, Bcast bu,
+                                                        const C* __restrict__ kn, const C* __restrict__ tv, Bcast bc,
+                                                        const T* __restrict__ ws, PostOut o) {
+  using L = Lat<DIM>;
+  constexpr int NN = L::NN, D = L::D, NS = DIM == 3 ? 6 : 3;
+  const int64_t N = (int64_t)box.nx * box.ny * box.nz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int l = blockIdx.y, b = blockIdx.z;
+  const int64_t sys = (int64_t)b * nl + l;
+  const T* uu = u + bu.at(b, l);
+  const C* kk = kn + bc.at(b, l);
+  const C* tt = tv + bc.at(b, l);
+  const T* w = ws + sys * N * 4;
+  int x, y, z;
+  box.coords(p, x, y, z);
+  int nvalid = 0;
+#pragma unroll
+  for (int k = 0; k < NN; ++k) nvalid += box.nbr<DIM>(x, y, z, k) >= 0;
+  const bool full = nvalid == NN;
+  const T si[2] = {w[4 * p + 0], w[4 * p + 1]}, ti[2] = {w[4 * p + 2], w[4 * p + 3]};
+  const T vol = (T)(DIM == 3 ? h * h * h : h * h);
+  T ku[3] = {T(0), T(0), T(0)}, st[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  T en = T(0);
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    const int64_t q = box.nbr<DIM>(x, y, z, k);
+    T dl = T(0), f = T(0);
+    if (q >= 0) {
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        if (L::off(k, a)) dl += L::template nd<T>(k, a) * (uu[D * p + a] - uu[D * q + a]);
+      const int c = L::shell(k);
+      const T kv = (T)coef_at<C, MODE, NN>(kk, N, p, q, k);
+      const T tv_ = (T)coef_at<C, MODE, NN>(tt, N, p, q, k);
+      f = T(2) * kv * dl + T(0.5) * (ti[c] + w[4 * q + 2 + c]) + T(0.5) * tv_ * (si[c] + w[4 * q + c]);
+      en += T(0.5) * kv * dl * dl;
+#pragma unroll
+      for (int a = 0; a < D; ++a) ku[a] += f * L::template nd<T>(k, a);
+      const bool opp_ok = box.nbr<DIM>(x, y, z, k ^ 1) >= 0;
+      const T len = (T)(k < (DIM == 3 ? 6 : 4) ? h : h * 1.4142135623730950488);
+      const T wgt = ((full || opp_ok) ? T(0.5) : T(1)) / vol * len * f;
+      const T n0 = L::template nd<T>(k, 0), n1 = L::template nd<T>(k, 1), n2 = L::template nd<T>(k, 2);
+      if constexpr (DIM == 3) {
+        st[0] += wgt * n0 * n0;
+        st[1] += wgt * n1 * n1;
+        st[2] += wgt * n2 * n2;
+        st[3] += wgt * n1 * n2;
+        st[4] += wgt * n0 * n2;
+        st[5] += wgt * n0 * n1;
+      } else {
+        st[0] += wgt * n0 * n0;
+        st[1] += wgt * n1 * n1;
+        st[2] += wgt * n0 * n1;
+      }
+    }
+    if (o.dl) ((T*)o.dl)[(sys * N + p) * NN + k] = dl;
+    if (o.force) ((T*)o.force)[(sys * N + p) * NN + k] = f;
+  }
+  en += T(0.5) * (si[0] * ti[0] + si[1] * ti[1]);
+  if (o.ku)
+    for (int a = 0; a < D; ++a) ((T*)o.ku)[(sys * N + p) * D + a] = ku[a];
+  if (o.stress)
+    for (int a = 0; a < NS; ++a) ((T*)o.stress)[(sys * N + p) * NS + a] = st[a];
+  if (o.energy) ((T*)o.energy)[sys * N + p] = en;
+}
+
+// ----------------------------------------------------------------------------------------- adjoint gradient
+// For E = sum_s lam_s . (K(Kn, Tv) u_s), the derivative per directed slot (force form of the operator):
+//   dE/dKn_ik = 2 dL_ik(u) mu_ik,   dE/dTv_ik = 1/2 mu_ik (S_i^c(u) + S_q^c(u)) + 1/2 dL_ik(u) S_i^c(lam),
+// with mu_ik = n_k . lam_i.  This is what autograd of lam . apply(u; Kn, Tv) returns in the reference adjoint
+// (coarse3d/torch_lpm3d.py:165-175).  Systems s = cb * spc + j (j < spc) share coefficient batch cb; the sum over
+// them is taken in registers in a fixed order.  wsu / wsl hold the shell sums of u and lam.  Output directed
+// (ncb, N, NN); grid (ceil(N/256), ncb).
+template <int DIM, typename T>
+__global__ __launch_bounds__(256) void bond_grad_kernel(Box box, int spc, const T* __restrict__ u,
+                                                        const T* __restrict__ lam, const T* __restrict__ wsu,
+                                                        const T* __restrict__ wsl, T scale, T* __restrict__ gkn,
+                                                        T* __restrict__ gtv) {
+  using L = Lat<DIM>;
+  constexpr int NN = L::NN, D = L::D;
+  const int64_t N = (int64_t)box.nx * box.ny * box.nz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int cb = blockIdx.y;
+  int x, y, z;
+  box.coords(p, x, y, z);
+  T gk[NN], gt[NN];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) gk[k] = gt[k] = T(0);
+  for (int j = 0; j < spc; ++j) {
+    const int64_t s = (int64_t)cb * spc + j;
+    const T* uu = u + s * N * D;
+    const T* ll = lam + s * N * D;
+    const T* wu = wsu + s * N * 4;
+    const T* wl = wsl + s * N * 4;
+    T lv[3] = {T(0), T(0), T(0)}, uv[3] = {T(0), T(0), T(0)};
+    for (int a = 0; a < D; ++a) {
+      lv[a] = ll[D * p + a];
+      uv[a] = uu[D * p + a];
+    }
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+      const int64_t q = box.nbr<DIM>(x, y, z, k);
+      if (q < 0) continue;
+      T dl = T(0), mu = T(0);
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        if (L::off(k, a)) {
+          dl += L::template nd<T>(k, a) * (uv[a] - uu[D * q + a]);
+          mu += L::template nd<T>(k, a) * lv[a];
+        }
+      }
+      const int c = L::shell(k);
+      gk[k] += T(2) * dl * mu;
+      gt[k] += T(0.5) * mu * (wu[4 * p + c] + wu[4 * q + c]) + T(0.5) * dl * wl[4 * p + c];
+    }
+  }
+  T* ok = gkn + ((int64_t)cb * N + p) * NN;
+  T* ot = gtv + ((int64_t)cb * N + p) * NN;
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    ok[k] = scale * gk[k];
+    ot[k] = scale * gt[k];
+  }
+}
+
+// ----------------------------------------------------------------------------------------- block Jacobi
+// D_p = sum_a (Kn_pa + Kn_{q,a^1} + Tv_{q,a^1}) n_a n_a^T + sum_c 1/2 (T_pc s_pc^T + s_pc T_pc^T),
+// T_pc = sum_{a in c} Tv_pa n_a, s_pc = sum_{a in c} n_a over existing bonds (coarse3d/lin3d.py:239-269).
+// The inverse of the free-DOF sub-block is stored (rows / columns of fixed DOFs zero), so a particle with all
+// DOFs fixed gets a zero block as in coarse3d/torch_lpm3d.py:104-106.  Output symmetric: 3D (xx yy zz yz xz xy),
+// 2D (xx yy xy), in the solver precision.  `raw` skips the inverse (returns D itself, full (N, D, D) layout).
+template <int DIM, typename T, typename C, int MODE>
+__global__ __launch_bounds__(256) void block_jacobi_kernel(Box box, int nl, const C* __restrict__ kn,
+                                                           const C* __restrict__ tv, Bcast bc,
+                                                           const uint8_t* __restrict__ mask, Bcast bm,
+                                                           T* __restrict__ out, Bcast bo, int raw) {
+  using L = Lat<DIM>;
+  constexpr int NN = L::NN, D = L::D;
+  const int64_t N = (int64_t)box.nx * box.ny * box.nz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int l = blockIdx.y, b = blockIdx.z;
+  const C* kk = kn + bc.at(b, l);
+  const C* tt = tv + bc.at(b, l);
+  int x, y, z;
+  box.coords(p, x, y, z);
+  double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double Tc[2][3] = {{0, 0, 0}, {0, 0, 0}}, sc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    const int64_t q = box.nbr<DIM>(x, y, z, k);
+    if (q < 0) continue;
+    const int c = L::shell(k);
+    // coefficient of the reverse slot at the neighbour
+    double kq, tq;
+    if constexpr (MODE == COEF_DIRECTED) {
+      kq = (double)kk[q * NN + (k ^ 1)];
+      tq = (double)tt[q * NN + (k ^ 1)];
+    } else {
+      kq = (double)coef_at<C, MODE, NN>(kk, N, p, q, k);  // symmetric: same bond
+      tq = (double)coef_at<C, MODE, NN>(tt, N, p, q, k);
+    }
+    const double kp = (double)coef_at<C, MODE, NN>(kk, N, p, q, k);
+    const double tp = (double)coef_at<C, MODE, NN>(tt, N, p, q, k);
+    const double cf = kp + kq + tq;
+    double n[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) n[a] = L::template nd<double>(k, a);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      Tc[c][a] += tp * n[a];
+      sc[c][a] += n[a];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) M[a][e] += cf * n[a] * n[e];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) M[a][e] += 0.5 * (Tc[c][a] * sc[c][e] + sc[c][a] * Tc[c][e]);
+  T* o = out + bo.at(b, l);
+  if (raw) {
+    for (int a = 0; a < D; ++a)
+      for (int e = 0; e < D; ++e) o[p * D * D + a * D + e] = (T)M[a][e];
+    return;
+  }
+  bool fr[3] = {true, true, DIM == 3};
+  if (mask) {
+    const uint8_t* mk = mask + bm.at(b, l) + D * p;
+    for (int a = 0; a < D; ++a) fr[a] = mk[a] != 0;
+  }
+  // identity on fixed DOFs (and the unused third DOF in 2D), invert, zero fixed rows / columns
+  for (int a = 0; a < 3; ++a)
+    if (!fr[a])
+      for (int e = 0; e < 3; ++e) M[a][e] = M[e][a] = (a == e) ? 1.0 : 0.0;
+  const double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+  const double c01 = -(M[1][0] * M[2][2] - M[1][2] * M[2][0]);
+  const double c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+  const double det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
+  double I[3][3];
+  I[0][0] = c00 / det;
+  I[1][0] = c01 / det;
+  I[2][0] = c02 / det;
+  I[0][1] = -(M[0][1] * M[2][2] - M[0][2] * M[2][1]) / det;
+  I[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) / det;
+  I[2][1] = -(M[0][0] * M[2][1] - M[0][1] * M[2][0]) / det;
+  I[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) / det;
+  I[1][2] = -(M[0][0] * M[1][2] - M[0][2] * M[1][0]) / det;
+  I[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) / det;
+  for (int a = 0; a < 3; ++a)
+    for (int e = 0; e < 3; ++e)
+      if (!fr[a] || !fr[e]) I[a][e] = 0.0;
+  if constexpr (DIM == 3) {
+    T* w = o + p * 6;
+    w[0] = (T)I[0][0];
+    w[1] = (T)I[1][1];
+    w[2] = (T)I[2][2];
+    w[3] = (T)(0.5 * (I[1][2] + I[2][1]));
+    w[4] = (T)(0.5 * (I[0][2] + I[2][0]));
+    w[5] = (T)(0.5 * (I[0][1] + I[1][0]));
+  } else {
+    T* w = o + p * 3;
+    w[0] = (T)I[0][0];
+    w[1] = (T)I[1][1];
+    w[2] = (T)(0.5 * (I[0][1] + I[1][0]));
+  }
+}
+
+// ----------------------------------------------------------------------------------------- packing
+// Directed (nbatch, N, NN) -> forward planes (nbatch, NN/2, N) in the solver precision; asym[batch] counts the
+// bonds whose two directed slots differ by more than rtol (relative), so the caller can keep the directed layout.
+template <int DIM, typename CI, typename CO>
+__global__ __launch_bounds__(256) void pack_coef_kernel(Box box, const CI* __restrict__ kn, const CI* __restrict__ tv,
+                                                        CO* __restrict__ kp, CO* __restrict__ tp, double rtol,
+                                                        unsigned int* __restrict__ asym) {
+  using L = Lat<DIM>;
+  constexpr int NN = L::NN;
+  const int64_t N = (int64_t)box.nx * box.ny * box.nz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int b = blockIdx.y;
+  const CI* kk = kn + (int64_t)b * N * NN;
+  const CI* tt = tv + (int64_t)b * N * NN;
+  int x, y, z;
+  box.coords(p, x, y, z);
+  unsigned int bad = 0;
+#pragma unroll
+  for (int k = 0; k < NN; k += 2) {
+    const int64_t q = box.nbr<DIM>(x, y, z, k);
+    const CI a1 = kk[p * NN + k], t1 = tt[p * NN + k];
+    kp[((int64_t)b * (NN / 2) + k / 2) * N + p] = (CO)a1;
+    tp[((int64_t)b * (NN / 2) + k / 2) * N + p] = (CO)t1;
+    if (q >= 0) {
+      const CI a2 = kk[q * NN + (k ^ 1)], t2 = tt[q * NN + (k ^ 1)];
+      const double sa = fabs((double)a1) + fabs((double)a2), st = fabs((double)t1) + fabs((double)t2);
+      if (fabs((double)a1 - (double)a2) > rtol * sa || fabs((double)t1 - (double)t2) > rtol * st) ++bad;
+    }
+  }
+  if (bad) atomicAdd(&asym[b], bad);
+}
+
+}  // namespace lpm

@@ -1,0 +1,293 @@
+"""Torch-facing operators of the LPM solve path.  Each one hands device pointers to liblpm.so (HIP, gfx950).
+
+Shapes follow the reference: vectors (..., N, d), directed coefficients (..., N, NN) with NN = 18 (3D) or 8 (2D),
+at most two broadcast batch dimensions (B, L) (coarse3d/torch_lpm3d.py:212-227).  `dims` is (nx, ny[, nz]).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+import _lpmlib as L
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _dims(dims):
+    dims = tuple(int(d) for d in dims)
+    N = int(np.prod(dims))
+    return dims, N, len(dims), L.nn_of(len(dims))
+
+
+def _coef_inner(dim, N, layout):
+    NN = L.nn_of(dim)
+    return (N, NN) if layout == L.DIRECTED else (NN // 2, N)
+
+
+def apply(dims, h, u, Kn, Tv, layout=L.DIRECTED):
+    """K u for broadcast batches: u (..., N, d), Kn/Tv (..., N, NN) directed or (..., NN/2, N) packed."""
+    dims, N, d, NN = _dims(dims)
+    L.require_gpu(u, Kn, Tv)
+    dt = u.dtype
+    if Kn.dtype != dt or Tv.dtype != dt:
+        raise TypeError("u, Kn and Tv must share one dtype")
+    ci = _coef_inner(d, N, layout)
+    batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2])
+    nb, nl = L.batch2(batch)
+    ue, bu = L.bview(u, batch, (N, d))
+    ke, bc = L.bview(Kn, batch, ci)
+    te, bt = L.bview(Tv, batch, ci)
+    if (bc.sb, bc.sl) != (bt.sb, bt.sl):
+        raise ValueError("Kn and Tv must broadcast the same way")
+    out = torch.empty(tuple(batch) + (N, d), device=u.device, dtype=dt)
+    g = L.grid_of(dims, h)
+    L.check(L.lib().lpm_apply(ctypes.byref(g), nb, nl, L.dtype_code(dt), layout, _ptr(ue), bu, _ptr(ke), _ptr(te),
+                              bc, _ptr(out), L.stream_of(u)))
+    return out
+
+
+def post(dims, h, u, Kn, Tv, ku=False, stress=False, energy=False, dl=False, force=False, layout=L.DIRECTED):
+    """Per-particle outputs; returns a dict with the requested entries (batch = broadcast of u and Kn)."""
+    dims, N, d, NN = _dims(dims)
+    L.require_gpu(u, Kn, Tv)
+    dt = u.dtype
+    ci = _coef_inner(d, N, layout)
+    batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2])
+    nb, nl = L.batch2(batch)
+    ue, bu = L.bview(u, batch, (N, d))
+    ke, bc = L.bview(Kn, batch, ci)
+    te, _ = L.bview(Tv, batch, ci)
+    dev = u.device
+    B = tuple(batch)
+    work = torch.empty(B + (N, 4), device=dev, dtype=dt)
+    ns = 6 if d == 3 else 3
+    out = dict(ku=torch.empty(B + (N, d), device=dev, dtype=dt) if ku else None,
+               stress=torch.empty(B + (N, ns), device=dev, dtype=dt) if stress else None,
+               energy=torch.empty(B + (N,), device=dev, dtype=dt) if energy else None,
+               dl=torch.empty(B + (N, NN), device=dev, dtype=dt) if dl else None,
+               force=torch.empty(B + (N, NN), device=dev, dtype=dt) if force else None)
+    g = L.grid_of(dims, h)
+    L.check(L.lib().lpm_post(ctypes.byref(g), nb, nl, L.dtype_code(dt), layout, _ptr(ue), bu, _ptr(ke), _ptr(te), bc,
+                             _ptr(work), _ptr(out["ku"]), _ptr(out["stress"]), _ptr(out["energy"]), _ptr(out["dl"]),
+                             _ptr(out["force"]), L.stream_of(u)))
+    out["shell"] = work
+    return out
+
+
+def block_jacobi(dims, h, Kn, Tv, free=None, out_dtype=None, raw=False, batch=None, layout=L.DIRECTED):
+    """Block-Jacobi inverse blocks (batch, N, 6|3) symmetric, or the diagonal blocks (batch, N, d, d) if raw."""
+    dims, N, d, NN = _dims(dims)
+    L.require_gpu(Kn, Tv)
+    ci = _coef_inner(d, N, layout)
+    shapes = [Kn.shape[:-2]]
+    if free is not None:
+        f = torch.as_tensor(free)
+        shapes.append(f.shape[:-2] if (f.dim() >= 2 and tuple(f.shape[-2:]) == (N, d)) else f.shape[:-1])
+    if batch is not None:
+        shapes.append(tuple(batch))
+    bshape = torch.broadcast_shapes(*shapes)
+    nb, nl = L.batch2(bshape)
+    ke, bc = L.bview(Kn, bshape, ci)
+    te, _ = L.bview(Tv, bshape, ci)
+    me, bm = L.dof_mask(free, bshape, N, d, Kn.device)
+    odt = out_dtype or Kn.dtype
+    per = (d, d) if raw else ((6,) if d == 3 else (3,))
+    out = torch.empty(tuple(bshape) + (N,) + per, device=Kn.device, dtype=odt)
+    g = L.grid_of(dims, h)
+    L.check(L.lib().lpm_block_jacobi(ctypes.byref(g), nb, nl, L.dtype_code(Kn.dtype), L.dtype_code(odt), layout,
+                                     _ptr(ke), _ptr(te), bc, _ptr(me), bm, _ptr(out), 1 if raw else 0,
+                                     L.stream_of(Kn)))
+    return out
+
+
+def sym_from_full(M):
+    """(..., d, d) -> symmetric storage (..., 6 | 3) in the kernels' order (xx yy zz yz xz xy | xx yy xy)."""
+    if M.shape[-1] == 3:
+        return torch.stack([M[..., 0, 0], M[..., 1, 1], M[..., 2, 2], 0.5 * (M[..., 1, 2] + M[..., 2, 1]),
+                            0.5 * (M[..., 0, 2] + M[..., 2, 0]), 0.5 * (M[..., 0, 1] + M[..., 1, 0])], -1)
+    return torch.stack([M[..., 0, 0], M[..., 1, 1], 0.5 * (M[..., 0, 1] + M[..., 1, 0])], -1)
+
+
+def full_from_sym(S):
+    if S.shape[-1] == 6:
+        r0 = torch.stack([S[..., 0], S[..., 5], S[..., 4]], -1)
+        r1 = torch.stack([S[..., 5], S[..., 1], S[..., 3]], -1)
+        r2 = torch.stack([S[..., 4], S[..., 3], S[..., 2]], -1)
+        return torch.stack([r0, r1, r2], -2)
+    r0 = torch.stack([S[..., 0], S[..., 2]], -1)
+    r1 = torch.stack([S[..., 2], S[..., 1]], -1)
+    return torch.stack([r0, r1], -2)
+
+
+def pack(dims, Kn, Tv, out_dtype=None, rtol=1e-12):
+    """Directed (..., N, NN) -> packed forward planes (..., NN/2, N); returns (kp, tp, n_asymmetric_bonds)."""
+    dims, N, d, NN = _dims(dims)
+    L.require_gpu(Kn, Tv)
+    lead = Kn.shape[:-2]
+    nbat = int(np.prod(lead)) if len(lead) else 1
+    k = Kn.reshape(nbat, N, NN).contiguous()
+    t = Tv.reshape(nbat, N, NN).contiguous()
+    odt = out_dtype or Kn.dtype
+    kp = torch.empty((nbat, NN // 2, N), device=Kn.device, dtype=odt)
+    tp = torch.empty_like(kp)
+    asym = torch.zeros(nbat, device=Kn.device, dtype=torch.int32)
+    g = L.grid_of(dims, 1.0)
+    L.check(L.lib().lpm_pack_coef(ctypes.byref(g), nbat, L.dtype_code(Kn.dtype), L.dtype_code(odt), _ptr(k), _ptr(t),
+                                  _ptr(kp), _ptr(tp), float(rtol), _ptr(asym), L.stream_of(Kn)))
+    return kp.reshape(tuple(lead) + (NN // 2, N)), tp.reshape(tuple(lead) + (NN // 2, N)), asym
+
+
+def bond_grad(dims, h, Kn, Tv, u, lam, scale=1.0, layout=L.DIRECTED):
+    """scale * d/d(Kn, Tv) sum_s lam_s . K u_s per directed slot, reduced to the broadcast shape of Kn.
+    u, lam (..., N, d) with the full batch; Kn (..., N, NN)."""
+    dims, N, d, NN = _dims(dims)
+    L.require_gpu(Kn, Tv, u, lam)
+    dt = u.dtype
+    batch = torch.broadcast_shapes(u.shape[:-2], lam.shape[:-2], Kn.shape[:-2])
+    ci = _coef_inner(d, N, layout)
+    nb, nl = L.batch2(batch)
+    kb = tuple(Kn.shape[:-2])
+    kb2 = (1,) * (len(batch) - len(kb)) + kb
+    ue = u.expand(tuple(batch) + (N, d)).contiguous()
+    le = lam.expand(tuple(batch) + (N, d)).contiguous()
+    if len(batch) == 2 and kb2 == (nb, 1):
+        ncb, spc = nb, nl                       # coefficients per specimen, load cases summed in the kernel
+        kk = Kn.reshape(nb, *ci).contiguous()
+        tt = Tv.reshape(nb, *ci).contiguous()
+    elif all(s == 1 for s in kb2):
+        ncb, spc = 1, nb * nl
+        kk = Kn.reshape(1, *ci).contiguous()
+        tt = Tv.reshape(1, *ci).contiguous()
+    else:                                       # general broadcast: one coefficient set per system, summed below
+        ncb, spc = nb * nl, 1
+        kk = Kn.expand(tuple(batch) + ci).reshape(nb * nl, *ci).contiguous()
+        tt = Tv.expand(tuple(batch) + ci).reshape(nb * nl, *ci).contiguous()
+    work = torch.empty((2, nb * nl, N, 4), device=u.device, dtype=dt)
+    gk = torch.empty((ncb, N, NN), device=u.device, dtype=dt)
+    gt = torch.empty_like(gk)
+    g = L.grid_of(dims, h)
+    L.check(L.lib().lpm_bond_grad(ctypes.byref(g), ncb, spc, L.dtype_code(dt), layout, _ptr(kk), _ptr(tt), _ptr(ue),
+                                  _ptr(le), float(scale), _ptr(work), _ptr(gk), _ptr(gt), L.stream_of(u)))
+    if ncb == nb * nl and spc == 1 and not (len(batch) == 2 and kb2 == (nb, 1)) and not all(s == 1 for s in kb2):
+        gk = gk.reshape(tuple(batch) + (N, NN)).sum_to_size(kb2 + (N, NN))
+        gt = gt.reshape(tuple(batch) + (N, NN)).sum_to_size(kb2 + (N, NN))
+    return gk.reshape(kb + (N, NN)), gt.reshape(kb + (N, NN))
+
+
+class SolveInfo:
+    def __init__(self, iters, relres, conv, outer=None):
+        self.iters, self.relres, self.conv, self.outer = iters, relres, conv, outer
+
+    @property
+    def it(self):
+        return int(self.iters.max()) if self.iters.size else 0
+
+    @property
+    def relmax(self):
+        return float(self.relres.max()) if self.relres.size else 0.0
+
+
+def _opts(precond, tol, maxit, check, graph, inner_tol=1e-5, max_outer=20):
+    o = L.CgOpts()
+    o.precond = 1 if precond else 0
+    o.maxit = int(maxit)
+    o.check = int(check)
+    o.graph = 1 if graph else 0
+    o.tol = float(tol)
+    o.inner_tol = float(inner_tol)
+    o.max_outer = int(max_outer)
+    return o
+
+
+def pcg(dims, h, b, Kn, Tv, minv=None, free=None, tol=1e-10, maxit=20000, x0=None, check=10, graph=False,
+        layout=L.DIRECTED):
+    """Batched (P)CG on K_ff x = b_f.  b (B, L, N, d) or broadcastable; minv symmetric blocks (…, N, 6|3) or None
+    (plain CG).  Returns x (batch, N, d) and SolveInfo (per-system iterations, relative residual, convergence)."""
+    dims, N, d, NN = _dims(dims)
+    L.require_gpu(b, Kn, Tv)
+    dt = b.dtype
+    ci = _coef_inner(d, N, layout)
+    shapes = [b.shape[:-2], Kn.shape[:-2]]
+    if minv is not None:
+        shapes.append(minv.shape[:-2])
+    if free is not None:
+        f = torch.as_tensor(free)
+        shapes.append(f.shape[:-2] if (f.dim() >= 2 and tuple(f.shape[-2:]) == (N, d)) else f.shape[:-1])
+    batch = torch.broadcast_shapes(*shapes)
+    nb, nl = L.batch2(batch)
+    be = b.expand(tuple(batch) + (N, d)).contiguous()
+    ke, bc = L.bview(Kn, batch, ci)
+    te, _ = L.bview(Tv, batch, ci)
+    me, bm = L.dof_mask(free, batch, N, d, b.device)
+    if minv is not None:
+        mi, bmi = L.bview(minv.to(dt), batch, (N, 6 if d == 3 else 3))
+    else:
+        mi, bmi = None, L.Bcast(0, 0)
+    if x0 is not None:
+        x = x0.to(dt).expand(tuple(batch) + (N, d)).contiguous().clone()
+    else:
+        x = torch.empty(tuple(batch) + (N, d), device=b.device, dtype=dt)
+    g = L.grid_of(dims, h)
+    nsys = nb * nl
+    wbytes = L.lib().lpm_pcg_workspace(ctypes.byref(g), nsys, L.dtype_code(dt))
+    work = torch.empty(int(wbytes), device=b.device, dtype=torch.uint8)
+    iters = np.zeros(nsys, np.int32)
+    rel = np.zeros(nsys, np.float64)
+    conv = np.zeros(nsys, np.int32)
+    o = _opts(minv is not None, tol, maxit, check, graph)
+    L.check(L.lib().lpm_pcg_solve(ctypes.byref(g), nb, nl, L.dtype_code(dt), layout, _ptr(ke), _ptr(te), bc, _ptr(me),
+                                  bm, _ptr(mi), bmi, _ptr(be), _ptr(x), 1 if x0 is not None else 0, ctypes.byref(o),
+                                  _ptr(work), iters.ctypes.data_as(ctypes.c_void_p),
+                                  rel.ctypes.data_as(ctypes.c_void_p), conv.ctypes.data_as(ctypes.c_void_p),
+                                  L.stream_of(b)))
+    return x, SolveInfo(iters.reshape(nb, nl), rel.reshape(nb, nl), conv.reshape(nb, nl))
+
+
+def pcg_refined(dims, h, b, Kn64, Tv64, Kn32, Tv32, minv32=None, free=None, tol=1e-8, maxit=20000, inner_tol=1e-5,
+                max_outer=30, x0=None, check=10, graph=False, layout64=L.DIRECTED, layout32=L.DIRECTED):
+    """FP32 (P)CG inner solves inside an FP64 iterative refinement; b FP64 (…, N, d).  Returns x FP64 and SolveInfo
+    (relres = true FP64 relative residual, iters = total inner iterations, outer = refinement steps)."""
+    dims, N, d, NN = _dims(dims)
+    L.require_gpu(b, Kn64, Tv64, Kn32, Tv32)
+    if b.dtype != torch.float64 or Kn64.dtype != torch.float64 or Kn32.dtype != torch.float32:
+        raise TypeError("refined solve: b and the FP64 operator in float64, the inner operator in float32")
+    ci64 = _coef_inner(d, N, layout64)
+    ci32 = _coef_inner(d, N, layout32)
+    shapes = [b.shape[:-2], Kn64.shape[:-2], Kn32.shape[:-2]]
+    if minv32 is not None:
+        shapes.append(minv32.shape[:-2])
+    if free is not None:
+        f = torch.as_tensor(free)
+        shapes.append(f.shape[:-2] if (f.dim() >= 2 and tuple(f.shape[-2:]) == (N, d)) else f.shape[:-1])
+    batch = torch.broadcast_shapes(*shapes)
+    nb, nl = L.batch2(batch)
+    be = b.expand(tuple(batch) + (N, d)).contiguous()
+    k64, bc64 = L.bview(Kn64, batch, ci64)
+    t64, _ = L.bview(Tv64, batch, ci64)
+    k32, bc32 = L.bview(Kn32, batch, ci32)
+    t32, _ = L.bview(Tv32, batch, ci32)
+    me, bm = L.dof_mask(free, batch, N, d, b.device)
+    if minv32 is not None:
+        mi, bmi = L.bview(minv32.to(torch.float32), batch, (N, 6 if d == 3 else 3))
+    else:
+        mi, bmi = None, L.Bcast(0, 0)
+    if x0 is not None:
+        x = x0.to(torch.float64).expand(tuple(batch) + (N, d)).contiguous().clone()
+    else:
+        x = torch.empty(tuple(batch) + (N, d), device=b.device, dtype=torch.float64)
+    g = L.grid_of(dims, h)
+    nsys = nb * nl
+    work = torch.empty(int(L.lib().lpm_refined_workspace(ctypes.byref(g), nsys)), device=b.device, dtype=torch.uint8)
+    iters = np.zeros(nsys, np.int32)
+    rel = np.zeros(nsys, np.float64)
+    conv = np.zeros(nsys, np.int32)
+    outer = np.zeros(1, np.int32)
+    o = _opts(minv32 is not None, tol, maxit, check, graph, inner_tol, max_outer)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    L.check(L.lib().lpm_pcg_refined(ctypes.byref(g), nb, nl, layout64, _ptr(k64), _ptr(t64), bc64, layout32, _ptr(k32),
+                                    _ptr(t32), bc32, _ptr(me), bm, _ptr(mi), bmi, _ptr(be), _ptr(x),
+                                    1 if x0 is not None else 0, ctypes.byref(o), _ptr(work), vp(iters), vp(rel),
+                                    vp(outer), vp(conv), L.stream_of(b)))
+    return x, SolveInfo(iters.reshape(nb, nl), rel.reshape(nb, nl), conv.reshape(nb, nl), int(outer[0]))

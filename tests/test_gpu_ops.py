@@ -1,0 +1,172 @@
+"""HIP kernels of liblpm.so against the oracle (oracle/lpm_ref.py) on the same seeded inputs.
+
+Tolerances: FP64 kernels agree with the FP64 oracle to 1e-12 relative (max-norm); FP32 kernels to 2e-6 relative
+(FP32 round-off of sums of 18 terms with O(1) coefficients)."""
+import numpy as np
+import pytest
+import torch
+
+import lpm_ref as R
+import lpm_ops as ops
+import _lpmlib as L
+
+pytestmark = pytest.mark.gpu
+
+TOL = {torch.float64: 1e-12, torch.float32: 2e-6}
+
+
+def rel(a, b):
+    a = np.asarray(a.detach().cpu() if torch.is_tensor(a) else a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def case(dims, seed=0, two_phase=False):
+    rng = np.random.default_rng(seed)
+    lat = R.Box(dims)
+    if two_phase:
+        ph = rng.random(lat.N) < 0.3
+        E = np.where(ph, 10.0, 1.0)
+        C = R.iso_C(E, np.full(lat.N, 0.25))
+        pp = R.sc_params_from_C(C, 0.5) if lat.dim == 3 else R.square_params_from_C(C)
+        Kn, Tv = R.two_end_mean(lat, pp)
+    else:
+        Kn, Tv = R.random_symmetric_params(lat, rng)
+    u = rng.standard_normal((lat.N, lat.dim))
+    return lat, Kn, Tv, u
+
+
+DIMS = [(12, 12, 12), (7, 5, 9), (50, 9, 6), (3, 2, 1), (1, 1, 1), (16, 16), (37, 5), (1, 3)]
+
+
+@pytest.mark.parametrize("dims", DIMS)
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_apply_directed(dev, dims, dt):
+    print("apply", dims, dt, flush=True)
+    lat, Kn, Tv, u = case(dims, seed=len(dims) * 7 + dims[0])
+    ref = R.apply(lat, u, Kn, Tv)
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=dt)
+    out = ops.apply(dims, 1.0, t(u), t(Kn), t(Tv))
+    assert rel(out, ref) < TOL[dt]
+
+
+@pytest.mark.parametrize("dims", [(12, 12, 12), (64, 10, 5), (20, 14)])
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_apply_packed_matches_directed(dev, dims, dt):
+    print("packed", dims, dt, flush=True)
+    lat, Kn, Tv, u = case(dims, seed=3, two_phase=True)
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=dt)
+    kp, tp, asym = ops.pack(dims, t(Kn), t(Tv))
+    assert int(asym.sum()) == 0
+    out = ops.apply(dims, 1.0, t(u), kp, tp, layout=L.PACKED)
+    assert rel(out, R.apply(lat, u, Kn, Tv)) < TOL[dt]
+
+
+def test_pack_flags_asymmetry(dev):
+    lat, Kn, Tv, u = case((6, 5, 4), seed=1)
+    Kn = Kn.copy()
+    Kn[7, 0] += 0.5
+    _, _, asym = ops.pack((6, 5, 4), torch.as_tensor(Kn, device=dev), torch.as_tensor(Tv, device=dev))
+    assert int(asym.sum()) == 1
+
+
+def test_apply_broadcast_batch(dev):
+    dims = (10, 9, 8)
+    rng = np.random.default_rng(11)
+    lat = R.Box(dims)
+    B, Lc = 2, 3
+    Ks = [R.random_symmetric_params(lat, rng) for _ in range(B)]
+    U = rng.standard_normal((B, Lc, lat.N, 3))
+    t = lambda a: torch.as_tensor(np.asarray(a), device=dev, dtype=torch.float64)
+    Kn = t([k for k, _ in Ks])[:, None]
+    Tv = t([v for _, v in Ks])[:, None]
+    out = ops.apply(dims, 1.0, t(U), Kn, Tv)
+    for b in range(B):
+        for l in range(Lc):
+            assert rel(out[b, l], R.apply(lat, U[b, l], *Ks[b])) < 1e-12
+
+
+@pytest.mark.parametrize("dims", [(9, 8, 7), (13, 11)])
+def test_post_outputs(dev, dims):
+    lat, Kn, Tv, u = case(dims, seed=5, two_phase=True)
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    o = ops.post(dims, 1.0, t(u), t(Kn), t(Tv), ku=True, stress=True, energy=True, dl=True, force=True)
+    dL, F = R.bond_forces(lat, u, Kn, Tv)
+    assert rel(o["ku"], R.apply(lat, u, Kn, Tv)) < 1e-12
+    assert rel(o["stress"], R.stress(lat, u, Kn, Tv)) < 1e-12
+    assert rel(o["energy"], R.energy(lat, u, Kn, Tv)) < 1e-12
+    assert rel(o["dl"], dL) < 1e-12
+    assert rel(o["force"], F) < 1e-12
+
+
+@pytest.mark.parametrize("dims", [(6, 7, 5), (9, 6)])
+def test_block_jacobi(dev, dims):
+    lat, Kn, Tv, u = case(dims, seed=9, two_phase=True)
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    D = ops.block_jacobi(dims, 1.0, t(Kn), t(Tv), raw=True)
+    assert rel(D, R.block_diag(lat, Kn, Tv)) < 1e-12
+    rng = np.random.default_rng(2)
+    free = rng.random(lat.N) > 0.2
+    M = ops.block_jacobi(dims, 1.0, t(Kn), t(Tv), free=torch.as_tensor(free, device=dev))
+    assert rel(ops.full_from_sym(M), R.precond(lat, Kn, Tv, free)) < 1e-11
+
+
+def _bc_problem(dims, seed):
+    lat, Kn, Tv, _ = case(dims, seed=seed, two_phase=True)
+    if lat.dim == 3:
+        nc = dims
+        fixed, up, _ = R.prescribed3("k_xy", lat.x, lat.ijk, nc, 1)
+    else:
+        fixed, up, _ = R.prescribed2("ring_xx", lat.x, lat.ijk, dims[0], dims[1], 1, 1.0)
+    up = up * fixed[:, None]
+    b = -R.apply(lat, up, Kn, Tv) * (~fixed)[:, None]
+    return lat, Kn, Tv, fixed, up, b
+
+
+@pytest.mark.parametrize("dims", [(10, 9, 8), (14, 12)])
+@pytest.mark.parametrize("precond", [True, False])
+def test_pcg_fp64_matches_oracle(dev, dims, precond):
+    print("pcg", dims, precond, flush=True)
+    lat, Kn, Tv, fixed, up, b = _bc_problem(dims, 4)
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    free = torch.as_tensor(~fixed, device=dev)
+    minv = ops.block_jacobi(dims, 1.0, t(Kn), t(Tv), free=free) if precond else None
+    x, info = ops.pcg(dims, 1.0, t(b), t(Kn), t(Tv), minv=minv, free=free, tol=1e-12, maxit=5000)
+    if precond:
+        xr, it, rr = R.pcg(lat, b, Kn, Tv, R.precond(lat, Kn, Tv, ~fixed), ~fixed, tol=1e-12, maxit=5000)
+    else:
+        xr, it, rr = R.cg_plain(lat, b, Kn, Tv, np.repeat((~fixed)[:, None], lat.dim, 1), tol=1e-12)
+    assert info.conv.all()
+    assert abs(info.it - it) <= 2
+    K = R.stiffness(lat, Kn, Tv)
+    ud = R.solve_direct(K, np.repeat(fixed, lat.dim), up.ravel()).reshape(-1, lat.dim)
+    assert rel(x.cpu().numpy() + up, ud) < 1e-9
+    assert rel(x, xr) < 1e-9
+
+
+@pytest.mark.parametrize("dims", [(12, 10, 9), (20, 18)])
+def test_refined_reaches_fp64_accuracy(dev, dims):
+    print("refined", dims, flush=True)
+    lat, Kn, Tv, fixed, up, b = _bc_problem(dims, 8)
+    t64 = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    free = torch.as_tensor(~fixed, device=dev)
+    k64, t64_ = t64(Kn), t64(Tv)
+    k32, t32 = k64.float(), t64_.float()
+    minv = ops.block_jacobi(dims, 1.0, k64, t64_, free=free, out_dtype=torch.float32)
+    x, info = ops.pcg_refined(dims, 1.0, t64(b), k64, t64_, k32, t32, minv32=minv, free=free, tol=1e-10)
+    assert info.conv.all() and info.relmax <= 1e-10
+    K = R.stiffness(lat, Kn, Tv)
+    ud = R.solve_direct(K, np.repeat(fixed, lat.dim), up.ravel()).reshape(-1, lat.dim)
+    assert rel(x.cpu().numpy() + up, ud) < 1e-7
+
+
+@pytest.mark.parametrize("dims", [(7, 6, 5), (9, 8)])
+def test_bond_grad_matches_oracle(dev, dims):
+    lat, Kn, Tv, u = case(dims, seed=12, two_phase=True)
+    rng = np.random.default_rng(3)
+    lam = rng.standard_normal(u.shape)
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    gk, gt = ops.bond_grad(dims, 1.0, t(Kn), t(Tv), t(u), t(lam))
+    rk, rt = R.bond_grad(lat, u, lam, Kn, Tv)
+    assert rel(gk, rk) < 1e-12
+    assert rel(gt, rt) < 1e-12
