@@ -1,0 +1,193 @@
+"""Benchmark of the LPM solve path on one MI355X.
+
+Workload (one step = one solve): 256^3 simple-cubic LPM (16.8 M particles, 50 M DOF), four-phase cement-like
+microstructure of random overlapping spheres (porous hydrate Kn 0.1 / hydrate 1 / portlandite 2 / clinker 5 at
+0.15 / 0.55 / 0.15 / 0.15, Tv = 0.3 Kn, two-end mean per bond), tension along z (bottom layer clamped, top layer
+translated), block-Jacobi PCG with FP32 inner iterations inside FP64 iterative refinement to a true FP64 relative
+residual of 1e-8.  A step = right-hand side + preconditioner + refined PCG, inputs resident in HBM.
+
+Prints one JSON line: solves/s, latency percentiles, the K u kernel's roofline (HIP events around every K u launch
+of the timed solves; algorithmic bytes N (2 d w + 18 w) per launch) and the CPU baseline (the C restatement of the
+reference's block-Jacobi PCG, FP64, on the host cores, on a bounded number of iterations of the same problem).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lpm-solver-trained-coarse-graining_amd"))
+
+METRIC = "K·u HBM GB/s (% peak); CG solves/s + p99 latency at rel-res 1e-8"
+HBM_PEAK_GBS = 8000.0
+PHASE_KN = np.array([0.1, 1.0, 2.0, 5.0])
+
+
+def build_problem(n, seed, dev):
+    import torch
+    import synth
+    import lpm_ops as ops
+    import _lpmlib as L
+    dims = (n, n, n)
+    t0 = time.perf_counter()
+    lab = synth.cement_phases(n, seed)
+    kn = PHASE_KN[lab]
+    ppar = np.stack([kn, kn, 0.3 * kn], 1)
+    kp64, tp64 = synth.packed_from_particles(dims, ppar, dev, torch.float64)
+    kp32, tp32 = kp64.float(), tp64.float()
+    z = torch.arange(n, device=dev).repeat_interleave(n * n)
+    fixed = (z == 0) | (z == n - 1)
+    up = torch.zeros((n ** 3, 3), device=dev, dtype=torch.float64)
+    up[z == n - 1, 2] = 1e-3 * (n - 1)
+    free = (~fixed).to(torch.float64)
+    t_setup = time.perf_counter() - t0
+    return dict(dims=dims, lab=lab, kp64=kp64, tp64=tp64, kp32=kp32, tp32=tp32, up=up, free=free, fixed=fixed,
+                t_setup=t_setup, ops=ops, L=L, fractions=np.bincount(lab, minlength=4) / lab.size)
+
+
+def solve(P, tol):
+    import torch
+    ops, L = P["ops"], P["L"]
+    dims = P["dims"]
+    b = -ops.apply(dims, 1.0, P["up"], P["kp64"], P["tp64"], layout=L.PACKED) * P["free"][:, None]
+    M = ops.block_jacobi(dims, 1.0, P["kp64"], P["tp64"], free=P["free"], out_dtype=torch.float32, layout=L.PACKED)
+    x, info = ops.pcg_refined(dims, 1.0, b, P["kp64"], P["tp64"], P["kp32"], P["tp32"], minv32=M, free=P["free"],
+                              tol=tol, maxit=100000, inner_tol=1e-4, layout64=L.PACKED, layout32=L.PACKED)
+    return x, info
+
+
+def cpu_baseline(P, gpu_iters, budget_s=20.0):
+    """The C oracle's block-Jacobi PCG (FP64, OpenMP) on the same problem for a bounded number of iterations; the
+    solve time is the measured time per iteration times the GPU solve's iteration count."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import build_oracle
+    import synth
+    if not os.path.exists(build_oracle.LIB):
+        build_oracle.build()
+    lib = ctypes.CDLL(build_oracle.LIB)
+    Pp = ctypes.c_void_p
+    lib.orc_pcg3.argtypes = [ctypes.c_int64, Pp, Pp, Pp, Pp, Pp, Pp, Pp, ctypes.c_double, ctypes.c_int, Pp]
+    lib.orc_pcg3.restype = ctypes.c_int
+    lib.orc_threads.restype = ctypes.c_int
+    dims = P["dims"]
+    n = dims[0]
+    N = n ** 3
+    Kn, Tv = synth.directed_from_packed(dims, P["kp64"], P["tp64"])
+    Kn = np.ascontiguousarray(Kn.cpu().numpy())
+    Tv = np.ascontiguousarray(Tv.cpu().numpy())
+    import lpm_ops as ops
+    M = ops.full_from_sym(ops.block_jacobi(dims, 1.0, P["kp64"], P["tp64"], free=P["free"], layout=P["L"].PACKED))
+    M = np.ascontiguousarray(M.cpu().numpy())
+    b = -ops.apply(dims, 1.0, P["up"], P["kp64"], P["tp64"], layout=P["L"].PACKED) * P["free"][:, None]
+    b = np.ascontiguousarray(b.cpu().numpy())
+    fr = np.ascontiguousarray((~P["fixed"]).cpu().numpy().astype(np.uint8))
+    del torch
+    # neighbour table (N, 18) int64, -1 outside the box (the reference's lattice.nbr)
+    p = np.arange(N, dtype=np.int64)
+    ijk = np.stack([p % n, (p // n) % n, p // (n * n)], 1)
+    nbr = np.empty((N, 18), np.int64)
+    offs = np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, -1, 0),
+                     (1, -1, 0), (-1, 1, 0), (1, 0, 1), (-1, 0, -1), (1, 0, -1), (-1, 0, 1), (0, 1, 1), (0, -1, -1),
+                     (0, 1, -1), (0, -1, 1)])
+    for k, o in enumerate(offs):
+        q = ijk + o
+        ok = np.all((q >= 0) & (q < n), 1)
+        nbr[:, k] = np.where(ok, (q[:, 2] * n + q[:, 1]) * n + q[:, 0], -1)
+    del ijk, p
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rel = np.zeros(1)
+    x = np.zeros_like(b)
+    t0 = time.perf_counter()
+    lib.orc_pcg3(N, ptr(nbr), ptr(Kn), ptr(Tv), ptr(M), ptr(fr), ptr(b), ptr(x), 0.0, 2, ptr(rel))
+    t_probe = (time.perf_counter() - t0) / 2
+    its = int(max(3, min(200, budget_s / max(t_probe, 1e-6))))
+    x[:] = 0
+    t0 = time.perf_counter()
+    done = lib.orc_pcg3(N, ptr(nbr), ptr(Kn), ptr(Tv), ptr(M), ptr(fr), ptr(b), ptr(x), 0.0, its, ptr(rel))
+    t_it = (time.perf_counter() - t0) / max(done, 1)
+    t_solve = t_it * gpu_iters
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": int(lib.orc_threads()), "kind": "port",
+            "sample": "%d block-Jacobi PCG iterations (FP64, C/OpenMP restatement of coarse3d/lin3d.py:464-518) on "
+                      "the same 256^3 problem: %.3f s per iteration, scaled by the GPU solve's %d PCG iterations"
+                      % (done, t_it, gpu_iters),
+            "sec_per_iter": t_it}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every box of this build has one MI355X and the solve path runs on a "
+              "single device (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    dev = torch.device("cuda:0")
+    P = build_problem(a.n, a.seed, dev)
+    L = P["L"]
+    N = a.n ** 3
+    print("[bench] problem ready: %d^3, phases %s, setup %.1f s" % (a.n, np.round(P["fractions"], 3), P["t_setup"]),
+          file=sys.stderr, flush=True)
+    for w in range(a.warmup):
+        _, info = solve(P, a.tol)
+        torch.cuda.synchronize()
+        print("[bench] warmup %d: %d its, %d outer, relres %.2e" % (w, info.it, info.outer, info.relmax),
+              file=sys.stderr, flush=True)
+    lat = []
+    L.lib().lpm_profile_ku(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = []
+    for s in range(a.steps):
+        ts = time.perf_counter()
+        _, info = solve(P, a.tol)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+        its.append(info.it)
+        print("[bench] step %d: %.3f s, %d its, %d outer, relres %.2e" % (s, lat[-1], info.it, info.outer,
+                                                                         info.relmax), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    L.lib().lpm_profile_ku_read(ctypes.byref(ms), ctypes.byref(cnt))
+    L.lib().lpm_profile_ku(0)
+    ku_ms = ms.value / max(cnt.value, 1)
+    ku_bytes = N * (2 * 3 * 4 + 18 * 4)
+    achieved = ku_bytes / (ku_ms * 1e-3) / 1e9
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": METRIC, "value": a.steps / total, "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": total / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 inner + f64 refinement", "data": "synthetic",
+        "config": {"workload": "SC LPM %d^3 (%d DOF), 4-phase cement-like, tension z, block-Jacobi PCG FP32 + FP64 "
+                               "refinement to rel-res %g" % (a.n, 3 * N, a.tol), "n": a.n, "tol": a.tol,
+                   "seed": a.seed},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
+                       "max": float(lat_ms.max())},
+        "solve": {"pcg_iters": int(np.median(its)), "outer": info.outer, "relres_fp64": info.relmax},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "ku3d_kernel<float,float,PACKED,64,4,MASKDOT>", "launches": cnt.value,
+                     "avg_us": ku_ms * 1e3, "alg_bytes_per_launch": ku_bytes},
+    }
+    if not a.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(P, int(np.median(its)), a.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -7,7 +7,9 @@
  *
  * Lattices are full boxes, particle id p = (z * ny + y) * nx + x; vectors are (systems, N, dim) interleaved,
  * directed coefficients (batch, N, NN) with NN = 18 (3D) or 8 (2D) in the reference slot order; packed
- * coefficients are (batch, NN/2, N) forward-slot planes (valid for symmetric parameters).  A batch of systems is
+ * coefficients are (batch, NN/2, P) forward-slot planes (valid for symmetric parameters), each plane holding
+ * G zero guards, the N particles and G zero guards (P = N + 2G = lpm_packed_len, G = nx ny + nx + 1 in 3D,
+ * nx + 1 in 2D), so the kernels read a neighbour's entry without a bounds test.  A batch of systems is
  * (nb, nl) = (specimens, load cases); every operand carries element strides (sb, sl) for the two batch indices,
  * 0 meaning broadcast, which is the reference's (B,1,N,18) against (1|B,L,N,3) convention
  * (coarse3d/torch_lpm3d.py:212-227).
@@ -56,12 +58,23 @@ typedef struct {
 const char* lpm_last_error(void);
 int lpm_version(void);
 
+/* Timing of the K u launches inside lpm_pcg_solve / lpm_pcg_refined on the calling thread (HIP events around
+ * every launch, graph replays excluded).  No counterpart in the reference: it feeds the roofline measurement. */
+void lpm_profile_ku(int32_t enable);
+/* Tiling of the 3D K u kernels: wide > 0 selects the 64-wide x tile, < 0 the narrow one, 0 automatic; zc planes
+ * per block (0 automatic).  Process-wide; no counterpart in the reference (tuning knob). */
+void lpm_set_ku_tiling(int32_t wide, int32_t zc);
+void lpm_profile_ku_read(double* total_ms, int64_t* launches);
+
 /* out (nb, nl, N, dim) = K u.  Replaces SCT.apply (coarse3d/torch_lpm3d.py:81-83), lin3d.k_apply
  * (coarse3d/lin3d.py:126-149) and the 2D _kmv (timing/t2d/solver2d.py:269-305). */
 int lpm_apply(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* u,
               lpm_bcast bu, const void* kn, const void* tv, lpm_bcast bc, void* out, void* stream);
 
-/* Directed (nbatch, N, NN) -> packed (nbatch, NN/2, N) forward planes, converting dtype_in -> dtype_out.
+/* Plane length P of the packed layout (N plus the two guard bands). */
+int64_t lpm_packed_len(const lpm_grid* g);
+
+/* Directed (nbatch, N, NN) -> packed (nbatch, NN/2, P) forward planes, converting dtype_in -> dtype_out.
  * asym (device, nbatch uint32, zeroed by the caller) counts bonds whose two directed values differ by more than
  * rtol.  Replaces the symmetric duplication of the reference's Scatter (coarse3d/train3d.py:77-97). */
 int lpm_pack_coef(const lpm_grid* g, int32_t nbatch, int32_t dtype_in, int32_t dtype_out, const void* kn,

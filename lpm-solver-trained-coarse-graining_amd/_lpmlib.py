@@ -38,6 +38,10 @@ GP = ctypes.POINTER(Grid)
 SIGS = {
     "lpm_last_error": (ctypes.c_char_p, []),
     "lpm_version": (I32, []),
+    "lpm_profile_ku": (None, [I32]),
+    "lpm_set_ku_tiling": (None, [I32, I32]),
+    "lpm_packed_len": (ctypes.c_int64, [GP]),
+    "lpm_profile_ku_read": (None, [P, P]),
     "lpm_apply": (I32, [GP, I32, I32, I32, I32, P, Bcast, P, P, Bcast, P, P]),
     "lpm_pack_coef": (I32, [GP, I32, I32, I32, P, P, P, P, ctypes.c_double, P, P]),
     "lpm_block_jacobi": (I32, [GP, I32, I32, I32, I32, I32, P, P, Bcast, P, Bcast, P, I32, P]),

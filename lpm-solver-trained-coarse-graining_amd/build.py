@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liblpm.so")
 SRC = os.path.join(HERE, "csrc", "lpm_capi.hip")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
          # packed FP32 instructions are switched off for every kernel of this library
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

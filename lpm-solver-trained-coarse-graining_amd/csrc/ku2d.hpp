@@ -19,10 +19,10 @@ struct Ku2Smem {
 // (fx, fy) together with its u; `own` tells whether the thread owns a site of the lattice.
 template <typename T, typename C, int MODE, int TX, int TY>
 __device__ __forceinline__ void ku2d_tile(const T* __restrict__ u, const C* __restrict__ kn, const C* __restrict__ tv,
-                                          int nx, int ny, int x0, int y0, Ku2Smem<T, TX, TY>& sm, T& fx, T& fy,
-                                          T& ux, T& uy, bool& own, int64_t& pown) {
+                                          int nx, int ny, int64_t N, int x0, int y0, Ku2Smem<T, TX, TY>& sm,
+                                          T& fx, T& fy, T& ux, T& uy, bool& own, int64_t& pown) {
+  // N: plane stride of packed coefficients (unused for directed ones)
   constexpr int HX = TX + 4, HY = TY + 4, SX = TX + 2, SY = TY + 2, NT = TX * TY;
-  const int64_t N = (int64_t)nx * ny;
   const int tid = threadIdx.x;
   for (int idx = tid; idx < HX * HY; idx += NT) {
     const int hx = idx % HX, hy = idx / HX;
@@ -108,6 +108,7 @@ __device__ __forceinline__ void ku2d_tile(const T* __restrict__ u, const C* __re
 template <typename T, typename C>
 struct Ku2Args {
   int nx, ny;
+  int64_t ps;  // plane stride of packed coefficients
   int nl;
   int ntx, nty;
   const T* u;
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(TX* TY) void ku2d_kernel(Ku2Args<T, C> a) {
   T fx, fy, ux, uy;
   bool own;
   int64_t p;
-  ku2d_tile<T, C, MODE, TX, TY>(a.u + a.bu.at(b, l), a.kn + a.bc.at(b, l), a.tv + a.bc.at(b, l), a.nx, a.ny, x0, y0,
+  ku2d_tile<T, C, MODE, TX, TY>(a.u + a.bu.at(b, l), a.kn + a.bc.at(b, l), a.tv + a.bc.at(b, l), a.nx, a.ny, a.ps, x0, y0,
                                 sm, fx, fy, ux, uy, own, p);
   T acc = T(0);
   if (own) {

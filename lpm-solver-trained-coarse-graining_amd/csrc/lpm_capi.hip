@@ -19,6 +19,28 @@ namespace {
 
 thread_local std::string g_err;
 
+// Optional per-thread timing of the K u launches inside the (P)CG loop (HIP events on the solver's stream).
+struct KuProfile {
+  bool on = false;
+  double ms = 0.0;
+  long long count = 0;
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+};
+thread_local KuProfile g_prof;
+
+int prof_flush() {
+  for (size_t i = 0; i + 1 < g_prof.used; i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g_prof.ev[i], g_prof.ev[i + 1]) == hipSuccess) {
+      g_prof.ms += ms;
+      ++g_prof.count;
+    }
+  }
+  g_prof.used = 0;
+  return 0;
+}
+
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -44,7 +66,14 @@ int fail(int code, const char* fmt, ...) {
 
 inline Bcast bc_of(lpm_bcast b) { return Bcast{b.sb, b.sl}; }
 inline int64_t nsites(const lpm_grid* g) { return (int64_t)g->nx * g->ny * (g->dim == 3 ? g->nz : 1); }
-inline Box box_of(const lpm_grid* g) { return Box{g->nx, g->ny, g->dim == 3 ? g->nz : 1}; }
+inline int64_t guard_of(const lpm_grid* g) { return packed_guard(g->dim, g->nx, g->ny); }
+inline int64_t pstride_of(const lpm_grid* g) { return nsites(g) + 2 * guard_of(g); }
+inline Box box_of(const lpm_grid* g) { return Box{g->nx, g->ny, g->dim == 3 ? g->nz : 1, pstride_of(g)}; }
+// kernels address packed planes from their first particle (past the leading guard)
+inline const void* cbase(const lpm_grid* g, const void* p, int layout, int dtype) {
+  if (layout != LPM_COEF_PACKED || !p) return p;
+  return (const char*)p + guard_of(g) * (dtype == LPM_F32 ? 4 : 8);
+}
 
 int check_grid(const lpm_grid* g) {
   if (!g) return fail(LPM_E_BADARG, "grid is null");
@@ -63,12 +92,15 @@ struct KuGeom {
   int ntx, nty, nzc, zc, tx, ty;
 };
 
+// tiling overrides (0 = automatic): x tile width of the 3D kernels and planes per block
+int g_tile_wide = 0, g_zc = 0;
+
 // tile shapes: 3D float 64x4 or 32x8, double 32x4 or 16x8; 2D float 32x8, double 32x4
 template <int DIM, typename T>
 KuGeom ku_geom(const lpm_grid* g, int nb, int nl) {
   KuGeom k{};
   if constexpr (DIM == 3) {
-    const bool wide = g->nx >= 48;
+    const bool wide = g_tile_wide ? g_tile_wide > 0 : g->nx >= 48;
     if (sizeof(T) == 4) {
       k.tx = wide ? 64 : 32;
       k.ty = wide ? 4 : 8;
@@ -79,8 +111,9 @@ KuGeom ku_geom(const lpm_grid* g, int nb, int nl) {
     k.ntx = (g->nx + k.tx - 1) / k.tx;
     k.nty = (g->ny + k.ty - 1) / k.ty;
     const int64_t xy = (int64_t)k.ntx * k.nty * nb * nl;
-    const int64_t want = std::max<int64_t>(1, (4096 + xy - 1) / xy);
-    k.zc = std::max<int>(4, (int)((g->nz + want - 1) / want));
+    // about 1024 blocks (two resident per CU): long z marches amortise the 5-plane prologue
+    const int64_t want = std::max<int64_t>(1, (1024 + xy - 1) / xy);
+    k.zc = g_zc > 0 ? g_zc : std::max<int>(8, (int)((g->nz + want - 1) / want));
     k.nzc = (g->nz + k.zc - 1) / k.zc;
     k.grid = dim3(k.ntx * k.nty * k.nzc, nl, nb);
   } else {
@@ -157,12 +190,19 @@ struct KuCall {
 
 template <int DIM, typename T, int EPI>
 int ku_call(const lpm_grid* g, int nb, int nl, int layout, const KuGeom& kg, const KuCall& c, hipStream_t st) {
+  // the kernels address a system with 32-bit byte offsets
+  const int64_t N = nsites(g);
+  if (N * 3 * (int64_t)sizeof(T) >= ((int64_t)1 << 31) ||
+      (layout == LPM_COEF_DIRECTED && N * 2 * DIM * 3 * (int64_t)sizeof(T) >= ((int64_t)1 << 32)) ||
+      (layout == LPM_COEF_PACKED && pstride_of(g) * (int64_t)sizeof(T) >= ((int64_t)1 << 31)))
+    return fail(LPM_E_BADARG, "lattice too large for one system (use the packed layout or split the lattice)");
   if constexpr (DIM == 3) {
     Ku3Args<T, T> a{};
     a.u = (const T*)c.u;
     a.bu = c.bu;
-    a.kn = (const T*)c.kn;
-    a.tv = (const T*)c.tv;
+    a.kn = (const T*)cbase(g, c.kn, layout, sizeof(T) == 4 ? LPM_F32 : LPM_F64);
+    a.tv = (const T*)cbase(g, c.tv, layout, sizeof(T) == 4 ? LPM_F32 : LPM_F64);
+    a.ps = pstride_of(g);
     a.bc = c.bc;
     a.mask = c.mask;
     a.bm = c.bm;
@@ -179,8 +219,9 @@ int ku_call(const lpm_grid* g, int nb, int nl, int layout, const KuGeom& kg, con
     Ku2Args<T, T> a{};
     a.u = (const T*)c.u;
     a.bu = c.bu;
-    a.kn = (const T*)c.kn;
-    a.tv = (const T*)c.tv;
+    a.kn = (const T*)cbase(g, c.kn, layout, sizeof(T) == 4 ? LPM_F32 : LPM_F64);
+    a.tv = (const T*)cbase(g, c.tv, layout, sizeof(T) == 4 ? LPM_F32 : LPM_F64);
+    a.ps = pstride_of(g);
     a.bc = c.bc;
     a.mask = c.mask;
     a.bm = c.bm;
@@ -301,10 +342,24 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
   std::vector<int> host_ist((size_t)nsys * NINT);
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
+  bool capturing = false;
   auto body = [&]() -> int {
     KuCall c{p, bvec, kn, tv, bc, mask, bm, nullptr, bvec, ap, bvec, partial, ist + I_ACTIVE};
+    const bool prof = g_prof.on && !capturing;
+    if (prof) {
+      while (g_prof.ev.size() < g_prof.used + 2) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        g_prof.ev.push_back(e);
+      }
+      HIPCHK(hipEventRecord(g_prof.ev[g_prof.used], st));
+    }
     int rc = ku_call<DIM, T, EPI_MASKDOT>(g, nb, nl, layout, kg, c, st);
     if (rc) return rc;
+    if (prof) {
+      HIPCHK(hipEventRecord(g_prof.ev[g_prof.used + 1], st));
+      g_prof.used += 2;
+    }
     hipLaunchKernelGGL((cg_fin_alpha_kernel<T>), dim3(nsys), dim3(256), 0, st, partial, (int)kg.grid.x, scal, ist);
     hipLaunchKernelGGL((cg_update_kernel<D, T>), vgrid, dim3(256), 0, st, v);
     hipLaunchKernelGGL((cg_fin_beta_kernel<T>), dim3(nsys), dim3(256), 0, st, partial, vblk, scal, ist);
@@ -319,7 +374,9 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
     if (o->graph && nrep == check) {
       if (!gexec) {
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        capturing = true;
         for (int i = 0; i < check && rc == LPM_OK; ++i) rc = body();
+        capturing = false;
         hipError_t ec = hipStreamEndCapture(st, &graph);
         if (rc) return rc;
         if (ec != hipSuccess) return fail(LPM_E_HIP, "graph capture: %s", hipGetErrorString(ec));
@@ -333,6 +390,7 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
     done_it += nrep;
     HIPCHK(hipMemcpyAsync(host_ist.data(), ist, sizeof(int) * nsys * NINT, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    prof_flush();
     bool any = false;
     for (int s = 0; s < nsys; ++s) any |= host_ist[(size_t)s * NINT + I_ACTIVE] != 0;
     if (!any) break;
@@ -364,6 +422,25 @@ extern "C" {
 const char* lpm_last_error(void) { return g_err.c_str(); }
 int lpm_version(void) { return 1; }
 
+int64_t lpm_packed_len(const lpm_grid* g) { return check_grid(g) ? 0 : pstride_of(g); }
+
+void lpm_set_ku_tiling(int32_t wide, int32_t zc) {
+  g_tile_wide = wide;
+  g_zc = zc;
+}
+
+void lpm_profile_ku(int32_t enable) {
+  g_prof.on = enable != 0;
+  g_prof.ms = 0.0;
+  g_prof.count = 0;
+  g_prof.used = 0;
+}
+
+void lpm_profile_ku_read(double* total_ms, int64_t* launches) {
+  if (total_ms) *total_ms = g_prof.ms;
+  if (launches) *launches = g_prof.count;
+}
+
 int lpm_apply(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* u, lpm_bcast bu,
               const void* kn, const void* tv, lpm_bcast bc, void* out, void* stream) {
   if (int rc = check_grid(g)) return rc;
@@ -391,8 +468,8 @@ int lpm_pack_coef(const lpm_grid* g, int32_t nbatch, int32_t dtype_in, int32_t d
     return fail(LPM_E_BADARG, "lpm_pack_coef: null pointer or bad batch");
   hipStream_t st = (hipStream_t)stream;
   const Box box = box_of(g);
-  const dim3 grid((unsigned)((nsites(g) + 255) / 256), nbatch);
-#define PACK(DIM, CI, CO)                                                                                      \
+  const dim3 grid((unsigned)((box.ps + 255) / 256), nbatch);
+#define PACK(DIM, CI, CO)                                                                                     \
   hipLaunchKernelGGL((pack_coef_kernel<DIM, CI, CO>), grid, dim3(256), 0, st, box, (const CI*)kn, (const CI*)tv, \
                      (CO*)kn_fwd, (CO*)tv_fwd, rtol, asym)
   const int key = (g->dim == 3 ? 100 : 0) + dtype_in * 10 + dtype_out;
@@ -420,6 +497,8 @@ int lpm_block_jacobi(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype_c,
     return fail(LPM_E_BADARG, "lpm_block_jacobi: null pointer or bad batch");
   hipStream_t st = (hipStream_t)stream;
   const Box box = box_of(g);
+  kn = cbase(g, kn, layout, dtype_c);
+  tv = cbase(g, tv, layout, dtype_c);
   const int64_t N = nsites(g);
   const int D = g->dim;
   const int64_t per = raw ? (int64_t)D * D : (D == 3 ? 6 : 3);
@@ -461,6 +540,8 @@ int lpm_post(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t l
     return fail(LPM_E_BADARG, "lpm_post: null pointer or bad batch");
   hipStream_t st = (hipStream_t)stream;
   const Box box = box_of(g);
+  kn = cbase(g, kn, layout, dtype);
+  tv = cbase(g, tv, layout, dtype);
   const int64_t N = nsites(g);
   const dim3 grid((unsigned)((N + 255) / 256), nl, nb);
   PostOut o{ku, stress, energy, dl, force};
@@ -498,7 +579,9 @@ int lpm_bond_grad(const lpm_grid* g, int32_t ncb, int32_t spc, int32_t dtype, in
   const Box box = box_of(g);
   const int64_t N = nsites(g);
   const int D = g->dim, NN = D == 3 ? 18 : 8;
-  const int64_t cstride = layout == LPM_COEF_PACKED ? N * NN / 2 : N * NN;
+  const int64_t cstride = layout == LPM_COEF_PACKED ? box.ps * NN / 2 : N * NN;
+  kn = cbase(g, kn, layout, dtype);
+  tv = cbase(g, tv, layout, dtype);
   const Bcast bu{(int64_t)spc * N * D, N * D};
   const Bcast bc{cstride, 0};
   const dim3 grid((unsigned)((N + 255) / 256), spc, ncb);

@@ -41,16 +41,24 @@ __host__ __device__ constexpr T nd2(int k, int a) {
 
 // ------------------------------------------------------------------------------------------ coefficient layouts
 // DIRECTED: (..., N, NN) per directed slot, as the reference tensors Kn/Tv (torch_lpm3d.py:212-227).
-// PACKED:   (..., NN/2, N) forward-slot planes (slots 0, 2, 4, ...), valid for symmetric parameters
+// PACKED:   (..., NN/2, P) forward-slot planes (slots 0, 2, 4, ...), valid for symmetric parameters
 //           (Kn[i,k] == Kn[q,k^1]): slot k odd at particle i reads forward plane (k-1)/2 at its neighbour q.
+//           Each plane holds G zero guard values, the N particles, and G more guards (P = N + 2G,
+//           G = nx ny + nx + 1 in 3D, nx + 1 in 2D), so p + (neighbour offset) is inside the plane for every
+//           particle p and slot, bond or not: the kernels read it without a bounds select.
 enum CoefMode : int { COEF_DIRECTED = 0, COEF_PACKED = 1 };
 
+__host__ __device__ inline int64_t packed_guard(int dim, int nx, int ny) {
+  return dim == 3 ? (int64_t)nx * ny + nx + 1 : (int64_t)nx + 1;
+}
+
+// base: first particle of plane 0 (array + G) in PACKED mode; pstride: plane stride P
 template <typename C, int MODE, int NN>
-__device__ __forceinline__ C coef_at(const C* __restrict__ base, int64_t N, int64_t p, int64_t q, int k) {
+__device__ __forceinline__ C coef_at(const C* __restrict__ base, int64_t pstride, int64_t p, int64_t q, int k) {
   if constexpr (MODE == COEF_DIRECTED) {
     return base[p * NN + k];
   } else {
-    return (k & 1) ? base[(int64_t)(k >> 1) * N + q] : base[(int64_t)(k >> 1) * N + p];
+    return (k & 1) ? base[(int64_t)(k >> 1) * pstride + q] : base[(int64_t)(k >> 1) * pstride + p];
   }
 }
 

@@ -26,7 +26,8 @@ struct Lat<2> {
 };
 
 struct Box {
-  int nx, ny, nz;  // nz = 1 in 2D
+  int nx, ny, nz;   // nz = 1 in 2D
+  int64_t ps;       // plane stride of packed coefficients (N + 2 guards); unused for directed ones
   __device__ __forceinline__ void coords(int64_t p, int& x, int& y, int& z) const {
     x = (int)(p % nx);
     const int64_t r = p / nx;
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(256) void shell_sums_kernel(Box box, int nl, const 
 #pragma unroll
     for (int a = 0; a < L::D; ++a)
       if (L::off(k, a)) dl += L::template nd<T>(k, a) * (uu[L::D * p + a] - uu[L::D * q + a]);
-    const T t = (T)coef_at<C, MODE, L::NN>(tt, N, p, q, k);
+    const T t = (T)coef_at<C, MODE, L::NN>(tt, box.ps, p, q, k);
     s[L::shell(k)] += dl;
     ts[L::shell(k)] += t * dl;
   }
@@ -123,8 +124,8 @@ __global__ __launch_bounds__(256) void site_post_kernel(Box box, int nl, double 
       for (int a = 0; a < D; ++a)
         if (L::off(k, a)) dl += L::template nd<T>(k, a) * (uu[D * p + a] - uu[D * q + a]);
       const int c = L::shell(k);
-      const T kv = (T)coef_at<C, MODE, NN>(kk, N, p, q, k);
-      const T tv_ = (T)coef_at<C, MODE, NN>(tt, N, p, q, k);
+      const T kv = (T)coef_at<C, MODE, NN>(kk, box.ps, p, q, k);
+      const T tv_ = (T)coef_at<C, MODE, NN>(tt, box.ps, p, q, k);
       f = T(2) * kv * dl + T(0.5) * (ti[c] + w[4 * q + 2 + c]) + T(0.5) * tv_ * (si[c] + w[4 * q + c]);
       en += T(0.5) * kv * dl * dl;
 #pragma unroll
@@ -251,11 +252,11 @@ __global__ __launch_bounds__(256) void block_jacobi_kernel(Box box, int nl, cons
       kq = (double)kk[q * NN + (k ^ 1)];
       tq = (double)tt[q * NN + (k ^ 1)];
     } else {
-      kq = (double)coef_at<C, MODE, NN>(kk, N, p, q, k);  // symmetric: same bond
-      tq = (double)coef_at<C, MODE, NN>(tt, N, p, q, k);
+      kq = (double)coef_at<C, MODE, NN>(kk, box.ps, p, q, k);  // symmetric: same bond
+      tq = (double)coef_at<C, MODE, NN>(tt, box.ps, p, q, k);
     }
-    const double kp = (double)coef_at<C, MODE, NN>(kk, N, p, q, k);
-    const double tp = (double)coef_at<C, MODE, NN>(tt, N, p, q, k);
+    const double kp = (double)coef_at<C, MODE, NN>(kk, box.ps, p, q, k);
+    const double tp = (double)coef_at<C, MODE, NN>(tt, box.ps, p, q, k);
     const double cf = kp + kq + tq;
     double n[3];
 #pragma unroll
@@ -323,8 +324,9 @@ __global__ __launch_bounds__(256) void block_jacobi_kernel(Box box, int nl, cons
 }
 
 // ----------------------------------------------------------------------------------------- packing
-// Directed (nbatch, N, NN) -> forward planes (nbatch, NN/2, N) in the solver precision; asym[batch] counts the
-// bonds whose two directed slots differ by more than rtol (relative), so the caller can keep the directed layout.
+// Directed (nbatch, N, NN) -> forward planes (nbatch, NN/2, P) with zero guards, in the solver precision;
+// asym[batch] counts the bonds whose two directed slots differ by more than rtol (relative), so the caller can
+// keep the directed layout.  Grid (ceil(P/256), nbatch) over the padded plane.
 template <int DIM, typename CI, typename CO>
 __global__ __launch_bounds__(256) void pack_coef_kernel(Box box, const CI* __restrict__ kn, const CI* __restrict__ tv,
                                                         CO* __restrict__ kp, CO* __restrict__ tp, double rtol,
@@ -332,9 +334,19 @@ __global__ __launch_bounds__(256) void pack_coef_kernel(Box box, const CI* __res
   using L = Lat<DIM>;
   constexpr int NN = L::NN;
   const int64_t N = (int64_t)box.nx * box.ny * box.nz;
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
+  const int64_t G = (box.ps - N) / 2;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // position in the padded plane
+  if (e >= box.ps) return;
   const int b = blockIdx.y;
+  const int64_t p = e - G;
+  if (p < 0 || p >= N) {
+#pragma unroll
+    for (int k = 0; k < NN; k += 2) {
+      kp[((int64_t)b * (NN / 2) + k / 2) * box.ps + e] = CO(0);
+      tp[((int64_t)b * (NN / 2) + k / 2) * box.ps + e] = CO(0);
+    }
+    return;
+  }
   const CI* kk = kn + (int64_t)b * N * NN;
   const CI* tt = tv + (int64_t)b * N * NN;
   int x, y, z;
@@ -344,8 +356,8 @@ __global__ __launch_bounds__(256) void pack_coef_kernel(Box box, const CI* __res
   for (int k = 0; k < NN; k += 2) {
     const int64_t q = box.nbr<DIM>(x, y, z, k);
     const CI a1 = kk[p * NN + k], t1 = tt[p * NN + k];
-    kp[((int64_t)b * (NN / 2) + k / 2) * N + p] = (CO)a1;
-    tp[((int64_t)b * (NN / 2) + k / 2) * N + p] = (CO)t1;
+    kp[((int64_t)b * (NN / 2) + k / 2) * box.ps + e] = (CO)a1;
+    tp[((int64_t)b * (NN / 2) + k / 2) * box.ps + e] = (CO)t1;
     if (q >= 0) {
       const CI a2 = kk[q * NN + (k ^ 1)], t2 = tt[q * NN + (k ^ 1)];
       const double sa = fabs((double)a1) + fabs((double)a2), st = fabs((double)t1) + fabs((double)t2);

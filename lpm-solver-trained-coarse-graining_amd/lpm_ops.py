@@ -21,9 +21,22 @@ def _dims(dims):
     return dims, N, len(dims), L.nn_of(len(dims))
 
 
-def _coef_inner(dim, N, layout):
-    NN = L.nn_of(dim)
-    return (N, NN) if layout == L.DIRECTED else (NN // 2, N)
+def packed_guard(dims):
+    """Zero guard values before and after each packed plane: one z-plane plus one row plus one site."""
+    dims = tuple(int(d) for d in dims)
+    return dims[0] * dims[1] + dims[0] + 1 if len(dims) == 3 else dims[0] + 1
+
+
+def packed_len(dims):
+    return int(np.prod([int(d) for d in dims])) + 2 * packed_guard(dims)
+
+
+def _coef_inner(dims, layout):
+    """Per-batch shape of a coefficient array: directed (N, NN) or packed (NN/2, P)."""
+    NN = L.nn_of(len(dims))
+    if layout == L.DIRECTED:
+        return (int(np.prod(dims)), NN)
+    return (NN // 2, packed_len(dims))
 
 
 def apply(dims, h, u, Kn, Tv, layout=L.DIRECTED):
@@ -33,7 +46,7 @@ def apply(dims, h, u, Kn, Tv, layout=L.DIRECTED):
     dt = u.dtype
     if Kn.dtype != dt or Tv.dtype != dt:
         raise TypeError("u, Kn and Tv must share one dtype")
-    ci = _coef_inner(d, N, layout)
+    ci = _coef_inner(dims, layout)
     batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2])
     nb, nl = L.batch2(batch)
     ue, bu = L.bview(u, batch, (N, d))
@@ -53,7 +66,7 @@ def post(dims, h, u, Kn, Tv, ku=False, stress=False, energy=False, dl=False, for
     dims, N, d, NN = _dims(dims)
     L.require_gpu(u, Kn, Tv)
     dt = u.dtype
-    ci = _coef_inner(d, N, layout)
+    ci = _coef_inner(dims, layout)
     batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2])
     nb, nl = L.batch2(batch)
     ue, bu = L.bview(u, batch, (N, d))
@@ -80,7 +93,7 @@ def block_jacobi(dims, h, Kn, Tv, free=None, out_dtype=None, raw=False, batch=No
     """Block-Jacobi inverse blocks (batch, N, 6|3) symmetric, or the diagonal blocks (batch, N, d, d) if raw."""
     dims, N, d, NN = _dims(dims)
     L.require_gpu(Kn, Tv)
-    ci = _coef_inner(d, N, layout)
+    ci = _coef_inner(dims, layout)
     shapes = [Kn.shape[:-2]]
     if free is not None:
         f = torch.as_tensor(free)
@@ -122,7 +135,8 @@ def full_from_sym(S):
 
 
 def pack(dims, Kn, Tv, out_dtype=None, rtol=1e-12):
-    """Directed (..., N, NN) -> packed forward planes (..., NN/2, N); returns (kp, tp, n_asymmetric_bonds)."""
+    """Directed (..., N, NN) -> packed forward planes (..., NN/2, P) with zero guards (P = packed_len(dims));
+    returns (kp, tp, n_asymmetric_bonds per batch)."""
     dims, N, d, NN = _dims(dims)
     L.require_gpu(Kn, Tv)
     lead = Kn.shape[:-2]
@@ -130,13 +144,14 @@ def pack(dims, Kn, Tv, out_dtype=None, rtol=1e-12):
     k = Kn.reshape(nbat, N, NN).contiguous()
     t = Tv.reshape(nbat, N, NN).contiguous()
     odt = out_dtype or Kn.dtype
-    kp = torch.empty((nbat, NN // 2, N), device=Kn.device, dtype=odt)
+    P = packed_len(dims)
+    kp = torch.empty((nbat, NN // 2, P), device=Kn.device, dtype=odt)
     tp = torch.empty_like(kp)
     asym = torch.zeros(nbat, device=Kn.device, dtype=torch.int32)
     g = L.grid_of(dims, 1.0)
     L.check(L.lib().lpm_pack_coef(ctypes.byref(g), nbat, L.dtype_code(Kn.dtype), L.dtype_code(odt), _ptr(k), _ptr(t),
                                   _ptr(kp), _ptr(tp), float(rtol), _ptr(asym), L.stream_of(Kn)))
-    return kp.reshape(tuple(lead) + (NN // 2, N)), tp.reshape(tuple(lead) + (NN // 2, N)), asym
+    return kp.reshape(tuple(lead) + (NN // 2, P)), tp.reshape(tuple(lead) + (NN // 2, P)), asym
 
 
 def bond_grad(dims, h, Kn, Tv, u, lam, scale=1.0, layout=L.DIRECTED):
@@ -146,7 +161,7 @@ def bond_grad(dims, h, Kn, Tv, u, lam, scale=1.0, layout=L.DIRECTED):
     L.require_gpu(Kn, Tv, u, lam)
     dt = u.dtype
     batch = torch.broadcast_shapes(u.shape[:-2], lam.shape[:-2], Kn.shape[:-2])
-    ci = _coef_inner(d, N, layout)
+    ci = _coef_inner(dims, layout)
     nb, nl = L.batch2(batch)
     kb = tuple(Kn.shape[:-2])
     kb2 = (1,) * (len(batch) - len(kb)) + kb
@@ -208,7 +223,7 @@ def pcg(dims, h, b, Kn, Tv, minv=None, free=None, tol=1e-10, maxit=20000, x0=Non
     dims, N, d, NN = _dims(dims)
     L.require_gpu(b, Kn, Tv)
     dt = b.dtype
-    ci = _coef_inner(d, N, layout)
+    ci = _coef_inner(dims, layout)
     shapes = [b.shape[:-2], Kn.shape[:-2]]
     if minv is not None:
         shapes.append(minv.shape[:-2])
@@ -253,8 +268,8 @@ def pcg_refined(dims, h, b, Kn64, Tv64, Kn32, Tv32, minv32=None, free=None, tol=
     L.require_gpu(b, Kn64, Tv64, Kn32, Tv32)
     if b.dtype != torch.float64 or Kn64.dtype != torch.float64 or Kn32.dtype != torch.float32:
         raise TypeError("refined solve: b and the FP64 operator in float64, the inner operator in float32")
-    ci64 = _coef_inner(d, N, layout64)
-    ci32 = _coef_inner(d, N, layout32)
+    ci64 = _coef_inner(dims, layout64)
+    ci32 = _coef_inner(dims, layout32)
     shapes = [b.shape[:-2], Kn64.shape[:-2], Kn32.shape[:-2]]
     if minv32 is not None:
         shapes.append(minv32.shape[:-2])

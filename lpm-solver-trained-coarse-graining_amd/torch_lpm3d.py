@@ -1,0 +1,287 @@
+"""Batched, differentiable 3D simple-cubic LPM on MI355X: the interface of coarse3d/torch_lpm3d.py backed by the HIP
+kernels of liblpm.so.
+
+Operator (coarse3d/torch_lpm3d.py:63-83): dL_ik = n_k.(u_i - u_q), F_ik = 2 Kn dL + 1/2 (TS_i^c + TS_q^c)
++ 1/2 Tv (S_i^c + S_q^c), K u = sum_k F_ik n_k; one fused kernel per K u.  Solves are batched block-Jacobi PCG over
+all (specimen x load) systems with per-system convergence (coarse3d/torch_lpm3d.py:109-148).  LinSolve keeps the
+reference's adjoint contract (coarse3d/torch_lpm3d.py:151-175): backward = one adjoint PCG plus the closed-form
+bond-parameter gradient kernel (no autograd graph through the operator).
+
+With mixed=True an SCT in float64 solves with FP32 inner PCG iterations inside an FP64 iterative refinement
+(true residual and solution update in FP64).  Symmetric coefficient sets are packed once per solve into
+forward-slot planes, which halves the coefficient traffic of every K u in the iteration.
+"""
+import numpy as np
+import torch
+
+import _lpmlib as L
+import lpm_ops as ops
+from geom3d import LatticeGeom, SC_SHELL, LOADS, PLATES, GDIR, prescribed, free_surface_cells  # noqa: F401
+
+NN6 = None
+
+
+def _nn6(ndir):
+    n = ndir
+    return torch.stack([n[:, 0] * n[:, 0], n[:, 1] * n[:, 1], n[:, 2] * n[:, 2], n[:, 1] * n[:, 2],
+                        n[:, 0] * n[:, 2], n[:, 0] * n[:, 1]], 1)
+
+
+def inv3(D):
+    """Inverse of (..., 3, 3) blocks (cofactor form, as coarse3d/torch_lpm3d.py:22-33)."""
+    return torch.linalg.inv(D)
+
+
+class SCT:
+    """Matrix-free SC operator on a full-box LatticeGeom."""
+
+    def __init__(self, lat, device="cuda", dtype=torch.float64, mixed=False, pack=True, graph=False):
+        if getattr(lat, "N", None) != lat.nx * lat.ny * lat.nz:
+            raise NotImplementedError("SCT on MI355X supports full-box lattices (no inactive sites)")
+        self.lat, self.dev, self.dt = lat, torch.device(device), dtype
+        L.require_gpu(torch.empty(0, device=self.dev))
+        L.lib()
+        self.N = lat.N
+        self.dims = (lat.nx, lat.ny, lat.nz)
+        self.h = float(lat.h)
+        self.mixed, self.pack_coef, self.graph = bool(mixed), bool(pack), bool(graph)
+        nbr = lat.nbr.copy()
+        nbr[nbr < 0] = self.N
+        self.nbr = torch.as_tensor(nbr, device=self.dev)
+        self.valid = torch.as_tensor(lat.valid, device=self.dev, dtype=dtype)
+        self.ndir = torch.as_tensor(lat.ndir, device=self.dev, dtype=dtype)
+        self.slot = torch.arange(18, device=self.dev)[None, :]
+        self.opp = torch.as_tensor(np.arange(18) ^ 1, device=self.dev)[None, :]
+        self.shell = torch.as_tensor(SC_SHELL, device=self.dev)
+        self.nn6 = _nn6(self.ndir)
+        self.last_info = None
+
+    # ------------------------------------------------------------------------------------------ kernels
+    def apply(self, u, Kn, Tv):
+        return ops.apply(self.dims, self.h, u, Kn, Tv)
+
+    def forces(self, u, Kn, Tv):
+        o = ops.post(self.dims, self.h, u, Kn, Tv, dl=True, force=True)
+        return o["dl"], o["force"]
+
+    def state(self, u, Kn, Tv):
+        """dL (..., N, 18), per-slot own shell sums Si, TSi and neighbour shell sums Sq, TSq."""
+        o = ops.post(self.dims, self.h, u, Kn, Tv, dl=True)
+        w = o["shell"]                                                   # (..., N, 4) S0 S1 TS0 TS1
+        S, TS = w[..., :2], w[..., 2:]
+        Si, TSi = S[..., self.shell], TS[..., self.shell]
+        pad = lambda x: torch.cat([x, torch.zeros_like(x[..., :1, :])], -2)
+        Sq = pad(Si)[..., self.nbr, self.slot] * self.valid
+        TSq = pad(TSi)[..., self.nbr, self.slot] * self.valid
+        return o["dl"], Si, TSi, Sq, TSq
+
+    def stress(self, u, Kn, Tv):
+        return ops.post(self.dims, self.h, u, Kn, Tv, stress=True)["stress"]
+
+    def energy(self, u, Kn, Tv):
+        return ops.post(self.dims, self.h, u, Kn, Tv, energy=True)["energy"]
+
+    def block_diag(self, Kn, Tv):
+        return ops.block_jacobi(self.dims, self.h, Kn, Tv, raw=True)
+
+    def precond(self, Kn, Tv, free):
+        """Block-Jacobi inverse (..., N, 3, 3), zero on fixed particles (free: (..., N) float)."""
+        return ops.full_from_sym(ops.block_jacobi(self.dims, self.h, Kn, Tv, free=free))
+
+    # ------------------------------------------------------------------------------------------ PCG
+    def _coef(self, Kn, Tv, dtype):
+        """Coefficients for the iteration: packed forward planes when the set is symmetric."""
+        if self.pack_coef:
+            kp, tp, asym = ops.pack(self.dims, Kn, Tv, out_dtype=dtype)
+            if int(asym.sum()) == 0:
+                return kp, tp, L.PACKED
+        return Kn.to(dtype), Tv.to(dtype), L.DIRECTED
+
+    def pcg(self, b, Kn, Tv, M, free, tol=1e-10, maxit=20000, x0=None, check=10):
+        """Solve K_ff x = b for every system of the batch (b zero on fixed, free (..., N)).  Stops each system at
+        ||r|| <= tol ||b||.  Returns x, iterations (max over systems), max relative residual."""
+        Msym = None if M is None else ops.sym_from_full(M)
+        if self.mixed and b.dtype == torch.float64:
+            k64, t64, lay64 = self._coef(Kn, Tv, torch.float64)
+            k32, t32, lay32 = self._coef(Kn, Tv, torch.float32)
+            x, info = ops.pcg_refined(self.dims, self.h, b, k64, t64, k32, t32,
+                                      minv32=None if Msym is None else Msym.float(), free=free, tol=tol, maxit=maxit,
+                                      x0=x0, check=check, graph=self.graph, layout64=lay64, layout32=lay32)
+        else:
+            kk, tt, lay = self._coef(Kn, Tv, b.dtype)
+            x, info = ops.pcg(self.dims, self.h, b, kk, tt, minv=Msym, free=free, tol=tol, maxit=maxit, x0=x0,
+                              check=check, graph=self.graph, layout=lay)
+        self.last_info = info
+        return x, info.it, info.relmax
+
+    def solve(self, Kn, Tv, up, free, tol=1e-10, maxit=20000, M=None):
+        """u with u = up on fixed particles and (K u)_free = 0; up (..., N, 3) zero on free, free (..., N)."""
+        b = -self.apply(up, Kn, Tv) * free[..., None]
+        if M is None:
+            M = self.precond(Kn, Tv, free)
+        x, it, rel = self.pcg(b, Kn, Tv, M, free, tol, maxit)
+        return up + x, it, rel
+
+
+class LinSolve(torch.autograd.Function):
+    """u = solve(Kn, Tv) with the exact adjoint gradient w.r.t. Kn, Tv (op: SCT; up, free fixed)."""
+
+    @staticmethod
+    def forward(ctx, Kn, Tv, op, up, free, tol):
+        with torch.no_grad():
+            M = op.precond(Kn, Tv, free)
+            u, it, rel = op.solve(Kn, Tv, up, free, tol, M=M)
+        ctx.save_for_backward(Kn, Tv, u, M)
+        ctx.op, ctx.free, ctx.tol = op, free, tol
+        ctx.info = (it, rel)
+        return u
+
+    @staticmethod
+    def backward(ctx, g):
+        Kn, Tv, u, M = ctx.saved_tensors
+        op, free = ctx.op, ctx.free
+        with torch.no_grad():
+            lam, _, _ = op.pcg((g * free[..., None]).contiguous(), Kn, Tv, M, free, ctx.tol)
+            gK, gT = ops.bond_grad(op.dims, op.h, Kn, Tv, u, lam, scale=-1.0)
+        return gK, gT, None, None, None, None
+
+
+class LoadSet:
+    """Boundary conditions of the load cases on a lattice whose particles belong to blocks IJK."""
+
+    def __init__(self, lat, IJK, nc, s, h, device, dtype, loads=LOADS):
+        self.loads = loads
+        fr, ups, mov = [], [], []
+        for ld in loads:
+            fixed, up, moving = prescribed(ld, lat.x, IJK, nc, s, h)
+            fr.append(~fixed)
+            ups.append(up)
+            mov.append(np.zeros(lat.N, bool) if moving is None else moving)
+        self.free = torch.as_tensor(np.array(fr), device=device, dtype=dtype)
+        self.up = torch.as_tensor(np.array(ups), device=device, dtype=dtype)
+        self.moving = torch.as_tensor(np.array(mov), device=device, dtype=dtype)
+
+
+def _reactions(P, moving, loads, B, like):
+    cols = []
+    for l, ld in enumerate(loads):
+        if ld in PLATES:
+            cols.append((P[:, l, :, GDIR[ld]] * moving[l]).sum(-1))
+        else:
+            cols.append(torch.zeros(B, device=like.device, dtype=like.dtype))
+    return torch.stack(cols, 1)
+
+
+class CoarseT:
+    """Coarse SC model: nc^3 particles at the block centres, spacing s h."""
+
+    def __init__(self, nc, s, h=1.0, device="cuda", dtype=torch.float64, loads=LOADS, mixed=False):
+        self.nc, self.s, self.h = nc, s, h
+        self.lat = LatticeGeom(nc, nc, nc, s * h)
+        self.op = SCT(self.lat, device, dtype, mixed=mixed)
+        self.N = self.lat.N
+        self.loads = loads
+        self.bc = LoadSet(self.lat, self.lat.ijk, (nc, nc, nc), s, h, device, dtype, loads)
+        self.freecell = self.bc.free.bool()
+        surf = [free_surface_cells(ld, (nc, nc, nc)) if ld in PLATES else np.zeros(self.N, bool) for ld in loads]
+        self.surf = torch.as_tensor(np.array(surf), device=device)
+        self.V = (s * h) ** 3
+
+    def run(self, Kn, Tv, tol=1e-10, differentiable=True):
+        """Kn, Tv (B, N, 18) -> dict u (B,L,N,3), sig (B,L,N,6), w (B,L,N), F (B,L)."""
+        Kn4, Tv4 = Kn[:, None], Tv[:, None]
+        B = Kn.shape[0]
+        up, free = self.bc.up[None], self.bc.free[None]
+        if differentiable:
+            u = LinSolve.apply(Kn4, Tv4, self.op, up.expand(B, -1, -1, -1), free.expand(B, -1, -1), tol)
+        else:
+            with torch.no_grad():
+                u, _, _ = self.op.solve(Kn4, Tv4, up, free, tol)
+        o = _PostFn.apply(Kn4, Tv4, u, self.op)
+        sig, U, P = o
+        F = _reactions(P, self.bc.moving, self.loads, B, Kn)
+        return dict(u=u, sig=sig, w=U / self.V, F=F)
+
+
+class _PostFn(torch.autograd.Function):
+    """(stress, energy, K u) of u with gradients w.r.t. u, Kn and Tv (all three are linear in u for fixed
+    parameters and linear in the parameters for fixed u; the vector-Jacobian products are formed with the same
+    kernels on the GPU)."""
+
+    @staticmethod
+    def forward(ctx, Kn, Tv, u, op):
+        o = ops.post(op.dims, op.h, u, Kn, Tv, ku=True, stress=True, energy=True)
+        ctx.save_for_backward(Kn, Tv, u)
+        ctx.op = op
+        return o["stress"], o["energy"], o["ku"]
+
+    @staticmethod
+    def backward(ctx, gs, ge, gp):
+        Kn, Tv, u = ctx.saved_tensors
+        op = ctx.op
+        with torch.enable_grad():
+            # Small dense-free VJP: the outputs are bilinear in (u, params); use the reference-form torch graph
+            # over the kernel's per-slot quantities so the gradient follows the same formulas.
+            return _post_vjp(op, Kn, Tv, u, gs, ge, gp)
+
+
+def _post_vjp(op, Kn, Tv, u, gs, ge, gp):
+    """VJP of (stress, energy, K u) through torch ops on per-slot tensors produced by the kernels."""
+    Kn_ = Kn.detach().requires_grad_(Kn.requires_grad or True)
+    Tv_ = Tv.detach().requires_grad_(True)
+    u_ = u.detach().requires_grad_(True)
+    dL = ((u_.unsqueeze(-2) - torch.cat([u_, torch.zeros_like(u_[..., :1, :])], -2)[..., op.nbr, :]) *
+          op.ndir).sum(-1) * op.valid
+    TdL = Tv_ * dL
+    S = torch.stack([dL[..., :6].sum(-1), dL[..., 6:].sum(-1)], -1)
+    TS = torch.stack([TdL[..., :6].sum(-1), TdL[..., 6:].sum(-1)], -1)
+    Si, TSi = S[..., op.shell], TS[..., op.shell]
+    pad = lambda x: torch.cat([x, torch.zeros_like(x[..., :1, :])], -2)
+    Sq, TSq = pad(Si)[..., op.nbr, op.slot], pad(TSi)[..., op.nbr, op.slot]
+    F = (2 * Kn_ * dL + 0.5 * (TSi + TSq) + 0.5 * Tv_ * (Si + Sq)) * op.valid
+    full = op.lat.valid.sum(1) == 18
+    of = np.where(full[:, None], 0.5, np.where(op.lat.valid[:, np.arange(18) ^ 1], 0.5, 1.0))
+    stw = torch.as_tensor(of / op.lat.V * op.lat.l0[None, :] * op.lat.valid, device=u.device, dtype=u.dtype)
+    sig = (stw * F) @ op.nn6
+    U = 0.5 * (Kn_ * dL * dL).sum(-1) + 0.5 * (S * TS).sum(-1)
+    P = F @ op.ndir
+    outs, grads = [], []
+    for o, g in ((sig, gs), (U, ge), (P, gp)):
+        if g is not None:
+            outs.append(o)
+            grads.append(g)
+    gK, gT, gu = torch.autograd.grad(outs, (Kn_, Tv_, u_), grads, allow_unused=True)
+    gK = None if gK is None else gK.sum_to_size(Kn.shape)
+    gT = None if gT is None else gT.sum_to_size(Tv.shape)
+    return gK, gT, gu, None
+
+
+class FineRefT:
+    """Fine reference (no grad) of an n^3 voxel box and block size s: block means of u, stress, energy density and
+    the grip reactions of the load cases, all samples of a batch at once."""
+
+    def __init__(self, n, s, h=1.0, device="cuda", dtype=torch.float64, loads=LOADS, mixed=False):
+        self.n, self.s, self.h = n, s, h
+        self.nc = n // s
+        self.lat = LatticeGeom(n, n, n, h)
+        self.op = SCT(self.lat, device, dtype, mixed=mixed)
+        IJK = self.lat.ijk // s
+        self.bc = LoadSet(self.lat, IJK, (self.nc,) * 3, s, h, device, dtype, loads)
+        self.loads = loads
+        cid = (IJK[:, 2] * self.nc + IJK[:, 1]) * self.nc + IJK[:, 0]
+        self.cid = torch.as_tensor(cid, device=device)
+        self.Nc = self.nc ** 3
+
+    def bmean(self, f):
+        out = torch.zeros(f.shape[:-2] + (self.Nc, f.shape[-1]), device=f.device, dtype=f.dtype)
+        out.index_add_(f.dim() - 2, self.cid, f)
+        return out / self.s ** 3
+
+    @torch.no_grad()
+    def run(self, Kn, Tv, tol=1e-10):
+        Kn4, Tv4 = Kn[:, None], Tv[:, None]
+        u, it, rel = self.op.solve(Kn4, Tv4, self.bc.up[None], self.bc.free[None], tol)
+        o = ops.post(self.op.dims, self.op.h, u, Kn4, Tv4, ku=True, stress=True, energy=True)
+        F = _reactions(o["ku"], self.bc.moving, self.loads, Kn.shape[0], Kn)
+        return dict(u=self.bmean(u), sig=self.bmean(o["stress"]), w=self.bmean(o["energy"][..., None])[..., 0] / self.h ** 3,
+                    F=F, iters=it, relres=rel)

@@ -50,7 +50,7 @@ def test_apply_directed(dev, dims, dt):
     assert rel(out, ref) < TOL[dt]
 
 
-@pytest.mark.parametrize("dims", [(12, 12, 12), (64, 10, 5), (20, 14)])
+@pytest.mark.parametrize("dims", [(12, 12, 12), (64, 10, 5), (70, 9, 4), (1, 1, 1), (3, 2, 1), (20, 14), (2, 3)])
 @pytest.mark.parametrize("dt", [torch.float64, torch.float32])
 def test_apply_packed_matches_directed(dev, dims, dt):
     print("packed", dims, dt, flush=True)
