@@ -25,6 +25,36 @@ sys.path.insert(0, os.path.join(ROOT, "lpm-solver-trained-coarse-graining_amd"))
 METRIC = "K·u HBM GB/s (% peak); CG solves/s + p99 latency at rel-res 1e-8"
 HBM_PEAK_GBS = 8000.0
 PHASE_KN = np.array([0.1, 1.0, 2.0, 5.0])
+# HBM bytes per launch of the masked K p kernel at 256^3 (FP32, packed), from rocprofv3 --pmc FETCH_SIZE and
+# --pmc WRITE_SIZE runs of `bench.py --probe 256`, scaled by the same counters of a 1.61 GB copy
+# (profiles/ku3d_256cube_fp32_packed.md)
+KU_TRAFFIC_256 = 812452 * 1024 * 2.0 + 196613 * 1024 * 1.0
+
+
+def probe(n, reps):
+    """Kernel probe for the profilers: reps x (calibration copy of 1.61 GB, plain K u), then reps PCG iterations
+    (masked K p inside the solver) on an n^3 lattice with random symmetric coefficients."""
+    import torch
+    import synth
+    import lpm_ops as ops
+    import _lpmlib as L
+    dev = torch.device("cuda:0")
+    dims = (n, n, n)
+    N = n ** 3
+    pp = np.random.default_rng(0).uniform(0.5, 2.0, (N, 3))
+    kp, tp = synth.packed_from_particles(dims, pp, dev, torch.float32)
+    u = torch.randn(N, 3, device=dev)
+    free = torch.ones(N, device=dev)
+    x = torch.ones(N * 24, device=dev)
+    y = torch.empty_like(x)
+    minv = ops.block_jacobi(dims, 1.0, kp, tp, free=free, layout=L.PACKED)
+    for _ in range(reps):
+        torch.mul(x, 2.0, out=y)
+        ops.apply(dims, 1.0, u, kp, tp, layout=L.PACKED)
+    ops.pcg(dims, 1.0, torch.randn(N, 3, device=dev), kp, tp, minv=minv, free=free, tol=0.0, maxit=reps,
+            layout=L.PACKED)
+    torch.cuda.synchronize()
+    print("[probe] done", file=sys.stderr, flush=True)
 
 
 def build_problem(n, seed, dev):
@@ -128,7 +158,12 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--probe", type=int, default=0, help="run the profiler probe on an n^3 lattice and exit")
+    ap.add_argument("--probe-reps", type=int, default=10)
     a = ap.parse_args()
+    if a.probe:
+        probe(a.probe, a.probe_reps)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box of this build has one MI355X and the solve path runs on a "
               "single device (run with --gpus 1)", file=sys.stderr)
@@ -179,7 +214,7 @@ def main():
                        "max": float(lat_ms.max())},
         "solve": {"pcg_iters": int(np.median(its)), "outer": info.outer, "relres_fp64": info.relmax},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": KU_TRAFFIC_256 if a.n == 256 else None,
                      "kernel": "ku3d_kernel<float,float,PACKED,64,4,MASKDOT>", "launches": cnt.value,
                      "avg_us": ku_ms * 1e3, "alg_bytes_per_launch": ku_bytes},
     }

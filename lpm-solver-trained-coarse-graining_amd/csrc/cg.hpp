@@ -45,11 +45,10 @@ __device__ __forceinline__ void apply_minv(const T* __restrict__ m, const T* r, 
 template <int D, typename T>
 __global__ __launch_bounds__(256) void cg_init_kernel(CgVec<T> v, const T* __restrict__ bvec, int r_ready) {
   __shared__ T red[4];
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int l = blockIdx.y, b = blockIdx.z;
   const int64_t s = (int64_t)b * v.nl + l;
   T rz = T(0), rr = T(0), bb = T(0);
-  if (p < v.N) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < v.N; p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t o = (s * v.N + p) * D;
     T r[3], z[3];
     for (int a = 0; a < D; ++a) {
@@ -140,9 +139,8 @@ __global__ __launch_bounds__(256) void cg_update_kernel(CgVec<T> v) {
   const int64_t s = (int64_t)b * v.nl + l;
   if (!v.ist[s * NINT + I_ACTIVE]) return;
   const T alpha = v.scal[s * NSCAL + S_ALPHA];
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   T rz = T(0), rr = T(0);
-  if (p < v.N) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < v.N; p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t o = (s * v.N + p) * D;
     T r[3], z[3];
     for (int a = 0; a < D; ++a) {
@@ -207,11 +205,11 @@ __global__ __launch_bounds__(256) void cg_pupdate_kernel(CgVec<T> v) {
   const int64_t s = (int64_t)b * v.nl + l;
   if (!v.ist[s * NINT + I_ACTIVE]) return;
   const T beta = v.scal[s * NSCAL + S_BETA];
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= v.N) return;
-  const int64_t o = (s * v.N + p) * D;
   const T* z = v.z ? v.z : v.r;
-  for (int a = 0; a < D; ++a) v.p[o + a] = z[o + a] + beta * v.p[o + a];
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < v.N; p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = (s * v.N + p) * D;
+    for (int a = 0; a < D; ++a) v.p[o + a] = z[o + a] + beta * v.p[o + a];
+  }
 }
 
 // ------------------------------------------------------------------------------------------ refinement helpers
@@ -243,9 +241,8 @@ __global__ __launch_bounds__(256) void masked_norm2_kernel(const T* __restrict__
   __shared__ T red[4];
   const int l = blockIdx.y, b = blockIdx.z;
   const int64_t s = (int64_t)b * nl + l;
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   T acc = T(0);
-  if (p < N) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
     for (int a = 0; a < D; ++a) {
       const T m = mask ? (mask[bm.at(b, l) + D * p + a] ? T(1) : T(0)) : T(1);
       const T v = m * bvec[(s * N + p) * D + a];

@@ -112,7 +112,7 @@ __device__ __forceinline__ void load18(const C* __restrict__ base, int64_t ps, i
 }
 
 template <typename T, typename C, int MODE, int TX, int TY, int EPI>
-__global__ __launch_bounds__(TX* TY, 2) void ku3d_kernel(Ku3Args<T, C> a) {
+__global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 2 : 1) void ku3d_kernel(Ku3Args<T, C> a) {
   constexpr int HX = TX + 4, HY = TY + 4;  // u tile with a 2-site halo
   constexpr int SX = TX + 2, SY = TY + 2;  // shell sums with a 1-site halo
   constexpr int NT = TX * TY;
@@ -176,7 +176,11 @@ __global__ __launch_bounds__(TX* TY, 2) void ku3d_kernel(Ku3Args<T, C> a) {
   T dC[18], dN[18];  // own elongations of the output plane and of the next shell plane
   T sC[4], sN[4];    // own shell sums, same two planes
   T tvC[18], knC[18], tvN[18], tvR[18];
-  T tvNN[18], knN[18], tvRN[18];  // loads for the next step, issued a full step ahead
+  // FP32: the next step's coefficients are loaded a full step ahead; FP64 has no registers for a second set and
+  // loads them after the output phase
+  constexpr bool DEEP = sizeof(T) == 4;
+  constexpr int ND = DEEP ? 18 : 1;
+  T tvNN[ND], knN[ND], tvRN[ND];
 
   auto fetch_u = [&](int z) {
     const int zo = cl(z, nz) * nxy;
@@ -274,15 +278,23 @@ __global__ __launch_bounds__(TX* TY, 2) void ku3d_kernel(Ku3Args<T, C> a) {
   const int hx = tx + 2, hy = ty + 2, sx = tx + 1, sy = ty + 1;
   for (int z = zb; z < ze; ++z) {
     fetch_u(z + 3);  // lands during this step
-    load18<T, C, MODE>(tv, ps, own_p(z + 2), nx, nxy, tvNN);
-    load18<T, C, MODE>(kn, ps, own_p(z + 1), nx, nxy, knN);
-    load18<T, C, MODE>(tv, ps, ring_p(z + 2), nx, nxy, tvRN);
+    if constexpr (DEEP) {
+      load18<T, C, MODE>(tv, ps, own_p(z + 2), nx, nxy, tvNN);
+      load18<T, C, MODE>(kn, ps, own_p(z + 1), nx, nxy, knN);
+      load18<T, C, MODE>(tv, ps, ring_p(z + 2), nx, nxy, tvRN);
+    }
     {
       T tmp[4];
+      if constexpr (!DEEP) load18<T, C, MODE>(tv, ps, own_p(z + 1), nx, nxy, tvN);
       shell_site(Keep{}, z + 1, sx, sy, omx, opx, omy, opy, tvN, dN, sN);
+      if constexpr (!DEEP) load18<T, C, MODE>(tv, ps, ring_p(z + 1), nx, nxy, tvR);
       if (has_r) shell_site(Drop{}, z + 1, rsx, rsy, rmx, rpx, rmy, rpy, tvR, dN, tmp);
     }
     __syncthreads();
+    if constexpr (!DEEP) {
+      load18<T, C, MODE>(kn, ps, own_p(z), nx, nxy, knC);
+      load18<T, C, MODE>(tv, ps, own_p(z), nx, nxy, tvC);
+    }
     if (own_xy) {
       const int p = (z * ny + gyo) * nx + gxo;
       const bool mx = omx, px = opx, my = omy, py = opy, mz = z > 0, pz = z < nz - 1;
@@ -347,12 +359,14 @@ __global__ __launch_bounds__(TX* TY, 2) void ku3d_kernel(Ku3Args<T, C> a) {
     for (int k = 0; k < 18; ++k) dC[k] = dN[k];
 #pragma unroll
     for (int k = 0; k < 4; ++k) sC[k] = sN[k];
+    if constexpr (DEEP) {
 #pragma unroll
-    for (int k = 0; k < 18; ++k) {
-      tvC[k] = tvN[k];
-      tvN[k] = tvNN[k];
-      knC[k] = knN[k];
-      tvR[k] = tvRN[k];
+      for (int k = 0; k < 18; ++k) {
+        tvC[k] = tvN[k];
+        tvN[k] = tvNN[k];
+        knC[k] = knN[k];
+        tvR[k] = tvRN[k];
+      }
     }
     store_u(z + 3);  // into the slot of plane z-2 (no longer read)
     __syncthreads();

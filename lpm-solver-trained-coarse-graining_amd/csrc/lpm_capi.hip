@@ -86,6 +86,9 @@ int check_grid(const lpm_grid* g) {
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// blocks per system of the grid-stride vector kernels: at most 1024, so the per-system partial sums stay short
+inline int vec_blocks(int64_t N) { return (int)std::min<int64_t>(1024, std::max<int64_t>(1, (N + 255) / 256)); }
+
 // ------------------------------------------------------------------------------------------------ K u launch
 struct KuGeom {
   dim3 grid, block;
@@ -264,7 +267,7 @@ PcgLayout pcg_layout(const lpm_grid* g, int nsys) {
   constexpr int D = DIM;
   const int64_t N = nsites(g);
   const KuGeom kg = ku_geom<DIM, T>(g, 1, nsys);
-  const int vblk = (int)((N + 255) / 256);
+  const int vblk = vec_blocks(N);
   PcgLayout L{};
   L.maxblk = std::max<int>((int)kg.grid.x, vblk);
   size_t o = 0;
@@ -306,7 +309,7 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
   T* scal = (T*)(work + L.scal);
   int* ist = (int*)(work + L.ist);
   const Bcast bvec{(int64_t)nl * N * D, N * D};
-  const dim3 vgrid((unsigned)((N + 255) / 256), nl, nb);
+  const dim3 vgrid((unsigned)vec_blocks(N), nl, nb);
   const int vblk = (int)vgrid.x;
 
   CgVec<T> v{};
@@ -690,7 +693,7 @@ RefLayout ref_layout(const lpm_grid* g, int nsys) {
   L.d32 = o;
   o += align256(sizeof(float) * (size_t)nsys * N * DIM);
   const KuGeom kg = ku_geom<DIM, double>(g, 1, nsys);
-  const int64_t nblk = std::max<int64_t>(kg.grid.x, (N + 255) / 256);
+  const int64_t nblk = std::max<int64_t>(kg.grid.x, vec_blocks(N));
   L.part = o;
   o += align256(sizeof(double) * (size_t)nsys * nblk);
   L.bn2 = o;
@@ -721,7 +724,7 @@ int refined_core(const lpm_grid* g, int nb, int nl, int layout64, const void* kn
   double* rr2 = (double*)(work + L.rr2);
   double* tin = (double*)(work + L.tin);
   const Bcast bvec{(int64_t)nl * N * D, N * D};
-  const dim3 vgrid((unsigned)((N + 255) / 256), nl, nb);
+  const dim3 vgrid((unsigned)vec_blocks(N), nl, nb);
   const unsigned egrid = (unsigned)((nvec + 255) / 256);
   const KuGeom kg = ku_geom<DIM, double>(g, nb, nl);
 
