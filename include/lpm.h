@@ -104,6 +104,15 @@ int lpm_bond_grad(const lpm_grid* g, int32_t ncb, int32_t spc, int32_t dtype, in
                   const void* tv, const void* u, const void* lam, double scale, void* work, void* gkn, void* gtv,
                   void* stream);
 
+/* Vector-Jacobian product of lpm_post's (K u, stress, energy): for cotangents gku (ncb*spc, N, dim), gstress
+ * (ncb*spc, N, 6|3) and genergy (ncb*spc, N) (any may be NULL = zero) writes gu (ncb*spc, N, dim) and the
+ * directed parameter gradients gkn, gtv (ncb, N, NN) summed over the spc systems of each coefficient batch.
+ * work: ncb*spc*N*(4 + 2*NN) values.  Replaces the autograd backward through the post-processing of
+ * CoarseT.run (coarse3d/torch_lpm3d.py:212-227) and TorchLPM.stress / energy (coarse2d/torch_lpm.py:114-150). */
+int lpm_post_vjp(const lpm_grid* g, int32_t ncb, int32_t spc, int32_t dtype, int32_t layout, const void* kn,
+                 const void* tv, const void* u, const void* gku, const void* gstress, const void* genergy, void* work,
+                 void* gu, void* gkn, void* gtv, void* stream);
+
 /* Batched (P)CG: K_ff x_f = b_f for nb*nl systems, b and x (nb*nl, N, dim) contiguous, x0 = x when use_x0 else 0.
  * minv (symmetric blocks, lpm_block_jacobi layout) is required when opts->precond = 1.  iters / relres / conv are
  * host arrays of nb*nl entries (conv = 1 when ||r|| <= tol ||b||).  Replaces SCT.pcg (coarse3d/torch_lpm3d.py:109-140),

@@ -47,6 +47,7 @@ SIGS = {
     "lpm_block_jacobi": (I32, [GP, I32, I32, I32, I32, I32, P, P, Bcast, P, Bcast, P, I32, P]),
     "lpm_post": (I32, [GP, I32, I32, I32, I32, P, Bcast, P, P, Bcast, P, P, P, P, P, P, P]),
     "lpm_bond_grad": (I32, [GP, I32, I32, I32, I32, P, P, P, P, ctypes.c_double, P, P, P, P]),
+    "lpm_post_vjp": (I32, [GP, I32, I32, I32, I32, P, P, P, P, P, P, P, P, P, P, P]),
     "lpm_pcg_workspace": (ctypes.c_size_t, [GP, I32, I32]),
     "lpm_pcg_solve": (I32, [GP, I32, I32, I32, I32, P, P, Bcast, P, Bcast, P, Bcast, P, P, I32,
                             ctypes.POINTER(CgOpts), P, P, P, P, P]),

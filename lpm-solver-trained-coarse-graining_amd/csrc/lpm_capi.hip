@@ -619,6 +619,57 @@ int lpm_bond_grad(const lpm_grid* g, int32_t ncb, int32_t spc, int32_t dtype, in
   return LPM_OK;
 }
 
+int lpm_post_vjp(const lpm_grid* g, int32_t ncb, int32_t spc, int32_t dtype, int32_t layout, const void* kn,
+                 const void* tv, const void* u, const void* gku, const void* gstress, const void* genergy, void* work,
+                 void* gu, void* gkn, void* gtv, void* stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!kn || !tv || !u || !work || !gu || !gkn || !gtv || ncb < 1 || spc < 1 || ncb > 65535 || spc > 65535 ||
+      (int64_t)ncb * spc > 65535)
+    return fail(LPM_E_BADARG, "lpm_post_vjp: null pointer or bad batch");
+  hipStream_t st = (hipStream_t)stream;
+  const Box box = box_of(g);
+  const int64_t N = nsites(g);
+  const int D = g->dim, NN = D == 3 ? 18 : 8;
+  const int64_t cstride = layout == LPM_COEF_PACKED ? box.ps * NN / 2 : N * NN;
+  kn = cbase(g, kn, layout, dtype);
+  tv = cbase(g, tv, layout, dtype);
+  const int nsys = ncb * spc;
+  const Bcast bu{(int64_t)spc * N * D, N * D};
+  const Bcast bc{cstride, 0};
+  const dim3 grid((unsigned)((N + 255) / 256), spc, ncb);
+  const dim3 sgrid((unsigned)((N + 255) / 256), nsys);
+  const dim3 ggrid((unsigned)((N + 255) / 256), ncb);
+#define VJP(DIM, T, MODE)                                                                                           \
+  do {                                                                                                              \
+    T* ws = (T*)work;                                                                                               \
+    T* cw = ws + (size_t)nsys * N * 4;                                                                              \
+    T* Ww = cw + (size_t)nsys * N * NN;                                                                             \
+    hipLaunchKernelGGL((shell_sums_kernel<DIM, T, T, MODE>), grid, dim3(256), 0, st, box, spc, (const T*)u, bu,    \
+                       (const T*)tv, bc, ws);                                                                       \
+    hipLaunchKernelGGL((vjp_weights_kernel<DIM, T>), sgrid, dim3(256), 0, st, box, g->h, (const T*)gku,            \
+                       (const T*)gstress, cw);                                                                      \
+    hipLaunchKernelGGL((vjp_params_kernel<DIM, T, MODE>), ggrid, dim3(256), 0, st, box, spc, (const T*)u,          \
+                       (const T*)kn, (const T*)tv, cstride, (const T*)ws, (const T*)cw, (const T*)genergy, Ww,     \
+                       (T*)gkn, (T*)gtv);                                                                           \
+    hipLaunchKernelGGL((vjp_u_kernel<DIM, T>), sgrid, dim3(256), 0, st, box, (const T*)Ww, (T*)gu);                \
+  } while (0)
+  const int key = (g->dim == 3 ? 100 : 0) + dtype * 10 + (layout == LPM_COEF_PACKED);
+  switch (key) {
+    case 100: VJP(3, float, COEF_DIRECTED); break;
+    case 101: VJP(3, float, COEF_PACKED); break;
+    case 110: VJP(3, double, COEF_DIRECTED); break;
+    case 111: VJP(3, double, COEF_PACKED); break;
+    case 0: VJP(2, float, COEF_DIRECTED); break;
+    case 1: VJP(2, float, COEF_PACKED); break;
+    case 10: VJP(2, double, COEF_DIRECTED); break;
+    case 11: VJP(2, double, COEF_PACKED); break;
+    default: return fail(LPM_E_BADARG, "lpm_post_vjp: bad dtype");
+  }
+#undef VJP
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
 size_t lpm_pcg_workspace(const lpm_grid* g, int32_t nsys, int32_t dtype) {
   if (check_grid(g) || nsys < 1) return 0;
   if (g->dim == 3) return dtype == LPM_F32 ? pcg_layout<3, float>(g, nsys).total : pcg_layout<3, double>(g, nsys).total;

@@ -367,4 +367,165 @@ __global__ __launch_bounds__(256) void pack_coef_kernel(Box box, const CI* __res
   if (bad) atomicAdd(&asym[b], bad);
 }
 
+// ----------------------------------------------------------------------------------------- post-processing VJP
+// Vector-Jacobian product of (P = K u, sigma = computeStress, U = particle energy) for cotangents (gP, gS, gU):
+// E = sum_i gP_i.P_i + gS_i:sigma_i + gU_i U_i.  With the per-slot weight c_ik = gP_i.n_k + w_ik gS_i:(n_k n_k)
+// (w_ik the computeStress weight of the slot) the first two terms are sum_ik c_ik F_ik, and
+//   dE/dKn_ik = 2 c_ik dL_ik + 1/2 gU_i dL_ik^2
+//   dE/dTv_ik = 1/2 c_ik (S_i^c + S_q^c) + 1/2 C_i^c dL_ik + 1/2 gU_i S_i^c dL_ik
+//   dE/du_i   = sum_k (W_ik + W_{q,k^1}) n_k,
+//   W_jm = 2 Kn_jm c_jm + 1/2 Tv_jm C_j^c + 1/2 D_j^c + gU_j (Kn_jm dL_jm + 1/2 (Tv_jm S_j^c + TS_j^c)),
+// C_j^c = sum_{k in c} (c_jk + c_{q,k^1}), D_j^c = sum_{k in c} (c_jk Tv_jk + c_{q,k^1} Tv_{q,k^1}).
+// For gS = gU = 0 this is the adjoint of bond_grad_kernel with c = lam.n.  Three kernels: slot weights c, then
+// per coefficient batch the parameter gradients and W (summing the systems of the batch in registers), then du.
+
+// computeStress weight of slot k at site (x, y, z): of / V * l0
+template <int DIM, typename T>
+__device__ __forceinline__ T stress_weight(const Box& box, int x, int y, int z, int k, double h) {
+  using L = Lat<DIM>;
+  int nvalid = 0;
+#pragma unroll
+  for (int j = 0; j < L::NN; ++j) nvalid += box.nbr<DIM>(x, y, z, j) >= 0;
+  const bool opp_ok = box.nbr<DIM>(x, y, z, k ^ 1) >= 0;
+  const double vol = DIM == 3 ? h * h * h : h * h;
+  const double len = k < (DIM == 3 ? 6 : 4) ? h : h * 1.4142135623730950488;
+  return (T)(((nvalid == L::NN || opp_ok) ? 0.5 : 1.0) / vol * len);
+}
+
+// c (systems, N, NN) from gP (systems, N, D) and gS (systems, N, 3|6); either may be null
+template <int DIM, typename T>
+__global__ __launch_bounds__(256) void vjp_weights_kernel(Box box, double h, const T* __restrict__ gP,
+                                                          const T* __restrict__ gS, T* __restrict__ c) {
+  using L = Lat<DIM>;
+  constexpr int NN = L::NN, D = L::D, NS = DIM == 3 ? 6 : 3;
+  const int64_t N = (int64_t)box.nx * box.ny * box.nz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int64_t s = blockIdx.y;
+  int x, y, z;
+  box.coords(p, x, y, z);
+  T gp[3] = {T(0), T(0), T(0)}, gs[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  if (gP)
+    for (int a = 0; a < D; ++a) gp[a] = gP[(s * N + p) * D + a];
+  if (gS)
+    for (int a = 0; a < NS; ++a) gs[a] = gS[(s * N + p) * NS + a];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    T v = T(0);
+    if (box.nbr<DIM>(x, y, z, k) >= 0) {
+      const T n0 = L::template nd<T>(k, 0), n1 = L::template nd<T>(k, 1), n2 = L::template nd<T>(k, 2);
+      v = gp[0] * n0 + gp[1] * n1 + gp[2] * n2;
+      T nn;
+      if constexpr (DIM == 3)
+        nn = gs[0] * n0 * n0 + gs[1] * n1 * n1 + gs[2] * n2 * n2 + gs[3] * n1 * n2 + gs[4] * n0 * n2 + gs[5] * n0 * n1;
+      else
+        nn = gs[0] * n0 * n0 + gs[1] * n1 * n1 + gs[2] * n0 * n1;
+      v += stress_weight<DIM, T>(box, x, y, z, k, h) * nn;
+    }
+    c[(s * N + p) * NN + k] = v;
+  }
+}
+
+// parameter gradients (summed over the systems s = cb * spc + j of coefficient batch cb) and the slot weights W.
+// ws: shell sums of u (systems, N, 4).  Grid (ceil(N/256), ncb).
+template <int DIM, typename T, int MODE>
+__global__ __launch_bounds__(256) void vjp_params_kernel(Box box, int spc, const T* __restrict__ u,
+                                                         const T* __restrict__ kn, const T* __restrict__ tv,
+                                                         int64_t cstride, const T* __restrict__ ws,
+                                                         const T* __restrict__ c, const T* __restrict__ gU,
+                                                         T* __restrict__ W, T* __restrict__ gkn, T* __restrict__ gtv) {
+  using L = Lat<DIM>;
+  constexpr int NN = L::NN, D = L::D;
+  const int64_t N = (int64_t)box.nx * box.ny * box.nz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int cb = blockIdx.y;
+  const T* kk = kn + (int64_t)cb * cstride;
+  const T* tt = tv + (int64_t)cb * cstride;
+  int x, y, z;
+  box.coords(p, x, y, z);
+  int64_t q[NN];
+  T knv[NN], tvv[NN], tvq[NN];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    q[k] = box.nbr<DIM>(x, y, z, k);
+    const int64_t qq = q[k] >= 0 ? q[k] : p;
+    knv[k] = coef_at<T, MODE, NN>(kk, box.ps, p, qq, k);
+    tvv[k] = coef_at<T, MODE, NN>(tt, box.ps, p, qq, k);
+    // Tv of the reverse slot at the neighbour
+    if constexpr (MODE == COEF_DIRECTED)
+      tvq[k] = q[k] >= 0 ? tt[q[k] * NN + (k ^ 1)] : T(0);
+    else
+      tvq[k] = tvv[k];
+  }
+  T gk[NN], gt[NN];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) gk[k] = gt[k] = T(0);
+  for (int j = 0; j < spc; ++j) {
+    const int64_t s = (int64_t)cb * spc + j;
+    const T* uu = u + s * N * D;
+    const T* w = ws + s * N * 4;
+    const T* cs = c + s * N * NN;
+    const T gu = gU ? gU[s * N + p] : T(0);
+    T dl[NN], cc[NN], Cc[2] = {T(0), T(0)}, Dc[2] = {T(0), T(0)};
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+      dl[k] = T(0);
+      cc[k] = cs[p * NN + k];
+      if (q[k] < 0) continue;
+      for (int a = 0; a < D; ++a)
+        if (L::off(k, a)) dl[k] += L::template nd<T>(k, a) * (uu[D * p + a] - uu[D * q[k] + a]);
+      const T cq = cs[q[k] * NN + (k ^ 1)];
+      const int sh = L::shell(k);
+      Cc[sh] += cc[k] + cq;
+      Dc[sh] += cc[k] * tvv[k] + cq * tvq[k];
+    }
+    const T si[2] = {w[4 * p], w[4 * p + 1]}, ti[2] = {w[4 * p + 2], w[4 * p + 3]};
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+      T Wk = T(0);
+      if (q[k] >= 0) {
+        const int sh = L::shell(k);
+        const T sq = w[4 * q[k] + sh];
+        gk[k] += T(2) * cc[k] * dl[k] + T(0.5) * gu * dl[k] * dl[k];
+        gt[k] += T(0.5) * cc[k] * (si[sh] + sq) + T(0.5) * Cc[sh] * dl[k] + T(0.5) * gu * si[sh] * dl[k];
+        Wk = T(2) * knv[k] * cc[k] + T(0.5) * tvv[k] * Cc[sh] + T(0.5) * Dc[sh] +
+             gu * (knv[k] * dl[k] + T(0.5) * (tvv[k] * si[sh] + ti[sh]));
+      }
+      W[(s * N + p) * NN + k] = Wk;
+    }
+  }
+  T* ok = gkn + ((int64_t)cb * N + p) * NN;
+  T* ot = gtv + ((int64_t)cb * N + p) * NN;
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    ok[k] = gk[k];
+    ot[k] = gt[k];
+  }
+}
+
+// du_i = sum_k (W_ik + W_{q,k^1}) n_k; grid (ceil(N/256), systems)
+template <int DIM, typename T>
+__global__ __launch_bounds__(256) void vjp_u_kernel(Box box, const T* __restrict__ W, T* __restrict__ gu) {
+  using L = Lat<DIM>;
+  constexpr int NN = L::NN, D = L::D;
+  const int64_t N = (int64_t)box.nx * box.ny * box.nz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int64_t s = blockIdx.y;
+  int x, y, z;
+  box.coords(p, x, y, z);
+  const T* ws = W + s * N * NN;
+  T g[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    const int64_t q = box.nbr<DIM>(x, y, z, k);
+    if (q < 0) continue;
+    const T t = ws[p * NN + k] + ws[q * NN + (k ^ 1)];
+#pragma unroll
+    for (int a = 0; a < D; ++a) g[a] += t * L::template nd<T>(k, a);
+  }
+  for (int a = 0; a < D; ++a) gu[(s * N + p) * D + a] = g[a];
+}
+
 }  // namespace lpm

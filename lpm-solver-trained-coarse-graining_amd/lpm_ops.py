@@ -154,6 +154,29 @@ def pack(dims, Kn, Tv, out_dtype=None, rtol=1e-12):
     return kp.reshape(tuple(lead) + (NN // 2, P)), tp.reshape(tuple(lead) + (NN // 2, P)), asym
 
 
+def _coef_batches(batch, Kn, Tv, ci):
+    """Coefficient batching of a gradient kernel: systems s = cb * spc + j share coefficient batch cb.  Returns
+    (ncb, spc, kk, tt, general) with kk, tt (ncb, *ci) contiguous; general means one set per system, whose
+    gradients still have to be summed to the broadcast shape of Kn."""
+    nb, nl = L.batch2(batch)
+    kb2 = (1,) * (len(batch) - (Kn.dim() - 2)) + tuple(Kn.shape[:-2])
+    if len(batch) == 2 and kb2 == (nb, 1):
+        return nb, nl, Kn.reshape(nb, *ci).contiguous(), Tv.reshape(nb, *ci).contiguous(), False
+    if all(s == 1 for s in kb2):
+        return 1, nb * nl, Kn.reshape(1, *ci).contiguous(), Tv.reshape(1, *ci).contiguous(), False
+    kk = Kn.expand(tuple(batch) + ci).reshape(nb * nl, *ci).contiguous()
+    tt = Tv.expand(tuple(batch) + ci).reshape(nb * nl, *ci).contiguous()
+    return nb * nl, 1, kk, tt, True
+
+
+def _sum_coef_grad(g, batch, Kn, N, NN, general):
+    kb = tuple(Kn.shape[:-2])
+    if general:
+        kb2 = (1,) * (len(batch) - len(kb)) + kb
+        g = g.reshape(tuple(batch) + (N, NN)).sum_to_size(kb2 + (N, NN))
+    return g.reshape(kb + (N, NN))
+
+
 def bond_grad(dims, h, Kn, Tv, u, lam, scale=1.0, layout=L.DIRECTED):
     """scale * d/d(Kn, Tv) sum_s lam_s . K u_s per directed slot, reduced to the broadcast shape of Kn.
     u, lam (..., N, d) with the full batch; Kn (..., N, NN)."""
@@ -163,32 +186,43 @@ def bond_grad(dims, h, Kn, Tv, u, lam, scale=1.0, layout=L.DIRECTED):
     batch = torch.broadcast_shapes(u.shape[:-2], lam.shape[:-2], Kn.shape[:-2])
     ci = _coef_inner(dims, layout)
     nb, nl = L.batch2(batch)
-    kb = tuple(Kn.shape[:-2])
-    kb2 = (1,) * (len(batch) - len(kb)) + kb
     ue = u.expand(tuple(batch) + (N, d)).contiguous()
     le = lam.expand(tuple(batch) + (N, d)).contiguous()
-    if len(batch) == 2 and kb2 == (nb, 1):
-        ncb, spc = nb, nl                       # coefficients per specimen, load cases summed in the kernel
-        kk = Kn.reshape(nb, *ci).contiguous()
-        tt = Tv.reshape(nb, *ci).contiguous()
-    elif all(s == 1 for s in kb2):
-        ncb, spc = 1, nb * nl
-        kk = Kn.reshape(1, *ci).contiguous()
-        tt = Tv.reshape(1, *ci).contiguous()
-    else:                                       # general broadcast: one coefficient set per system, summed below
-        ncb, spc = nb * nl, 1
-        kk = Kn.expand(tuple(batch) + ci).reshape(nb * nl, *ci).contiguous()
-        tt = Tv.expand(tuple(batch) + ci).reshape(nb * nl, *ci).contiguous()
+    ncb, spc, kk, tt, general = _coef_batches(batch, Kn, Tv, ci)
     work = torch.empty((2, nb * nl, N, 4), device=u.device, dtype=dt)
     gk = torch.empty((ncb, N, NN), device=u.device, dtype=dt)
     gt = torch.empty_like(gk)
     g = L.grid_of(dims, h)
     L.check(L.lib().lpm_bond_grad(ctypes.byref(g), ncb, spc, L.dtype_code(dt), layout, _ptr(kk), _ptr(tt), _ptr(ue),
                                   _ptr(le), float(scale), _ptr(work), _ptr(gk), _ptr(gt), L.stream_of(u)))
-    if ncb == nb * nl and spc == 1 and not (len(batch) == 2 and kb2 == (nb, 1)) and not all(s == 1 for s in kb2):
-        gk = gk.reshape(tuple(batch) + (N, NN)).sum_to_size(kb2 + (N, NN))
-        gt = gt.reshape(tuple(batch) + (N, NN)).sum_to_size(kb2 + (N, NN))
-    return gk.reshape(kb + (N, NN)), gt.reshape(kb + (N, NN))
+    return _sum_coef_grad(gk, batch, Kn, N, NN, general), _sum_coef_grad(gt, batch, Kn, N, NN, general)
+
+
+def post_vjp(dims, h, u, Kn, Tv, g_ku=None, g_stress=None, g_energy=None, layout=L.DIRECTED):
+    """Vector-Jacobian product of post()'s (ku, stress, energy) for the given cotangents (None = zero).
+    Returns (gu with u's batch broadcast to the full batch, gKn, gTv directed and reduced to Kn's shape)."""
+    dims, N, d, NN = _dims(dims)
+    L.require_gpu(u, Kn, Tv)
+    dt = u.dtype
+    batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2])
+    ci = _coef_inner(dims, layout)
+    nb, nl = L.batch2(batch)
+    B = tuple(batch)
+    ns = 6 if d == 3 else 3
+    ue = u.expand(B + (N, d)).contiguous()
+    cot = []
+    for gg, tail in ((g_ku, (N, d)), (g_stress, (N, ns)), (g_energy, (N,))):
+        cot.append(None if gg is None else gg.to(dt).expand(B + tail).contiguous())
+    ncb, spc, kk, tt, general = _coef_batches(batch, Kn, Tv, ci)
+    work = torch.empty((nb * nl, N, 4 + 2 * NN), device=u.device, dtype=dt)
+    gu = torch.empty(B + (N, d), device=u.device, dtype=dt)
+    gk = torch.empty((ncb, N, NN), device=u.device, dtype=dt)
+    gt = torch.empty_like(gk)
+    g = L.grid_of(dims, h)
+    L.check(L.lib().lpm_post_vjp(ctypes.byref(g), ncb, spc, L.dtype_code(dt), layout, _ptr(kk), _ptr(tt), _ptr(ue),
+                                 _ptr(cot[0]), _ptr(cot[1]), _ptr(cot[2]), _ptr(work), _ptr(gu), _ptr(gk), _ptr(gt),
+                                 L.stream_of(u)))
+    return gu, _sum_coef_grad(gk, batch, Kn, N, NN, general), _sum_coef_grad(gt, batch, Kn, N, NN, general)
 
 
 class SolveInfo:

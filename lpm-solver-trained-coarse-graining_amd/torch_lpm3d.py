@@ -18,15 +18,6 @@ import _lpmlib as L
 import lpm_ops as ops
 from geom3d import LatticeGeom, SC_SHELL, LOADS, PLATES, GDIR, prescribed, free_surface_cells  # noqa: F401
 
-NN6 = None
-
-
-def _nn6(ndir):
-    n = ndir
-    return torch.stack([n[:, 0] * n[:, 0], n[:, 1] * n[:, 1], n[:, 2] * n[:, 2], n[:, 1] * n[:, 2],
-                        n[:, 0] * n[:, 2], n[:, 0] * n[:, 1]], 1)
-
-
 def inv3(D):
     """Inverse of (..., 3, 3) blocks (cofactor form, as coarse3d/torch_lpm3d.py:22-33)."""
     return torch.linalg.inv(D)
@@ -53,7 +44,6 @@ class SCT:
         self.slot = torch.arange(18, device=self.dev)[None, :]
         self.opp = torch.as_tensor(np.arange(18) ^ 1, device=self.dev)[None, :]
         self.shell = torch.as_tensor(SC_SHELL, device=self.dev)
-        self.nn6 = _nn6(self.ndir)
         self.last_info = None
 
     # ------------------------------------------------------------------------------------------ kernels
@@ -204,12 +194,12 @@ class CoarseT:
 
 
 class _PostFn(torch.autograd.Function):
-    """(stress, energy, K u) of u with gradients w.r.t. u, Kn and Tv (all three are linear in u for fixed
-    parameters and linear in the parameters for fixed u; the vector-Jacobian products are formed with the same
-    kernels on the GPU)."""
+    """(stress, energy, K u) of u with gradients w.r.t. Kn, Tv and u from the post-processing VJP kernels
+    (replaces autograd through SCT.stress / energy / apply in CoarseT.run, coarse3d/torch_lpm3d.py:212-227)."""
 
     @staticmethod
     def forward(ctx, Kn, Tv, u, op):
+        ctx.set_materialize_grads(False)
         o = ops.post(op.dims, op.h, u, Kn, Tv, ku=True, stress=True, energy=True)
         ctx.save_for_backward(Kn, Tv, u)
         ctx.op = op
@@ -219,41 +209,12 @@ class _PostFn(torch.autograd.Function):
     def backward(ctx, gs, ge, gp):
         Kn, Tv, u = ctx.saved_tensors
         op = ctx.op
-        with torch.enable_grad():
-            # Small dense-free VJP: the outputs are bilinear in (u, params); use the reference-form torch graph
-            # over the kernel's per-slot quantities so the gradient follows the same formulas.
-            return _post_vjp(op, Kn, Tv, u, gs, ge, gp)
-
-
-def _post_vjp(op, Kn, Tv, u, gs, ge, gp):
-    """VJP of (stress, energy, K u) through torch ops on per-slot tensors produced by the kernels."""
-    Kn_ = Kn.detach().requires_grad_(Kn.requires_grad or True)
-    Tv_ = Tv.detach().requires_grad_(True)
-    u_ = u.detach().requires_grad_(True)
-    dL = ((u_.unsqueeze(-2) - torch.cat([u_, torch.zeros_like(u_[..., :1, :])], -2)[..., op.nbr, :]) *
-          op.ndir).sum(-1) * op.valid
-    TdL = Tv_ * dL
-    S = torch.stack([dL[..., :6].sum(-1), dL[..., 6:].sum(-1)], -1)
-    TS = torch.stack([TdL[..., :6].sum(-1), TdL[..., 6:].sum(-1)], -1)
-    Si, TSi = S[..., op.shell], TS[..., op.shell]
-    pad = lambda x: torch.cat([x, torch.zeros_like(x[..., :1, :])], -2)
-    Sq, TSq = pad(Si)[..., op.nbr, op.slot], pad(TSi)[..., op.nbr, op.slot]
-    F = (2 * Kn_ * dL + 0.5 * (TSi + TSq) + 0.5 * Tv_ * (Si + Sq)) * op.valid
-    full = op.lat.valid.sum(1) == 18
-    of = np.where(full[:, None], 0.5, np.where(op.lat.valid[:, np.arange(18) ^ 1], 0.5, 1.0))
-    stw = torch.as_tensor(of / op.lat.V * op.lat.l0[None, :] * op.lat.valid, device=u.device, dtype=u.dtype)
-    sig = (stw * F) @ op.nn6
-    U = 0.5 * (Kn_ * dL * dL).sum(-1) + 0.5 * (S * TS).sum(-1)
-    P = F @ op.ndir
-    outs, grads = [], []
-    for o, g in ((sig, gs), (U, ge), (P, gp)):
-        if g is not None:
-            outs.append(o)
-            grads.append(g)
-    gK, gT, gu = torch.autograd.grad(outs, (Kn_, Tv_, u_), grads, allow_unused=True)
-    gK = None if gK is None else gK.sum_to_size(Kn.shape)
-    gT = None if gT is None else gT.sum_to_size(Tv.shape)
-    return gK, gT, gu, None
+        if gs is None and ge is None and gp is None:
+            return None, None, None, None
+        with torch.no_grad():
+            gu, gK, gT = ops.post_vjp(op.dims, op.h, u, Kn, Tv, g_ku=gp, g_stress=gs, g_energy=ge)
+        gu = gu.sum_to_size(u.shape) if ctx.needs_input_grad[2] else None
+        return (gK if ctx.needs_input_grad[0] else None, gT if ctx.needs_input_grad[1] else None, gu, None)
 
 
 class FineRefT:

@@ -170,3 +170,61 @@ def test_bond_grad_matches_oracle(dev, dims):
     rk, rt = R.bond_grad(lat, u, lam, Kn, Tv)
     assert rel(gk, rk) < 1e-12
     assert rel(gt, rt) < 1e-12
+
+
+@pytest.mark.parametrize("dims", [(7, 6, 5), (9, 8), (1, 1, 1)])
+@pytest.mark.parametrize("nb,nl", [(1, 1), (2, 3)])
+def test_post_vjp_matches_linearisation(dev, dims, nb, nl):
+    """<gu, du> and <gKn, dKn> + <gTv, dTv> equal the oracle's directional derivatives of
+    gP.Ku + gS:stress + gU.energy; Ku and stress are linear in u, energy is quadratic (central difference exact), and
+    all three are linear in (Kn, Tv) for fixed u."""
+    print("post_vjp", dims, nb, nl, flush=True)
+    lat = R.Box(dims)
+    rng = np.random.default_rng(21)
+    d, NN, N = lat.dim, lat.NN, lat.N
+    ns = 6 if d == 3 else 3
+    Kn = rng.uniform(0.5, 1.5, (nb, 1, N, NN))         # directed, not symmetric: per-slot gradients are defined
+    Tv = rng.uniform(-0.2, 0.2, (nb, 1, N, NN))
+    u = rng.standard_normal((nb, nl, N, d))
+    gP, gS, gU = rng.standard_normal((nb, nl, N, d)), rng.standard_normal((nb, nl, N, ns)), rng.standard_normal((nb, nl, N))
+    du = rng.standard_normal(u.shape)
+    dK, dT = rng.standard_normal(Kn.shape), rng.standard_normal(Tv.shape)
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    gu, gk, gt = ops.post_vjp(dims, 1.0, t(u), t(Kn), t(Tv), g_ku=t(gP), g_stress=t(gS), g_energy=t(gU))
+    assert gu.shape == u.shape and gk.shape == Kn.shape and gt.shape == Tv.shape
+
+    def obj(uu, kk, tt, b, l):
+        return (np.sum(gP[b, l] * R.apply(lat, uu, kk, tt)) + np.sum(gS[b, l] * R.stress(lat, uu, kk, tt)) +
+                np.sum(gU[b, l] * R.energy(lat, uu, kk, tt)))
+
+    ref_u, ref_p = 0.0, 0.0
+    for b in range(nb):
+        for l in range(nl):
+            k, v = Kn[b, 0], Tv[b, 0]
+            ref_u += 0.5 * (obj(u[b, l] + du[b, l], k, v, b, l) - obj(u[b, l] - du[b, l], k, v, b, l))
+            ref_p += obj(u[b, l], dK[b, 0], dT[b, 0], b, l)
+    got_u = float((gu.cpu().numpy() * du).sum())
+    got_p = float((gk.cpu().numpy() * dK).sum() + (gt.cpu().numpy() * dT).sum())
+    assert abs(got_u - ref_u) < 1e-10 * max(1.0, abs(ref_u))
+    assert abs(got_p - ref_p) < 1e-10 * max(1.0, abs(ref_p))
+
+
+@pytest.mark.parametrize("dims", [(6, 5, 4), (8, 7)])
+def test_post_vjp_packed_and_fp32(dev, dims):
+    """Packed symmetric coefficients give the same gu and directed gradients as the directed layout; FP32 agrees
+    with FP64 to FP32 round-off."""
+    lat, Kn, Tv, u = case(dims, seed=4, two_phase=True)
+    rng = np.random.default_rng(5)
+    ns = 6 if lat.dim == 3 else 3
+    gP, gS, gU = rng.standard_normal(u.shape), rng.standard_normal((lat.N, ns)), rng.standard_normal(lat.N)
+    t = lambda a, dt=torch.float64: torch.as_tensor(a, device=dev, dtype=dt)
+    ref = ops.post_vjp(dims, 1.0, t(u), t(Kn), t(Tv), g_ku=t(gP), g_stress=t(gS), g_energy=t(gU))
+    kp, tp, asym = ops.pack(dims, t(Kn), t(Tv))
+    assert int(asym.sum()) == 0
+    pk = ops.post_vjp(dims, 1.0, t(u), kp, tp, g_ku=t(gP), g_stress=t(gS), g_energy=t(gU), layout=L.PACKED)
+    for a, b in zip(pk, ref):
+        assert rel(a, b.cpu().numpy()) < 1e-12
+    f32 = ops.post_vjp(dims, 1.0, t(u, torch.float32), t(Kn, torch.float32), t(Tv, torch.float32),
+                       g_ku=t(gP, torch.float32), g_stress=t(gS, torch.float32), g_energy=t(gU, torch.float32))
+    for a, b in zip(f32, ref):
+        assert rel(a, b.cpu().numpy()) < 2e-5
