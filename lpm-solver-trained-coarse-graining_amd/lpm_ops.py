@@ -340,3 +340,110 @@ def pcg_refined(dims, h, b, Kn64, Tv64, Kn32, Tv32, minv32=None, free=None, tol=
                                     1 if x0 is not None else 0, ctypes.byref(o), _ptr(work), vp(iters), vp(rel),
                                     vp(outer), vp(conv), L.stream_of(b)))
     return x, SolveInfo(iters.reshape(nb, nl), rel.reshape(nb, nl), conv.reshape(nb, nl), int(outer[0]))
+
+
+# ------------------------------------------------------------------------------------------------ CSR assembly
+_OFFS = {3: np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, -1, 0),
+                      (1, -1, 0), (-1, 1, 0), (1, 0, 1), (-1, 0, -1), (1, 0, -1), (-1, 0, 1), (0, 1, 1),
+                      (0, -1, -1), (0, 1, -1), (0, -1, 1)]),
+         2: np.array([(1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, -1), (1, -1), (-1, 1)])}
+_PATTERNS = {}
+
+
+def csr_pattern(dims):
+    """Block pattern of K on a full box, the structure of the reference's assembly (coarse3d/lin3d.py:306-376):
+    particle q is a column block of row p if q = p, q is a bond neighbour of p, or q is a neighbour of a bond
+    neighbour i of p in the shell of the bond (p, i).  Returns (offsets (O, d) sorted by linear offset,
+    present (N, O) bool, rowptr (N+1,) int64 of column blocks).  Host-side index arithmetic, cached per box."""
+    dims = tuple(int(v) for v in dims)
+    if dims in _PATTERNS:
+        return _PATTERNS[dims]
+    d = len(dims)
+    offs = _OFFS[d]
+    NN = len(offs)
+    nsh = 6 if d == 3 else 4
+    shell = np.arange(NN) >= nsh
+    ijk = np.indices(dims[::-1]).reshape(d, -1)[::-1].T                     # (N, d), x fastest
+    ext = np.array(dims)
+    inbox = lambda c: np.all((c >= 0) & (c < ext), axis=-1)
+    cand = {(0,) * d: np.ones(len(ijk), bool)}
+    for a in range(NN):
+        ia = ijk + offs[a]
+        va = inbox(ia)
+        key = tuple(offs[a])
+        cand[key] = cand.get(key, False) | va
+        for b in range(NN):
+            if shell[b] != shell[a]:
+                continue
+            o = offs[a] + offs[b]
+            key = tuple(o)
+            cand[key] = cand.get(key, False) | (va & inbox(ia + offs[b]))
+    strides = np.cumprod([1] + list(dims[:-1]))
+    keys = sorted(cand, key=lambda o: int(np.dot(o, strides)))
+    offsets = np.array(keys, np.int64)
+    present = np.stack([cand[k] for k in keys], 1)
+    rowptr = np.zeros(len(ijk) + 1, np.int64)
+    np.cumsum(present.sum(1), out=rowptr[1:])
+    _PATTERNS[dims] = (offsets, present, rowptr)
+    return _PATTERNS[dims]
+
+
+def stiffness_csr(dims, h, Kn, Tv, colours_per_pass=None):
+    """Assembled K (d N x d N) in CSR form for one symmetric coefficient set Kn, Tv (N, NN) on the GPU.
+
+    K is probed with the batched matrix-free operator: two particles closer than 5 sites along every axis never
+    share a row's pattern, so with colours c(q) = (q_x mod 5, q_y mod 5[, q_z mod 5]) the product of K with the
+    indicator of (colour c, component b) holds in row (p, a) exactly the entry K[(p, a), (q, b)] of the single
+    column particle q of colour c in p's pattern.  5^d x d probe vectors are applied in batches; entries are
+    gathered into the pattern of csr_pattern (columns ascending).  Returns (indptr int64, indices int32, data)
+    as host NumPy arrays; data in Kn's dtype."""
+    dims, N, d, NN = _dims(dims)
+    L.require_gpu(Kn, Tv)
+    if Kn.shape != (N, NN) or Tv.shape != (N, NN):
+        raise ValueError("stiffness_csr: one coefficient set (N, NN) expected")
+    dev, dt = Kn.device, Kn.dtype
+    offsets, present, rowptr = csr_pattern(dims)
+    O = len(offsets)
+    ijk = np.indices(dims[::-1]).reshape(d, -1)[::-1].T
+    colour_of = np.zeros(N, np.int64)
+    for ax in range(d):
+        colour_of += (ijk[:, ax] % 5) * 5 ** ax
+    ncol = 5 ** d
+    strides = np.cumprod([1] + list(dims[:-1]))
+    lin = offsets @ strides                                                 # (O,)
+    p_idx, o_idx = np.nonzero(present)                                      # row-major: p, then ascending offset
+    q_idx = p_idx + lin[o_idx]
+    c_idx = colour_of[q_idx]
+    vals = torch.empty((len(p_idx), d, d), device=dev, dtype=dt)            # [block, a, b]
+    cpp = colours_per_pass or max(1, min(ncol, int(2e9 // max(1, N * d * d * 8 * 2))))
+    colour_t = torch.as_tensor(colour_of, device=dev)
+    for c0 in range(0, ncol, cpp):
+        cs = torch.arange(c0, min(ncol, c0 + cpp), device=dev)
+        probe = torch.zeros((len(cs), d, N, d), device=dev, dtype=dt)       # [colour, b, particle, component]
+        hit = (colour_t[None, :] == cs[:, None]).to(dt)                     # (nc, N)
+        for b in range(d):
+            probe[:, b, :, b] = hit
+        Y = apply(dims, h, probe, Kn, Tv)                                   # [colour, b, p, a]
+        sel = np.nonzero((c_idx >= c0) & (c_idx < c0 + len(cs)))[0]
+        if len(sel) == 0:
+            continue
+        st = torch.as_tensor(sel, device=dev)
+        cc = torch.as_tensor(c_idx[sel] - c0, device=dev)
+        pp = torch.as_tensor(p_idx[sel], device=dev)
+        vals[st] = Y[cc, :, pp, :].transpose(1, 2)                          # (blocks, a, b)
+    # CSR row (p, a) holds the blocks of p in offset order, each with d columns b
+    nblk = np.diff(rowptr)
+    blk_start = np.repeat(rowptr[:-1], nblk)                                # first block of the row's particle
+    local = np.arange(len(p_idx)) - blk_start                               # block number within the row
+    rowlen = np.repeat(d * nblk, d)
+    indptr = np.zeros(d * N + 1, np.int64)
+    np.cumsum(rowlen, out=indptr[1:])
+    data = torch.empty(int(indptr[-1]), device=dev, dtype=dt)
+    indices = np.empty(int(indptr[-1]), np.int32)
+    for a in range(d):
+        base = indptr[d * p_idx + a] + d * local                            # (blocks,)
+        for b in range(d):
+            pos = base + b
+            indices[pos] = d * q_idx + b
+            data[torch.as_tensor(pos, device=dev)] = vals[:, a, b]
+    return indptr, indices, data.cpu().numpy()

@@ -24,26 +24,34 @@ def inv3(D):
 
 
 class SCT:
-    """Matrix-free SC operator on a full-box LatticeGeom."""
+    """Matrix-free LPM operator on a full-box lattice: 3D simple cubic (geom3d.LatticeGeom) or, with the same
+    methods, the 2D square lattice (torch_lpm.SquareLattice).  symmetric_only: refuse coefficient sets with
+    Kn[i,k] != Kn[q,k^1] in solves (the 2D dense reference assembles the energy-form K, which equals this
+    operator only for symmetric sets)."""
 
-    def __init__(self, lat, device="cuda", dtype=torch.float64, mixed=False, pack=True, graph=False):
-        if getattr(lat, "N", None) != lat.nx * lat.ny * lat.nz:
-            raise NotImplementedError("SCT on MI355X supports full-box lattices (no inactive sites)")
+    def __init__(self, lat, device="cuda", dtype=torch.float64, mixed=False, pack=True, graph=False,
+                 symmetric_only=False):
+        dims = tuple(int(v) for v in lat.dims)
+        if getattr(lat, "N", None) != int(np.prod(dims)):
+            raise NotImplementedError("the MI355X operator supports full-box lattices (no inactive sites)")
         self.lat, self.dev, self.dt = lat, torch.device(device), dtype
         L.require_gpu(torch.empty(0, device=self.dev))
         L.lib()
         self.N = lat.N
-        self.dims = (lat.nx, lat.ny, lat.nz)
+        self.dims = dims
+        self.d = len(dims)
         self.h = float(lat.h)
         self.mixed, self.pack_coef, self.graph = bool(mixed), bool(pack), bool(graph)
+        self.symmetric_only = bool(symmetric_only)
+        NN = lat.NN
         nbr = lat.nbr.copy()
         nbr[nbr < 0] = self.N
         self.nbr = torch.as_tensor(nbr, device=self.dev)
         self.valid = torch.as_tensor(lat.valid, device=self.dev, dtype=dtype)
         self.ndir = torch.as_tensor(lat.ndir, device=self.dev, dtype=dtype)
-        self.slot = torch.arange(18, device=self.dev)[None, :]
-        self.opp = torch.as_tensor(np.arange(18) ^ 1, device=self.dev)[None, :]
-        self.shell = torch.as_tensor(SC_SHELL, device=self.dev)
+        self.slot = torch.arange(NN, device=self.dev)[None, :]
+        self.opp = torch.as_tensor(np.arange(NN) ^ 1, device=self.dev)[None, :]
+        self.shell = torch.as_tensor(np.asarray(lat.shell), device=self.dev)
         self.last_info = None
 
     # ------------------------------------------------------------------------------------------ kernels
@@ -75,16 +83,18 @@ class SCT:
         return ops.block_jacobi(self.dims, self.h, Kn, Tv, raw=True)
 
     def precond(self, Kn, Tv, free):
-        """Block-Jacobi inverse (..., N, 3, 3), zero on fixed particles (free: (..., N) float)."""
+        """Block-Jacobi inverse (..., N, d, d), zero on fixed particles (free: (..., N) float)."""
         return ops.full_from_sym(ops.block_jacobi(self.dims, self.h, Kn, Tv, free=free))
 
     # ------------------------------------------------------------------------------------------ PCG
     def _coef(self, Kn, Tv, dtype):
         """Coefficients for the iteration: packed forward planes when the set is symmetric."""
-        if self.pack_coef:
+        if self.pack_coef or self.symmetric_only:
             kp, tp, asym = ops.pack(self.dims, Kn, Tv, out_dtype=dtype)
             if int(asym.sum()) == 0:
                 return kp, tp, L.PACKED
+            if self.symmetric_only:
+                raise ValueError("bond parameters are not symmetric (Kn[i,k] != Kn[q,k^1] or Tv likewise)")
         return Kn.to(dtype), Tv.to(dtype), L.DIRECTED
 
     def pcg(self, b, Kn, Tv, M, free, tol=1e-10, maxit=20000, x0=None, check=10):

@@ -9,6 +9,7 @@ import pytest
 import torch
 
 import lpm_ref as R
+import lpm_ops as ops
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "lpm-solver-trained-coarse-graining_amd")
@@ -87,3 +88,40 @@ def test_parameter_maps_match_oracle():
     # generalised calcKnTv of an isotropic cell reduces to the isotropic map (both shells, one Tv)
     cp = G.cell_params_from_C66(G.iso_C66(C), 0.5)
     assert np.allclose(cp[:, 0], pp[:, 0]) and np.allclose(cp[:, 3], pp[:, 1]) and np.allclose(cp[:, 9], pp[:, 2])
+
+
+@pytest.mark.parametrize("dims", [(5, 4, 3), (6, 5), (2, 1, 3), (1, 2)])
+def test_csr_pattern_is_the_structure_of_BtHB(dims):
+    """The block pattern of the probed assembly equals the structural nonzeros of the oracle's B^T H B for generic
+    coefficients (the group rule of coarse3d/lin3d.py:310-376)."""
+    lat = R.Box(dims)
+    Kn, Tv = R.random_symmetric_params(lat, np.random.default_rng(0))
+    K = R.stiffness(lat, Kn, Tv).tocsr()
+    K.eliminate_zeros()
+    d = lat.dim
+    blocks = set(zip(*(np.array((K != 0).nonzero()) // d)))
+    offsets, present, rowptr = ops.csr_pattern(dims)
+    strides = np.cumprod([1] + list(dims[:-1]))
+    mine = {(int(p), int(p + offsets[o] @ strides)) for p, o in zip(*np.nonzero(present))}
+    assert mine == {(int(a), int(b)) for a, b in blocks}
+    assert rowptr[-1] == present.sum()
+
+
+def test_geom2d_matches_oracle():
+    import geom2d
+    lat = geom2d.SquareGeom(7, 5, 0.5)
+    ref = R.Box((7, 5), 0.5)
+    assert np.array_equal(lat.nbr, ref.nbr) and np.allclose(lat.ndir, ref.ndir) and np.allclose(lat.x, ref.x)
+    assert np.allclose(lat.l0, ref.l0)
+    for ld in geom2d.LOADS:
+        a = geom2d.prescribed(ld, lat.x, lat.ij, 7, 5, 1, 0.5, 1)
+        b = R.prescribed2(ld, ref.x, ref.ijk, 7, 5, 1, 0.5, 1)
+        assert np.array_equal(a[0], b[0]) and np.allclose(a[1], b[1])
+        assert (a[2] is None) == (b[2] is None) and (a[2] is None or np.array_equal(a[2], b[2]))
+    C = geom2d.iso_C(np.array([1.0, 10.0]), np.array([0.2, 0.3]))
+    assert np.allclose(C, R.iso_C(np.array([1.0, 10.0]), np.array([0.2, 0.3])))
+    pp = geom2d.square_params_from_C(C[[0, 1, 0, 1] * 9][:35])
+    assert np.allclose(pp, R.square_params_from_C(C[[0, 1, 0, 1] * 9][:35]))
+    Kn, Tv = geom2d.bond_params_two_end_mean(lat, pp)
+    Kr, Tr = R.two_end_mean(ref, pp)
+    assert np.allclose(Kn, Kr) and np.allclose(Tv, Tr)
