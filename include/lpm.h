@@ -2,8 +2,9 @@
  *
  * Every buffer is a device pointer owned by the caller (PyTorch tensors' data_ptr(), or hipMalloc from another
  * host).  `stream` is a hipStream_t (the caller's current stream).  Every entry point returns an int status
- * (LPM_OK = 0); lpm_last_error() describes the last failure of the calling thread.  No entry point allocates
- * device memory: solvers take a caller-owned workspace whose size the *_workspace query returns.
+ * (LPM_OK = 0); lpm_last_error() describes the last failure of the calling thread.  Apart from lpm_serve_create,
+ * which allocates a server's arena once, no entry point allocates device memory: solvers take a caller-owned
+ * workspace whose size the *_workspace query returns.
  *
  * Lattices are full boxes, particle id p = (z * ny + y) * nx + x; vectors are (systems, N, dim) interleaved,
  * directed coefficients (batch, N, NN) with NN = 18 (3D) or 8 (2D) in the reference slot order; packed
@@ -29,7 +30,8 @@ enum {
   LPM_E_HIP = 2,         /* HIP runtime error */
   LPM_E_NOT_SPD = 3,     /* preconditioner block or curvature not positive */
   LPM_E_NONFINITE = 4,   /* NaN / Inf met in the iteration */
-  LPM_E_UNSUPPORTED = 5  /* valid request this build does not implement */
+  LPM_E_UNSUPPORTED = 5, /* valid request this build does not implement */
+  LPM_E_FULL = 6         /* serving: no free slot or arena space (nothing changed) */
 };
 enum { LPM_F32 = 0, LPM_F64 = 1 };
 enum { LPM_COEF_DIRECTED = 0, LPM_COEF_PACKED = 1 };
@@ -108,7 +110,8 @@ int lpm_bond_grad(const lpm_grid* g, int32_t ncb, int32_t spc, int32_t dtype, in
  * (ncb*spc, N, 6|3) and genergy (ncb*spc, N) (any may be NULL = zero) writes gu (ncb*spc, N, dim) and the
  * directed parameter gradients gkn, gtv (ncb, N, NN) summed over the spc systems of each coefficient batch.
  * work: ncb*spc*N*(4 + 2*NN) values.  Replaces the autograd backward through the post-processing of
- * CoarseT.run (coarse3d/torch_lpm3d.py:212-227) and TorchLPM.stress / energy (coarse2d/torch_lpm.py:114-150). */
+ * CoarseT.run (coarse3d/torch_lpm3d.py:212-227) and of the 2D CoarseT.stress / energy_density
+ * (coarse2d/torch_lpm.py:246-258). */
 int lpm_post_vjp(const lpm_grid* g, int32_t ncb, int32_t spc, int32_t dtype, int32_t layout, const void* kn,
                  const void* tv, const void* u, const void* gku, const void* gstress, const void* genergy, void* work,
                  void* gu, void* gkn, void* gtv, void* stream);
@@ -134,6 +137,29 @@ int lpm_pcg_refined(const lpm_grid* g, int32_t nb, int32_t nl, int32_t layout64,
                     const uint8_t* mask, lpm_bcast bm, const void* minv32, lpm_bcast bminv, const void* b64,
                     void* x64, int32_t use_x0, const lpm_cg_opts* opts, void* work, int32_t* iters, double* relres,
                     int32_t* outer, int32_t* conv, void* stream);
+
+/* ---- Serving: independent 2D square-lattice solves of different sizes, continuously batched.
+ * A server owns a device arena and max_slots slots.  lpm_serve_admit copies one request (kn, tv (N, 8) FP32
+ * directed symmetric, mask (N, 2) 1 = free DOF, up (N, 2) FP64 prescribed displacement, zero on free DOFs) into
+ * a slot, or returns LPM_E_FULL when no slot or arena space is free.  lpm_serve_step launches (optionally as a
+ * replayed hipGraph) one persistent kernel in which every active slot advances its own FP32 block-Jacobi PCG
+ * inside an FP64 refinement by at most `budget` site-iterations; requests admitted between steps join the next
+ * one.  lpm_serve_poll synchronises the stream and reports per slot: state (0 free, 1-3 running, 4 done,
+ * 5 failed), inner iterations, refinement steps and the true FP64 relative residual.  lpm_serve_result copies
+ * u = up + x (N, 2) FP64 of a done slot; lpm_serve_release returns the slot and its arena block.  A server is
+ * not thread-safe.  The solve replaces CoarseT.solve's dense Cholesky (coarse2d/torch_lpm.py:216-229) for
+ * request streams; the slot arena follows the survey's proposed lpm_serving_* interface (admit, step, retire). */
+size_t lpm_serve_footprint(int32_t nx, int32_t ny);
+int lpm_serve_create(int32_t max_slots, size_t arena_bytes, void** server);
+void lpm_serve_destroy(void* server);
+int lpm_serve_admit(void* server, int32_t nx, int32_t ny, double h, const float* kn, const float* tv,
+                    const uint8_t* mask, const double* up, double tol, double inner_tol, int32_t maxit, void* stream,
+                    int32_t* slot);
+int lpm_serve_step(void* server, int64_t budget, int32_t use_graph, void* stream);
+int lpm_serve_poll(void* server, int32_t* state, int32_t* iters, int32_t* outer, double* relres, void* stream);
+int lpm_serve_result(void* server, int32_t slot, double* u, void* stream);
+int lpm_serve_release(void* server, int32_t slot);
+size_t lpm_serve_arena_free(void* server);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,7 @@ import torch
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblpm.so")
 
-LPM_OK, LPM_E_BADARG, LPM_E_HIP, LPM_E_NOT_SPD, LPM_E_NONFINITE, LPM_E_UNSUPPORTED = range(6)
+LPM_OK, LPM_E_BADARG, LPM_E_HIP, LPM_E_NOT_SPD, LPM_E_NONFINITE, LPM_E_UNSUPPORTED, LPM_E_FULL = range(7)
 F32, F64 = 0, 1
 DIRECTED, PACKED = 0, 1
 
@@ -51,6 +51,16 @@ SIGS = {
     "lpm_pcg_workspace": (ctypes.c_size_t, [GP, I32, I32]),
     "lpm_pcg_solve": (I32, [GP, I32, I32, I32, I32, P, P, Bcast, P, Bcast, P, Bcast, P, P, I32,
                             ctypes.POINTER(CgOpts), P, P, P, P, P]),
+    "lpm_serve_footprint": (ctypes.c_size_t, [I32, I32]),
+    "lpm_serve_create": (I32, [I32, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "lpm_serve_destroy": (None, [P]),
+    "lpm_serve_admit": (I32, [P, I32, I32, ctypes.c_double, P, P, P, P, ctypes.c_double, ctypes.c_double, I32, P,
+                              ctypes.POINTER(I32)]),
+    "lpm_serve_step": (I32, [P, ctypes.c_int64, I32, P]),
+    "lpm_serve_poll": (I32, [P, P, P, P, P, P]),
+    "lpm_serve_result": (I32, [P, I32, P, P]),
+    "lpm_serve_release": (I32, [P, I32]),
+    "lpm_serve_arena_free": (ctypes.c_size_t, [P]),
     "lpm_refined_workspace": (ctypes.c_size_t, [GP, I32]),
     "lpm_pcg_refined": (I32, [GP, I32, I32, I32, P, P, Bcast, I32, P, P, Bcast, P, Bcast, P, Bcast, P, P, I32,
                               ctypes.POINTER(CgOpts), P, P, P, P, P, P]),

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -12,6 +13,7 @@
 #include "ku2d.hpp"
 #include "ku3d.hpp"
 #include "post.hpp"
+#include "serve.hpp"
 
 using namespace lpm;
 
@@ -854,6 +856,243 @@ int lpm_pcg_refined(const lpm_grid* g, int32_t nb, int32_t nl, int32_t layout64,
   return refined_core<2>(g, nb, nl, layout64, kn64, tv64, bc_of(bc64), layout32, kn32, tv32, bc_of(bc32), mask,
                          bc_of(bm), (const float*)minv32, bc_of(bminv), (const double*)b64, (double*)x64, use_x0, opts,
                          (char*)work, iters, relres, outer, conv, st);
+}
+
+}  // extern "C"
+
+// ================================================================================================== serving
+namespace {
+
+struct Server {
+  int max_slots = 0;
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+  std::map<size_t, size_t> free_blocks;  // offset -> size, coalesced
+  SlotDesc* d_slots = nullptr;           // device descriptors
+  SlotDesc* h_slots = nullptr;           // pinned host mirror
+  std::vector<size_t> off, len;          // arena block of each slot (len 0 = slot free)
+  std::map<long long, hipGraphExec_t> graphs;
+  hipGraph_t graph_keep[64];
+  int ngraph = 0;
+};
+
+size_t slot_bytes(int nx, int ny) {
+  const size_t N = (size_t)nx * ny;
+  const size_t sizes[] = {N * 8 * 4, N * 8 * 4, N * 2, N * 2 * 8, N * 2 * 8, N * 2 * 8, N * 4 * 8, N * 3 * 4,
+                          N * 2 * 4, N * 2 * 4, N * 2 * 4, N * 2 * 4, N * 2 * 4, N * 4 * 4};
+  size_t t = 0;
+  for (size_t v : sizes) t += align256(v);
+  return t;
+}
+
+bool arena_take(Server* s, size_t n, size_t* off) {
+  for (auto it = s->free_blocks.begin(); it != s->free_blocks.end(); ++it) {
+    if (it->second < n) continue;
+    *off = it->first;
+    const size_t rest = it->second - n, at = it->first + n;
+    s->free_blocks.erase(it);
+    if (rest) s->free_blocks[at] = rest;
+    return true;
+  }
+  return false;
+}
+
+void arena_give(Server* s, size_t off, size_t n) {
+  auto it = s->free_blocks.emplace(off, n).first;
+  auto nx = std::next(it);
+  if (nx != s->free_blocks.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    s->free_blocks.erase(nx);
+  }
+  if (it != s->free_blocks.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      s->free_blocks.erase(it);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t lpm_serve_footprint(int32_t nx, int32_t ny) {
+  if (nx < 1 || ny < 1) return 0;
+  return slot_bytes(nx, ny);
+}
+
+int lpm_serve_create(int32_t max_slots, size_t arena_bytes, void** server) {
+  if (!server || max_slots < 1 || max_slots > 65535 || arena_bytes < 256)
+    return fail(LPM_E_BADARG, "lpm_serve_create: bad arguments");
+  Server* s = new Server();
+  s->max_slots = max_slots;
+  s->arena_bytes = arena_bytes & ~(size_t)255;
+  hipError_t e = hipMalloc((void**)&s->arena, s->arena_bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->d_slots, sizeof(SlotDesc) * max_slots);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_slots, sizeof(SlotDesc) * max_slots, hipHostMallocDefault);
+  if (e == hipSuccess) {
+    std::memset(s->h_slots, 0, sizeof(SlotDesc) * max_slots);
+    e = hipMemcpy(s->d_slots, s->h_slots, sizeof(SlotDesc) * max_slots, hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    if (s->arena) (void)hipFree(s->arena);
+    if (s->d_slots) (void)hipFree(s->d_slots);
+    if (s->h_slots) (void)hipHostFree(s->h_slots);
+    delete s;
+    return fail(LPM_E_HIP, "lpm_serve_create: %s", hipGetErrorString(e));
+  }
+  s->free_blocks[0] = s->arena_bytes;
+  s->off.assign(max_slots, 0);
+  s->len.assign(max_slots, 0);
+  *server = s;
+  return LPM_OK;
+}
+
+void lpm_serve_destroy(void* server) {
+  Server* s = (Server*)server;
+  if (!s) return;
+  (void)hipDeviceSynchronize();
+  for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (int i = 0; i < s->ngraph; ++i) (void)hipGraphDestroy(s->graph_keep[i]);
+  (void)hipFree(s->arena);
+  (void)hipFree(s->d_slots);
+  (void)hipHostFree(s->h_slots);
+  delete s;
+}
+
+int lpm_serve_admit(void* server, int32_t nx, int32_t ny, double h, const float* kn, const float* tv,
+                    const uint8_t* mask, const double* up, double tol, double inner_tol, int32_t maxit, void* stream,
+                    int32_t* slot) {
+  Server* s = (Server*)server;
+  if (!s || !kn || !tv || !mask || !up || !slot || nx < 1 || ny < 1 || (int64_t)nx * ny >= (1 << 30) || maxit < 1 ||
+      !(tol > 0))
+    return fail(LPM_E_BADARG, "lpm_serve_admit: bad arguments");
+  int id = -1;
+  for (int i = 0; i < s->max_slots; ++i)
+    if (s->len[i] == 0) {
+      id = i;
+      break;
+    }
+  const size_t need = slot_bytes(nx, ny);
+  size_t off = 0;
+  if (id < 0 || !arena_take(s, need, &off)) return fail(LPM_E_FULL, "lpm_serve_admit: no free slot or arena space");
+  s->off[id] = off;
+  s->len[id] = need;
+  const size_t N = (size_t)nx * ny;
+  char* base = s->arena + off;
+  auto take = [&](size_t n) {
+    char* p = base;
+    base += align256(n);
+    return p;
+  };
+  SlotDesc d;
+  std::memset(&d, 0, sizeof d);
+  d.nx = nx;
+  d.ny = ny;
+  d.h = (float)h;
+  d.maxit = maxit;
+  d.tol = tol;
+  d.inner_tol = inner_tol;
+  float* dkn = (float*)take(N * 8 * 4);
+  float* dtv = (float*)take(N * 8 * 4);
+  uint8_t* dmask = (uint8_t*)take(N * 2);
+  d.kn = dkn;
+  d.tv = dtv;
+  d.mask = dmask;
+  d.up = (double*)take(N * 2 * 8);
+  d.x = (double*)take(N * 2 * 8);
+  d.b = (double*)take(N * 2 * 8);
+  d.w64 = (double*)take(N * 4 * 8);
+  d.minv = (float*)take(N * 3 * 4);
+  d.d = (float*)take(N * 2 * 4);
+  d.r = (float*)take(N * 2 * 4);
+  d.z = (float*)take(N * 2 * 4);
+  d.p = (float*)take(N * 2 * 4);
+  d.ap = (float*)take(N * 2 * 4);
+  d.w32 = (float*)take(N * 4 * 4);
+  d.state = SLOT_INIT;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(dkn, kn, N * 8 * 4, hipMemcpyDefault, st));
+  HIPCHK(hipMemcpyAsync(dtv, tv, N * 8 * 4, hipMemcpyDefault, st));
+  HIPCHK(hipMemcpyAsync(dmask, mask, N * 2, hipMemcpyDefault, st));
+  HIPCHK(hipMemcpyAsync(d.up, up, N * 2 * 8, hipMemcpyDefault, st));
+  s->h_slots[id] = d;
+  HIPCHK(hipMemcpyAsync(s->d_slots + id, s->h_slots + id, sizeof(SlotDesc), hipMemcpyHostToDevice, st));
+  *slot = id;
+  return LPM_OK;
+}
+
+int lpm_serve_step(void* server, int64_t budget, int32_t use_graph, void* stream) {
+  Server* s = (Server*)server;
+  if (!s || budget < 1) return fail(LPM_E_BADARG, "lpm_serve_step: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((unsigned)s->max_slots), block(srv::NT);
+  if (!use_graph) {
+    hipLaunchKernelGGL(serve_kernel, grid, block, 0, st, s->d_slots, s->max_slots, (long long)budget);
+    LAUNCHCHK();
+    return LPM_OK;
+  }
+  auto it = s->graphs.find(budget);
+  if (it == s->graphs.end()) {
+    if (s->ngraph >= 64) return fail(LPM_E_BADARG, "lpm_serve_step: too many distinct budgets");
+    hipGraph_t gr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(serve_kernel, grid, block, 0, st, s->d_slots, s->max_slots, (long long)budget);
+    HIPCHK(hipStreamEndCapture(st, &gr));
+    hipGraphExec_t ex;
+    HIPCHK(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+    s->graph_keep[s->ngraph++] = gr;
+    it = s->graphs.emplace(budget, ex).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, st));
+  return LPM_OK;
+}
+
+int lpm_serve_poll(void* server, int32_t* state, int32_t* iters, int32_t* outer, double* relres, void* stream) {
+  Server* s = (Server*)server;
+  if (!s || !state) return fail(LPM_E_BADARG, "lpm_serve_poll: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(s->h_slots, s->d_slots, sizeof(SlotDesc) * s->max_slots, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int i = 0; i < s->max_slots; ++i) {
+    const SlotDesc& d = s->h_slots[i];
+    state[i] = s->len[i] ? d.state : SLOT_FREE;
+    if (iters) iters[i] = d.total;
+    if (outer) outer[i] = d.outer;
+    if (relres) relres[i] = d.rel;
+  }
+  return LPM_OK;
+}
+
+int lpm_serve_result(void* server, int32_t slot, double* u, void* stream) {
+  Server* s = (Server*)server;
+  if (!s || !u || slot < 0 || slot >= s->max_slots || s->len[slot] == 0)
+    return fail(LPM_E_BADARG, "lpm_serve_result: bad slot");
+  const SlotDesc& d = s->h_slots[slot];
+  if (d.state != SLOT_DONE) return fail(LPM_E_BADARG, "lpm_serve_result: slot %d is not done", slot);
+  HIPCHK(hipMemcpyAsync(u, d.up, (size_t)d.nx * d.ny * 2 * 8, hipMemcpyDefault, (hipStream_t)stream));
+  return LPM_OK;
+}
+
+int lpm_serve_release(void* server, int32_t slot) {
+  Server* s = (Server*)server;
+  if (!s || slot < 0 || slot >= s->max_slots || s->len[slot] == 0)
+    return fail(LPM_E_BADARG, "lpm_serve_release: bad slot");
+  const int st = s->h_slots[slot].state;
+  if (st != SLOT_DONE && st != SLOT_FAILED)
+    return fail(LPM_E_BADARG, "lpm_serve_release: slot %d is still running", slot);
+  arena_give(s, s->off[slot], s->len[slot]);
+  s->len[slot] = 0;
+  return LPM_OK;
+}
+
+size_t lpm_serve_arena_free(void* server) {
+  Server* s = (Server*)server;
+  if (!s) return 0;
+  size_t t = 0;
+  for (auto& kv : s->free_blocks) t += kv.second;
+  return t;
 }
 
 }  // extern "C"

@@ -1,0 +1,326 @@
+// Serving: many independent 2D solves of different lattice sizes advanced together by one persistent kernel.
+//
+// Every admitted request owns a slot: a descriptor plus its arrays in a device arena (coefficients, mask, prescribed
+// displacement, FP64 solution and right-hand side, FP32 CG vectors).  One workgroup drives one slot through the
+// whole mixed-precision solve (coarse2d/torch_lpm.py:216-229 replaced by a matrix-free block-Jacobi PCG):
+//   INIT   b = -mask * K u_p (FP64), ||b||, 2x2 block-Jacobi inverse M (free DOFs only);
+//   OUTER  r64 = mask * (b - K x) in FP64; stop when ||r64|| <= tol ||b||; else start an FP32 inner PCG on
+//          K d = r64 with the relative tolerance min(1/2, max(inner_tol, tol / (2 rel)));
+//   INNER  FP32 PCG iterations; on inner convergence x += d (FP64) and back to OUTER;
+//   DONE   u = u_p + x written over u_p.
+// The workgroup runs until its slot is done or the launch's work budget (site-iterations) is spent, then leaves
+// its state in the descriptor; the next launch resumes it.  All control values are workgroup-uniform (computed
+// from broadcast reductions), so every barrier is reached by every wave.  The arithmetic of a slot does not depend
+// on the other slots nor on where launches split the work, so a request's result is bit-identical alone and in a
+// mix.  The operator is the force form of lpm::ku (coarse2d/lin2d.py:121-141) with directed FP32 coefficients.
+#pragma once
+#include "lpm_common.hpp"
+
+namespace lpm {
+
+enum SlotState : int { SLOT_FREE = 0, SLOT_INIT = 1, SLOT_OUTER = 2, SLOT_INNER = 3, SLOT_DONE = 4, SLOT_FAILED = 5 };
+
+struct SlotDesc {
+  // request (written at admission)
+  int nx, ny;
+  float h;
+  int maxit;        // total inner iterations allowed
+  double tol, inner_tol;
+  const float* kn;  // (N, 8) directed
+  const float* tv;
+  const uint8_t* mask;  // (N, 2) 1 = free
+  double* up;           // (N, 2) prescribed displacement; the solution on completion
+  double* x;            // (N, 2) FP64 correction
+  double* b;            // (N, 2)
+  double* w64;          // (N, 4) FP64 shell sums
+  float* minv;          // (N, 3) 2x2 symmetric inverse blocks
+  float *d, *r, *z, *p, *ap;  // (N, 2)
+  float* w32;                 // (N, 4)
+  // state (kernel-owned after admission)
+  int state;
+  int it;        // inner iterations of the current inner solve
+  int total;     // inner iterations so far
+  int outer;     // refinement steps so far
+  double bnorm, rel, thr2;  // ||b||, ||r64|| / ||b||, squared inner stopping threshold
+  float rz;
+  int pad_;
+};
+
+namespace srv {
+
+constexpr int NT = 256;  // threads per slot workgroup
+
+// directed slot geometry of the 2D square lattice
+__device__ __forceinline__ int nbr2(int x, int y, int nx, int ny, int k) {
+  const int qx = x + off2(k, 0), qy = y + off2(k, 1);
+  return (qx < 0 || qx >= nx || qy < 0 || qy >= ny) ? -1 : qy * nx + qx;
+}
+
+// sum of two values over the workgroup, broadcast to every thread (fixed order)
+template <typename T>
+__device__ __forceinline__ void wg_sum2(T& a, T& b, T* red) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[2 * w] = a;
+    red[2 * w + 1] = b;
+  }
+  __syncthreads();
+  T sa = T(0), sb = T(0);
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) {
+    sa += red[2 * i];
+    sb += red[2 * i + 1];
+  }
+  a = sa;
+  b = sb;
+  __syncthreads();
+}
+
+// shell sums (S0, S1, TS0, TS1) of u at every site into w
+template <typename T>
+__device__ void shell_pass(const SlotDesc& s, const T* __restrict__ u, T* __restrict__ w) {
+  const int N = s.nx * s.ny;
+  for (int p = threadIdx.x; p < N; p += NT) {
+    const int x = p % s.nx, y = p / s.nx;
+    T a[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int q = nbr2(x, y, s.nx, s.ny, k);
+      if (q < 0) continue;
+      const T dl = nd2<T>(k, 0) * (u[2 * p] - u[2 * q]) + nd2<T>(k, 1) * (u[2 * p + 1] - u[2 * q + 1]);
+      a[shell2(k)] += dl;
+      a[2 + shell2(k)] += (T)s.tv[8 * p + k] * dl;
+    }
+    w[4 * p] = a[0];
+    w[4 * p + 1] = a[1];
+    w[4 * p + 2] = a[2];
+    w[4 * p + 3] = a[3];
+  }
+}
+
+// (K u)_p from u and its shell sums w
+template <typename T>
+__device__ __forceinline__ void ku_site(const SlotDesc& s, const T* __restrict__ u, const T* __restrict__ w, int p,
+                                        T& f0, T& f1) {
+  const int x = p % s.nx, y = p / s.nx;
+  f0 = f1 = T(0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int q = nbr2(x, y, s.nx, s.ny, k);
+    if (q < 0) continue;
+    const int c = shell2(k);
+    const T dl = nd2<T>(k, 0) * (u[2 * p] - u[2 * q]) + nd2<T>(k, 1) * (u[2 * p + 1] - u[2 * q + 1]);
+    const T tv = (T)s.tv[8 * p + k];
+    const T F = T(2) * (T)s.kn[8 * p + k] * dl + T(0.5) * (w[4 * p + 2 + c] + w[4 * q + 2 + c]) +
+                T(0.5) * tv * (w[4 * p + c] + w[4 * q + c]);
+    f0 += F * nd2<T>(k, 0);
+    f1 += F * nd2<T>(k, 1);
+  }
+}
+
+__device__ __forceinline__ float mfree(const SlotDesc& s, int i) { return s.mask[i] ? 1.0f : 0.0f; }
+
+// INIT: b, ||b||, block-Jacobi inverse
+__device__ void phase_init(SlotDesc& s, double* red) {
+  const int N = s.nx * s.ny;
+  shell_pass<double>(s, s.up, s.w64);
+  __syncthreads();
+  double bb = 0.0, unused = 0.0;
+  for (int p = threadIdx.x; p < N; p += NT) {
+    double f0, f1;
+    ku_site<double>(s, s.up, s.w64, p, f0, f1);
+    const double b0 = -(double)mfree(s, 2 * p) * f0, b1 = -(double)mfree(s, 2 * p + 1) * f1;
+    s.b[2 * p] = b0;
+    s.b[2 * p + 1] = b1;
+    s.x[2 * p] = 0.0;
+    s.x[2 * p + 1] = 0.0;
+    bb += b0 * b0 + b1 * b1;
+    // diagonal block D_p = sum_k (Kn_pk + Kn_qk' + Tv_qk') n n^T + sum_c 1/2 (T_c s_c^T + s_c T_c^T)
+    const int x = p % s.nx, y = p / s.nx;
+    double m00 = 0, m01 = 0, m11 = 0, T[2][2] = {{0, 0}, {0, 0}}, S[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int q = nbr2(x, y, s.nx, s.ny, k);
+      if (q < 0) continue;
+      const double n0 = nd2<double>(k, 0), n1 = nd2<double>(k, 1);
+      const double cf = (double)s.kn[8 * p + k] + (double)s.kn[8 * q + (k ^ 1)] + (double)s.tv[8 * q + (k ^ 1)];
+      m00 += cf * n0 * n0;
+      m01 += cf * n0 * n1;
+      m11 += cf * n1 * n1;
+      const int c = shell2(k);
+      const double t = (double)s.tv[8 * p + k];
+      T[c][0] += t * n0;
+      T[c][1] += t * n1;
+      S[c][0] += n0;
+      S[c][1] += n1;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      m00 += T[c][0] * S[c][0];
+      m11 += T[c][1] * S[c][1];
+      m01 += 0.5 * (T[c][0] * S[c][1] + S[c][0] * T[c][1]);
+    }
+    const bool fr0 = s.mask[2 * p] != 0, fr1 = s.mask[2 * p + 1] != 0;
+    float i00 = 0.f, i01 = 0.f, i11 = 0.f;
+    if (fr0 && fr1) {
+      const double det = m00 * m11 - m01 * m01;
+      i00 = (float)(m11 / det);
+      i11 = (float)(m00 / det);
+      i01 = (float)(-m01 / det);
+    } else if (fr0) {
+      i00 = (float)(1.0 / m00);
+    } else if (fr1) {
+      i11 = (float)(1.0 / m11);
+    }
+    s.minv[3 * p] = i00;
+    s.minv[3 * p + 1] = i11;
+    s.minv[3 * p + 2] = i01;
+  }
+  wg_sum2(bb, unused, red);
+  s.bnorm = sqrt(bb);
+  s.state = s.bnorm > 0.0 ? SLOT_OUTER : SLOT_DONE;
+}
+
+// OUTER: true residual; either finish or start an inner solve
+__device__ void phase_outer(SlotDesc& s, double* red) {
+  const int N = s.nx * s.ny;
+  shell_pass<double>(s, s.x, s.w64);
+  __syncthreads();
+  double rr = 0.0, unused = 0.0;
+  for (int p = threadIdx.x; p < N; p += NT) {
+    double f0, f1;
+    ku_site<double>(s, s.x, s.w64, p, f0, f1);
+    const double r0 = (double)mfree(s, 2 * p) * (s.b[2 * p] - f0);
+    const double r1 = (double)mfree(s, 2 * p + 1) * (s.b[2 * p + 1] - f1);
+    s.r[2 * p] = (float)r0;
+    s.r[2 * p + 1] = (float)r1;
+    rr += r0 * r0 + r1 * r1;
+  }
+  wg_sum2(rr, unused, red);
+  s.rel = sqrt(rr) / s.bnorm;
+  if (s.rel <= s.tol || s.total >= s.maxit) {
+    s.state = s.rel <= s.tol ? SLOT_DONE : SLOT_FAILED;
+    return;
+  }
+  const double itol = fmin(0.5, fmax(s.inner_tol, 0.5 * s.tol / s.rel));
+  s.thr2 = itol * itol * rr;
+  double rz = 0.0;
+  unused = 0.0;
+  for (int p = threadIdx.x; p < N; p += NT) {
+    const float* m = s.minv + 3 * p;
+    const float r0 = s.r[2 * p], r1 = s.r[2 * p + 1];
+    const float z0 = m[0] * r0 + m[2] * r1, z1 = m[2] * r0 + m[1] * r1;
+    s.z[2 * p] = z0;
+    s.z[2 * p + 1] = z1;
+    s.p[2 * p] = z0;
+    s.p[2 * p + 1] = z1;
+    s.d[2 * p] = 0.f;
+    s.d[2 * p + 1] = 0.f;
+    rz += (double)(r0 * z0 + r1 * z1);
+  }
+  wg_sum2(rz, unused, red);
+  s.rz = (float)rz;
+  s.it = 0;
+  s.outer += 1;
+  s.state = SLOT_INNER;
+}
+
+// one inner PCG iteration (FP32)
+__device__ void inner_iteration(SlotDesc& s, float* red) {
+  const int N = s.nx * s.ny;
+  shell_pass<float>(s, s.p, s.w32);
+  __syncthreads();
+  float pap = 0.f, unused = 0.f;
+  for (int p = threadIdx.x; p < N; p += NT) {
+    float f0, f1;
+    ku_site<float>(s, s.p, s.w32, p, f0, f1);
+    f0 *= mfree(s, 2 * p);
+    f1 *= mfree(s, 2 * p + 1);
+    s.ap[2 * p] = f0;
+    s.ap[2 * p + 1] = f1;
+    pap += s.p[2 * p] * f0 + s.p[2 * p + 1] * f1;
+  }
+  wg_sum2(pap, unused, red);
+  if (!(pap > 0.f)) {
+    s.state = SLOT_FAILED;
+    return;
+  }
+  const float alpha = s.rz / pap;
+  float rz = 0.f, rr = 0.f;
+  for (int p = threadIdx.x; p < N; p += NT) {
+    const float* m = s.minv + 3 * p;
+    const int i = 2 * p;
+    s.d[i] += alpha * s.p[i];
+    s.d[i + 1] += alpha * s.p[i + 1];
+    const float r0 = s.r[i] - alpha * s.ap[i], r1 = s.r[i + 1] - alpha * s.ap[i + 1];
+    s.r[i] = r0;
+    s.r[i + 1] = r1;
+    const float z0 = m[0] * r0 + m[2] * r1, z1 = m[2] * r0 + m[1] * r1;
+    s.z[i] = z0;
+    s.z[i + 1] = z1;
+    rz += r0 * z0 + r1 * z1;
+    rr += r0 * r0 + r1 * r1;
+  }
+  wg_sum2(rz, rr, red);
+  s.it += 1;
+  s.total += 1;
+  if ((double)rr <= s.thr2 || s.total >= s.maxit) {
+    for (int p = threadIdx.x; p < 2 * N; p += NT) s.x[p] += (double)s.d[p];
+    __syncthreads();
+    s.state = SLOT_OUTER;
+    return;
+  }
+  const float beta = rz / s.rz;
+  s.rz = rz;
+  for (int i = threadIdx.x; i < 2 * N; i += NT) s.p[i] = s.z[i] + beta * s.p[i];
+  __syncthreads();
+}
+
+}  // namespace srv
+
+// One workgroup per slot; each advances its slot by at most `budget` site-iterations.  Slot descriptors live in
+// device memory; the workgroup keeps a private copy (identical in all threads) and thread 0 writes it back.
+__global__ __launch_bounds__(srv::NT) void serve_kernel(SlotDesc* __restrict__ slots, int nslots, long long budget) {
+  __shared__ double red64[2 * srv::NT / 64];
+  __shared__ float red32[2 * srv::NT / 64];
+  const int id = blockIdx.x;
+  if (id >= nslots) return;
+  SlotDesc s = slots[id];
+  if (s.state == SLOT_FREE || s.state == SLOT_DONE || s.state == SLOT_FAILED) return;
+  const long long N = (long long)s.nx * s.ny;
+  long long work = 0;
+  while (work < budget) {
+    if (s.state == SLOT_INIT) {
+      srv::phase_init(s, red64);
+      work += 3 * N;
+    } else if (s.state == SLOT_OUTER) {
+      srv::phase_outer(s, red64);
+      work += 3 * N;
+    } else if (s.state == SLOT_INNER) {
+      srv::inner_iteration(s, red32);
+      work += N;
+    } else {
+      break;
+    }
+  }
+  if (s.state == SLOT_DONE) {
+    for (int i = threadIdx.x; i < 2 * N; i += srv::NT) s.up[i] += s.x[i];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    SlotDesc* o = slots + id;
+    o->state = s.state;
+    o->it = s.it;
+    o->total = s.total;
+    o->outer = s.outer;
+    o->bnorm = s.bnorm;
+    o->rel = s.rel;
+    o->thr2 = s.thr2;
+    o->rz = s.rz;
+  }
+}
+
+}  // namespace lpm
