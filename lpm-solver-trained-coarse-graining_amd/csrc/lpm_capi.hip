@@ -88,6 +88,13 @@ int check_grid(const lpm_grid* g) {
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// per-thread non-blocking stream (default priority) used only to capture graphs
+hipStream_t capture_stream() {
+  thread_local hipStream_t cs = nullptr;
+  if (!cs && hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) cs = nullptr;
+  return cs;
+}
+
 // blocks per system of the grid-stride vector kernels: at most 1024, so the per-system partial sums stay short
 inline int vec_blocks(int64_t N) { return (int)std::min<int64_t>(1024, std::max<int64_t>(1, (N + 255) / 256)); }
 
@@ -348,7 +355,7 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   bool capturing = false;
-  auto body = [&]() -> int {
+  auto body = [&](hipStream_t sq) -> int {
     KuCall c{p, bvec, kn, tv, bc, mask, bm, nullptr, bvec, ap, bvec, partial, ist + I_ACTIVE};
     const bool prof = g_prof.on && !capturing;
     if (prof) {
@@ -357,18 +364,18 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
         HIPCHK(hipEventCreate(&e));
         g_prof.ev.push_back(e);
       }
-      HIPCHK(hipEventRecord(g_prof.ev[g_prof.used], st));
+      HIPCHK(hipEventRecord(g_prof.ev[g_prof.used], sq));
     }
-    int rc = ku_call<DIM, T, EPI_MASKDOT>(g, nb, nl, layout, kg, c, st);
+    int rc = ku_call<DIM, T, EPI_MASKDOT>(g, nb, nl, layout, kg, c, sq);
     if (rc) return rc;
     if (prof) {
-      HIPCHK(hipEventRecord(g_prof.ev[g_prof.used + 1], st));
+      HIPCHK(hipEventRecord(g_prof.ev[g_prof.used + 1], sq));
       g_prof.used += 2;
     }
-    hipLaunchKernelGGL((cg_fin_alpha_kernel<T>), dim3(nsys), dim3(256), 0, st, partial, (int)kg.grid.x, scal, ist);
-    hipLaunchKernelGGL((cg_update_kernel<D, T>), vgrid, dim3(256), 0, st, v);
-    hipLaunchKernelGGL((cg_fin_beta_kernel<T>), dim3(nsys), dim3(256), 0, st, partial, vblk, scal, ist);
-    hipLaunchKernelGGL((cg_pupdate_kernel<D, T>), vgrid, dim3(256), 0, st, v);
+    hipLaunchKernelGGL((cg_fin_alpha_kernel<T>), dim3(nsys), dim3(256), 0, sq, partial, (int)kg.grid.x, scal, ist);
+    hipLaunchKernelGGL((cg_update_kernel<D, T>), vgrid, dim3(256), 0, sq, v);
+    hipLaunchKernelGGL((cg_fin_beta_kernel<T>), dim3(nsys), dim3(256), 0, sq, partial, vblk, scal, ist);
+    hipLaunchKernelGGL((cg_pupdate_kernel<D, T>), vgrid, dim3(256), 0, sq, v);
     LAUNCHCHK();
     return LPM_OK;
   };
@@ -378,18 +385,22 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
     const int nrep = std::min(check, o->maxit - done_it);
     if (o->graph && nrep == check) {
       if (!gexec) {
-        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        // capture on a private stream (the caller's may be the null stream, which cannot capture); the graph is
+        // then launched on the caller's stream
+        hipStream_t cs = capture_stream();
+        if (!cs) return fail(LPM_E_HIP, "graph capture: no capture stream");
+        HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         capturing = true;
-        for (int i = 0; i < check && rc == LPM_OK; ++i) rc = body();
+        for (int i = 0; i < check && rc == LPM_OK; ++i) rc = body(cs);
         capturing = false;
-        hipError_t ec = hipStreamEndCapture(st, &graph);
+        hipError_t ec = hipStreamEndCapture(cs, &graph);
         if (rc) return rc;
         if (ec != hipSuccess) return fail(LPM_E_HIP, "graph capture: %s", hipGetErrorString(ec));
         HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
       }
       HIPCHK(hipGraphLaunch(gexec, st));
     } else {
-      for (int i = 0; i < nrep && rc == LPM_OK; ++i) rc = body();
+      for (int i = 0; i < nrep && rc == LPM_OK; ++i) rc = body(st);
       if (rc) return rc;
     }
     done_it += nrep;
@@ -874,6 +885,7 @@ struct Server {
   std::map<long long, hipGraphExec_t> graphs;
   hipGraph_t graph_keep[64];
   int ngraph = 0;
+  hipStream_t cap = nullptr;  // private stream for graph capture (the caller's may be the null stream)
 };
 
 size_t slot_bytes(int nx, int ny) {
@@ -931,6 +943,7 @@ int lpm_serve_create(int32_t max_slots, size_t arena_bytes, void** server) {
   hipError_t e = hipMalloc((void**)&s->arena, s->arena_bytes);
   if (e == hipSuccess) e = hipMalloc((void**)&s->d_slots, sizeof(SlotDesc) * max_slots);
   if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_slots, sizeof(SlotDesc) * max_slots, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking);
   if (e == hipSuccess) {
     std::memset(s->h_slots, 0, sizeof(SlotDesc) * max_slots);
     e = hipMemcpy(s->d_slots, s->h_slots, sizeof(SlotDesc) * max_slots, hipMemcpyHostToDevice);
@@ -939,6 +952,7 @@ int lpm_serve_create(int32_t max_slots, size_t arena_bytes, void** server) {
     if (s->arena) (void)hipFree(s->arena);
     if (s->d_slots) (void)hipFree(s->d_slots);
     if (s->h_slots) (void)hipHostFree(s->h_slots);
+    if (s->cap) (void)hipStreamDestroy(s->cap);
     delete s;
     return fail(LPM_E_HIP, "lpm_serve_create: %s", hipGetErrorString(e));
   }
@@ -958,6 +972,7 @@ void lpm_serve_destroy(void* server) {
   (void)hipFree(s->arena);
   (void)hipFree(s->d_slots);
   (void)hipHostFree(s->h_slots);
+  (void)hipStreamDestroy(s->cap);
   delete s;
 }
 
@@ -1037,9 +1052,9 @@ int lpm_serve_step(void* server, int64_t budget, int32_t use_graph, void* stream
   if (it == s->graphs.end()) {
     if (s->ngraph >= 64) return fail(LPM_E_BADARG, "lpm_serve_step: too many distinct budgets");
     hipGraph_t gr;
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(serve_kernel, grid, block, 0, st, s->d_slots, s->max_slots, (long long)budget);
-    HIPCHK(hipStreamEndCapture(st, &gr));
+    HIPCHK(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(serve_kernel, grid, block, 0, s->cap, s->d_slots, s->max_slots, (long long)budget);
+    HIPCHK(hipStreamEndCapture(s->cap, &gr));
     hipGraphExec_t ex;
     HIPCHK(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
     s->graph_keep[s->ngraph++] = gr;

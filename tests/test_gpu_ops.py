@@ -228,3 +228,22 @@ def test_post_vjp_packed_and_fp32(dev, dims):
                        g_ku=t(gP, torch.float32), g_stress=t(gS, torch.float32), g_energy=t(gU, torch.float32))
     for a, b in zip(f32, ref):
         assert rel(a, b.cpu().numpy()) < 2e-5
+
+
+@pytest.mark.parametrize("dims", [(10, 9, 8), (14, 12)])
+def test_pcg_graph_replay_matches_eager(dev, dims):
+    """The check-iteration block replayed as a captured hipGraph (many replays per solve, with the host's flag
+    polls between them) gives the eager iteration bit for bit, twice in a row."""
+    lat, Kn, Tv, _ = case(dims, seed=31, two_phase=True)
+    fixed, up, _ = (R.prescribed3("k_xy", lat.x, lat.ijk, dims, 1) if lat.dim == 3 else
+                    R.prescribed2("ring_xy", lat.x, lat.ijk, dims[0], dims[1], 1, 1.0))
+    b = -R.apply(lat, up * fixed[:, None], Kn, Tv) * (~fixed)[:, None]
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    free = torch.as_tensor(~fixed, device=dev)
+    minv = ops.block_jacobi(dims, 1.0, t(Kn), t(Tv), free=free)
+    xe, ie = ops.pcg(dims, 1.0, t(b), t(Kn), t(Tv), minv=minv, free=free, tol=1e-12, maxit=5000, check=5)
+    for _ in range(2):
+        xg, ig = ops.pcg(dims, 1.0, t(b), t(Kn), t(Tv), minv=minv, free=free, tol=1e-12, maxit=5000, check=5,
+                         graph=True)
+        assert ig.it == ie.it and ie.it > 20
+        assert torch.equal(xg, xe)
