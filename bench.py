@@ -148,6 +148,245 @@ def cpu_baseline(P, gpu_iters, budget_s=20.0):
             "sec_per_iter": t_it}
 
 
+# ------------------------------------------------------------------------------------------------ other workloads
+def _oracle_lib():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import build_oracle
+    if not os.path.exists(build_oracle.LIB):
+        build_oracle.build()
+    lib = ctypes.CDLL(build_oracle.LIB)
+    Pp = ctypes.c_void_p
+    lib.orc_cg2.argtypes = [ctypes.c_int64, Pp, Pp, Pp, Pp, Pp, Pp, ctypes.c_double, ctypes.c_int, Pp]
+    lib.orc_cg2.restype = ctypes.c_int
+    lib.orc_pcg3.argtypes = [ctypes.c_int64, Pp, Pp, Pp, Pp, Pp, Pp, Pp, ctypes.c_double, ctypes.c_int, Pp]
+    lib.orc_pcg3.restype = ctypes.c_int
+    lib.orc_threads.restype = ctypes.c_int
+    return lib
+
+
+def _line(metric, value, unit, steps, warmup, ms, dtype, workload, extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": {"workload": workload}}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def bench_plain2d(a):
+    """128^2 plain 2D LPM, FP64 unpreconditioned CG to 1e-8: device-driven batched solver, the host-driven plain
+    GPU CG, and the C restatement of cg2plain.cg_mf on the host cores."""
+    import torch
+    import lpm_ops as ops
+    import _lpmlib as L
+    import serve
+    import workloads
+    dev = torch.device("cuda:0")
+    P = workloads.plain2d(128, a.seed, dev)
+    dims = P["dims"]
+    kp, tp, _ = ops.pack(dims, P["Kn"], P["Tv"])
+    mfree = P["mask"]
+    b = -ops.apply(dims, 1.0, P["up"], P["Kn"], P["Tv"]) * mfree.double()
+
+    def gpu_solve():
+        return ops.pcg(dims, 1.0, b, kp, tp, minv=None, free=mfree, tol=a.tol, maxit=200000, check=50,
+                       layout=L.PACKED)
+    for _ in range(a.warmup):
+        gpu_solve()
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        x, info = gpu_solve()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    req = serve.Request(128, 128, 1.0, P["Kn"], P["Tv"], mfree, P["up"], "tension")
+    t0 = time.perf_counter()
+    up_plain, it_plain = serve.plain_cg(req, a.tol)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    diff = float((up_plain - (P["up"] + x)).abs().max() / (P["up"] + x).abs().max())
+    lib = _oracle_lib()
+    lat2 = P["lat"]
+    nbr = np.ascontiguousarray(lat2.nbr.astype(np.int64))
+    bb = np.ascontiguousarray(b.cpu().numpy().ravel())
+    xx = np.zeros_like(bb)
+    m = np.ascontiguousarray(P["mask_np"].astype(np.float64).ravel())
+    rel = np.zeros(1)
+    pt = lambda v: v.ctypes.data_as(ctypes.c_void_p)
+    t0 = time.perf_counter()
+    it_cpu = lib.orc_cg2(lat2.N, pt(nbr), pt(np.ascontiguousarray(P["Kn_np"])), pt(np.ascontiguousarray(P["Tv_np"])),
+                         pt(m), pt(bb), pt(xx), a.tol, 200000, pt(rel))
+    t_cpu = time.perf_counter() - t0
+    lat_ms = np.array(lat) * 1e3
+    _line(METRIC, a.steps / sum(lat), "solves/s", a.steps, a.warmup, float(lat_ms.mean()), "f64",
+          "plain 2D LPM 128^2 (32768 DOF), disks phi 0.30 contrast 10, unpreconditioned CG to rel-res %g" % a.tol,
+          {"latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+           "solve": {"cg_iters": info.it, "relres": float(info.relmax), "phase_fraction": P["phase_fraction"]},
+           "plain_gpu_cg": {"solves_per_s": 1.0 / t_plain, "iters": it_plain, "max_rel_diff": diff,
+                            "note": "host-driven alpha/beta, one solve, FP64, same HIP K u"},
+           "cpu_baseline": {"value": 1.0 / t_cpu, "unit": "solves/s", "cores": int(lib.orc_threads()), "kind": "port",
+                            "sample": "full solve, %d CG iterations (C/OpenMP restatement of timing/t2d/cg2plain.py:"
+                                      "11-49, FP64)" % it_cpu}})
+
+
+def bench_kubc3d(a):
+    """48^3 KUBC, nine right-hand sides in one batched FP32 PCG + FP64 refinement to 1e-8."""
+    import torch
+    import lpm_ops as ops
+    import _lpmlib as L
+    import synth
+    import workloads
+    dev = torch.device("cuda:0")
+    P = workloads.kubc3d(48, a.seed, dev)
+    dims = P["dims"]
+    N = 48 ** 3
+
+    def step():
+        b = -ops.apply(dims, 1.0, P["up"], P["kp64"], P["tp64"], layout=L.PACKED) * P["free"][:, None]
+        M = ops.block_jacobi(dims, 1.0, P["kp64"], P["tp64"], free=P["free"], out_dtype=torch.float32,
+                             layout=L.PACKED)
+        return ops.pcg_refined(dims, 1.0, b, P["kp64"], P["tp64"], P["kp32"], P["tp32"], minv32=M, free=P["free"],
+                               tol=a.tol, maxit=100000, inner_tol=1e-4, layout64=L.PACKED, layout32=L.PACKED)
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    L.lib().lpm_profile_ku(1)
+    lat = []
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        x, info = step()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    L.lib().lpm_profile_ku_read(ctypes.byref(ms), ctypes.byref(cnt))
+    L.lib().lpm_profile_ku(0)
+    ku_ms = ms.value / max(cnt.value, 1)
+    ku_bytes = N * (2 * 3 * 9 * 4 + 18 * 4)
+    u = P["up"] + x
+    sup = 0.0
+    for j, (p, q) in enumerate(workloads.COMBINED):
+        c = u[6 + j]
+        sup = max(sup, float((c - u[p] - u[q]).abs().max() / c.abs().max()))
+    # CPU: one right-hand side of the C block-Jacobi PCG for a bounded number of iterations
+    lib = _oracle_lib()
+    Kn, Tv = synth.directed_from_packed(dims, P["kp64"], P["tp64"])
+    Mf = ops.full_from_sym(ops.block_jacobi(dims, 1.0, P["kp64"], P["tp64"], free=P["free"], layout=L.PACKED))
+    b0 = (-ops.apply(dims, 1.0, P["up"][0], P["kp64"], P["tp64"], layout=L.PACKED) * P["free"][:, None])
+    nbr = np.ascontiguousarray(P["lat"].nbr.astype(np.int64))
+    arrs = [np.ascontiguousarray(t.cpu().numpy()) for t in (Kn, Tv, Mf, b0)]
+    fr = np.ascontiguousarray(P["fixed"] == False).astype(np.uint8)  # noqa: E712
+    xx = np.zeros_like(arrs[3])
+    rel = np.zeros(1)
+    pt = lambda v: v.ctypes.data_as(ctypes.c_void_p)
+    its = 200
+    t0 = time.perf_counter()
+    done = lib.orc_pcg3(N, pt(nbr), pt(arrs[0]), pt(arrs[1]), pt(arrs[2]), pt(fr), pt(arrs[3]), pt(xx), 0.0, its,
+                        pt(rel))
+    t_it = (time.perf_counter() - t0) / max(done, 1)
+    per_rhs_iters = info.iters.max()
+    t_cpu = t_it * float(info.iters.sum())
+    lat_ms = np.array(lat) * 1e3
+    _line(METRIC, 9 * a.steps / sum(lat), "RHS solves/s", a.steps, a.warmup, float(lat_ms.mean()),
+          "f32 inner + f64 refinement",
+          "SC LPM 48^3, spheres phi 0.25 contrast 10, 9 KUBC right-hand sides in one batched block-Jacobi PCG, "
+          "FP32 + FP64 refinement to rel-res %g" % a.tol,
+          {"latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+           "solve": {"inner_iters_max": int(per_rhs_iters), "outer": int(np.max(info.outer)),
+                     "relres_fp64": float(info.relmax), "superposition_rel_err": sup,
+                     "phase_fraction": P["phase_fraction"]},
+           "ku": {"avg_us": ku_ms * 1e3, "launches": cnt.value, "alg_bytes_per_launch": ku_bytes,
+                  "achieved_GBs": ku_bytes / (ku_ms * 1e-3) / 1e9,
+                  "note": "31.9 MB per batched K u: cache resident, reported as time"},
+           "cpu_baseline": {"value": 9.0 / t_cpu, "unit": "RHS solves/s", "cores": int(lib.orc_threads()),
+                            "kind": "port", "sample": "%d block-Jacobi PCG iterations of one RHS (FP64 C/OpenMP "
+                            "restatement of coarse3d/lin3d.py:464-518), %.4f s/iter, scaled by the GPU's total "
+                            "inner iterations over the 9 RHS" % (done, t_it)}})
+
+
+def bench_serve(a):
+    """Continuous batching of 512 coarse 2D requests (24^2-144^2, FP32 PCG + FP64 refinement to 1e-8)."""
+    import torch
+    import serve
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    reqs = serve.make_requests(a.requests, a.seed, dev)
+    print("[serve] %d requests built in %.1f s" % (len(reqs), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    # plain per-solve GPU CG on a sample
+    nplain = min(a.plain_sample, len(reqs))
+    t0 = time.perf_counter()
+    for r in reqs[:nplain]:
+        serve.plain_cg(r, a.tol)
+    torch.cuda.synchronize()
+    plain_rate = nplain / (time.perf_counter() - t0)
+    print("[serve] plain GPU CG: %.1f solves/s on %d requests" % (plain_rate, nplain), file=sys.stderr, flush=True)
+    srv = serve.Server(max_slots=a.slots, device=dev, budget=a.budget)
+    serve.run_stream(srv, reqs[:min(64, len(reqs))])                       # warm-up (graph capture, clocks)
+    sat = serve.run_stream(srv, reqs)
+    print("[serve] saturation: %.1f solves/s" % sat["solves_per_s"], file=sys.stderr, flush=True)
+    rate = a.load * sat["solves_per_s"]
+    arr = serve.poisson_arrivals(len(reqs), rate, a.seed + 1)
+    pois = serve.run_stream(srv, reqs, arr)
+    print("[serve] poisson %.1f/s: p50 %.2f ms p99 %.2f ms" % (rate, pois["p50_ms"], pois["p99_ms"]),
+          file=sys.stderr, flush=True)
+    srv.close()
+    sizes = [r.nx for r in reqs]
+    _line(METRIC, sat["solves_per_s"], "solves/s", len(reqs), 64, sat["wall_s"] / len(reqs) * 1e3,
+          "f32 inner + f64 refinement",
+          "serving: %d coarse 2D requests %d^2-%d^2, FP32 PCG + FP64 refinement to rel-res %g, continuous batching"
+          % (len(reqs), min(sizes), max(sizes), a.tol),
+          {"latency_ms": {"p50": pois["p50_ms"], "p99": pois["p99_ms"], "max": pois["max_ms"],
+                          "arrival_rate_per_s": rate, "load": a.load},
+           "saturation": sat, "poisson": pois,
+           "plain_gpu_cg": {"solves_per_s": plain_rate, "sample": nplain,
+                            "speedup": sat["solves_per_s"] / plain_rate},
+           "slots": a.slots, "budget": a.budget})
+
+
+def bench_train3d(a):
+    """Differentiable coarse 3D training step: 16 specimens x 9 loads on 24^3, forward PCG + adjoint + Adam."""
+    import torch
+    import torch_lpm3d as T3
+    import workloads
+    dev = torch.device("cuda:0")
+    W = workloads.train3d(16, 48, 2, a.seed, dev)
+    model = T3.CoarseT(W["nc"], W["s"], device=dev, dtype=torch.float64, mixed=True)
+    Kn0, Tv0 = W["Kn"].double(), W["Tv"].double()
+    sym = T3.SymScatter(model.lat, dev)
+    theta = torch.zeros((16, model.N, 9), device=dev, dtype=torch.float64, requires_grad=True)
+    with torch.no_grad():
+        g = torch.Generator(device="cpu").manual_seed(a.seed)
+        pert = sym(0.1 * torch.randn((16, model.N, 9), generator=g, dtype=torch.float64).to(dev))
+        tgt = model.run(Kn0 * torch.exp(pert), Tv0, tol=a.tol, differentiable=False)
+    opt = torch.optim.Adam([theta], lr=1e-2)
+
+    def step():
+        opt.zero_grad()
+        Kn = Kn0 * torch.exp(sym(theta))
+        o = model.run(Kn, Tv0, tol=a.tol)
+        loss = ((o["u"] - tgt["u"]) ** 2).mean() / (tgt["u"] ** 2).mean() + \
+            ((o["sig"] - tgt["sig"]) ** 2).mean() / (tgt["sig"] ** 2).mean()
+        loss.backward()
+        opt.step()
+        return float(loss)
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    lat, losses = [], []
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        losses.append(step())
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    lat_ms = np.array(lat) * 1e3
+    _line("training steps/s (16 specimens x 9 loads, forward + adjoint)", a.steps / sum(lat), "steps/s", a.steps,
+          a.warmup, float(lat_ms.mean()), "f32 inner + f64 refinement",
+          "coarse SC 24^3 x 16 specimens x 9 loads, forward refined PCG + adjoint PCG + gradient kernels + Adam",
+          {"latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+           "loss": losses, "solves_per_step": 2 * 16 * 9})
+
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +399,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--probe", type=int, default=0, help="run the profiler probe on an n^3 lattice and exit")
     ap.add_argument("--probe-reps", type=int, default=10)
+    ap.add_argument("--config", default="cube256", choices=["cube256", "plain2d", "kubc3d", "serve", "train3d"])
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--slots", type=int, default=512)
+    ap.add_argument("--budget", type=int, default=400000)
+    ap.add_argument("--load", type=float, default=0.8)
+    ap.add_argument("--plain-sample", type=int, default=16)
     a = ap.parse_args()
     if a.probe:
         probe(a.probe, a.probe_reps)
@@ -168,6 +413,9 @@ def main():
         print("bench.py measures one GPU: every box of this build has one MI355X and the solve path runs on a "
               "single device (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if a.config != "cube256":
+        {"plain2d": bench_plain2d, "kubc3d": bench_kubc3d, "serve": bench_serve, "train3d": bench_train3d}[a.config](a)
+        return
     import torch
     dev = torch.device("cuda:0")
     P = build_problem(a.n, a.seed, dev)

@@ -170,7 +170,7 @@ def coarse_request(fine, s, phi, corr, seed, device, load=None, contrast=10.0, h
     rng = np.random.default_rng(seed)
     img = synth.grf_two_phase(fine, fine, corr, phi, seed).reshape(fine, fine)
     E = np.where(img > 0, contrast, 1.0)
-    C = geom2d.iso_C(E, np.full(E.shape, 0.25))                               # (fine, fine, 3)
+    C = geom2d.iso_C(E, np.full(E.shape, 0.2))                                # (fine, fine, 3)
     nc = fine // s
     Cc = C[:nc * s, :nc * s].reshape(nc, s, nc, s, 3).mean(axis=(1, 3)).reshape(-1, 3)  # Voigt mean per cell
     lat = geom2d.SquareGeom(nc, nc, s * h)

@@ -146,6 +146,28 @@ class LinSolve(torch.autograd.Function):
         return gK, gT, None, None, None, None
 
 
+class SymScatter:
+    """Values per undirected bond (..., N, NN/2), one per forward slot 2k, -> symmetric directed (..., N, NN): slot
+    2k at p takes (p, k), slot 2k+1 at p takes (q, k) of its neighbour q, for which p sits in slot 2k; zero on
+    missing bonds.  Differentiable (a gather), as the reference's scatter of bond parameters into both directed
+    slots (coarse3d/train3d.py:77-97, coarse2d/train_cg.py:80-91)."""
+
+    def __init__(self, lat, device):
+        N, NN = lat.N, lat.NN
+        h = NN // 2
+        idx = np.zeros((N, NN), np.int64)
+        for k in range(h):
+            idx[:, 2 * k] = np.arange(N) * h + k
+            q = lat.nbr[:, 2 * k + 1]
+            idx[:, 2 * k + 1] = np.where(q >= 0, q * h + k, 0)
+        self.idx = torch.as_tensor(idx, device=device)
+        self.valid = torch.as_tensor(lat.valid, device=device)
+
+    def __call__(self, v):
+        flat = v.reshape(v.shape[:-2] + (-1,))
+        return flat[..., self.idx] * self.valid.to(v.dtype)
+
+
 class LoadSet:
     """Boundary conditions of the load cases on a lattice whose particles belong to blocks IJK."""
 
