@@ -20,6 +20,8 @@
 //    the neighbour's plane entry is read at p + delta without a bounds select; no branches in the slot loops;
 //  * loads use 32-bit byte offsets from a uniform base.
 #pragma once
+#include <utility>
+
 #include "lpm_common.hpp"
 
 namespace lpm {
@@ -73,6 +75,12 @@ __device__ __forceinline__ T ld_off(const T* __restrict__ base, O byte_off) {
   return *(const T*)((const char*)base + byte_off);
 }
 
+// the three displacement components of a site, fetched by one vector load
+template <typename T>
+struct Vec3 {
+  T x, y, z;
+};
+
 // dL for axial slots; e = sqrt2 dL for diagonal slots (u_q from the LDS plane)
 template <typename T, int K, int HY, int HX>
 __device__ __forceinline__ T dle3(const T (*uq)[HY][HX], int hy, int hx, T ux, T uy, T uz) {
@@ -94,32 +102,35 @@ __device__ __forceinline__ void for_slots(Seq<Ks...>, F&& f) {
   (f(std::integral_constant<int, Ks>{}), ...);
 }
 
-// the 18 directed coefficients of particle p (in-box); directed: own row; packed: plane entry at p or p + delta
+// The 18 directed coefficients of a site of one plane.  `zbase` is the plane's first particle (a uniform address,
+// directed: row 0 of the plane; packed: plane-0 entry of the plane's first particle) and `xy` the site's in-plane
+// index, which stays the same for every plane of a z march: the loads take one loop-invariant lane offset and a
+// uniform per-slot base (packed odd slots: the neighbour's forward-plane entry at p + delta, inside the zero guards).
 template <typename T, typename C, int MODE>
-__device__ __forceinline__ void load18(const C* __restrict__ base, int64_t ps, int p, int nx, int nxy, T (&out)[18]) {
+__device__ __forceinline__ void load18(const C* __restrict__ zbase, int64_t ps, unsigned xy, int nx, int nxy,
+                                       T (&out)[18]) {
   for_slots(Slots18{}, [&](auto kc) {
     constexpr int K = decltype(kc)::value;
     if constexpr (MODE == COEF_DIRECTED) {
-      out[K] = (T)ld_off(base, ((unsigned)p * 18u + K) * (unsigned)sizeof(C));  // < 4 GiB (host check)
+      out[K] = (T)ld_off(zbase, xy * (18u * (unsigned)sizeof(C)) + K * (unsigned)sizeof(C));  // < 4 GiB (host check)
     } else {
-      // offsets from the plane's first guard (G = nxy + nx + 1 before particle 0) are never negative, so the load
-      // takes a uniform base and an unsigned 32-bit offset
-      const int G = nxy + nx + 1;
-      const unsigned q = (unsigned)(p + G + ((K & 1) ? off3(K, 0) + off3(K, 1) * nx + off3(K, 2) * nxy : 0));
-      out[K] = (T)ld_off(base + ((K >> 1) * ps - G), q * (unsigned)sizeof(C));
+      const int64_t sh = (int64_t)(K >> 1) * ps + ((K & 1) ? off3(K, 0) + off3(K, 1) * nx + off3(K, 2) * nxy : 0);
+      out[K] = (T)ld_off(zbase + sh, xy * (unsigned)sizeof(C));
     }
   });
 }
 
 template <typename T, typename C, int MODE, int TX, int TY, int EPI>
-__global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 2 : 1) void ku3d_kernel(Ku3Args<T, C> a) {
+__global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1) void ku3d_kernel(Ku3Args<T, C> a) {
   constexpr int HX = TX + 4, HY = TY + 4;  // u tile with a 2-site halo
   constexpr int SX = TX + 2, SY = TY + 2;  // shell sums with a 1-site halo
   constexpr int NT = TX * TY;
   constexpr int NR = 2 * SX + 2 * TY;           // ring of halo sites of the shell sums
   constexpr int NLD = (HX * HY + NT - 1) / NT;  // u sites loaded per thread per plane
   static_assert(NR <= NT, "one ring site per thread at most");
-  __shared__ T su[5][3][HY][HX];
+  // LDS rings indexed by the plane's offset from zb modulo 6 (u) and 3 (shell sums): with the z loop unrolled by 6
+  // every ring slot of a step is a compile-time constant
+  __shared__ T su[6][3][HY][HX];
   __shared__ T ss[3][4][SY][SX];  // S0, sqrt2 S1, TS0, sqrt2 TS1
   __shared__ T red[NT / 64];
 
@@ -150,7 +161,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 2 : 1) void ku3d_kernel(Ku
   const bool own_xy = gxo < nx && gyo < ny;
   const int cgx = cl(gxo, nx), cgy = cl(gyo, ny);
   const bool omx = cgx > 0, opx = cgx < nx - 1, omy = cgy > 0, opy = cgy < ny - 1;
-  const int own_xy_off = cgy * nx + cgx;
+  const unsigned own_xy_off = (unsigned)(cgy * nx + cgx);
   // ring site (threads < NR): shell-sum coordinates (rsx, rsy)
   const bool has_r = tid < NR;
   int rsx, rsy;
@@ -170,33 +181,44 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 2 : 1) void ku3d_kernel(Ku
   }
   const int rgx = cl(x0 - 1 + rsx, nx), rgy = cl(y0 - 1 + rsy, ny);
   const bool rmx = rgx > 0, rpx = rgx < nx - 1, rmy = rgy > 0, rpy = rgy < ny - 1;
-  const int r_xy_off = rgy * nx + rgx;
+  const unsigned r_xy_off = (unsigned)(rgy * nx + rgx);
+
+  // u halo sites of this thread: in-plane byte offsets (loop invariant) and LDS positions
+  unsigned uoff[NLD];
+#pragma unroll
+  for (int j = 0; j < NLD; ++j) {
+    const int idx = min(tid + j * NT, HX * HY - 1);
+    const int hx = idx % HX, hy = idx / HX;
+    uoff[j] = (unsigned)((cl(y0 - 2 + hy, ny) * nx + cl(x0 - 2 + hx, nx)) * 3 * (int)sizeof(T));
+  }
+  // uniform bases of a plane (clamped into the box)
+  constexpr int CROW = MODE == COEF_DIRECTED ? 18 : 1;
+  auto kn_z = [&](int z) { return kn + (int64_t)cl(z, nz) * nxy * CROW; };
+  auto tv_z = [&](int z) { return tv + (int64_t)cl(z, nz) * nxy * CROW; };
 
   T ureg[NLD][3];
-  T dC[18], dN[18];  // own elongations of the output plane and of the next shell plane
-  T sC[4], sN[4];    // own shell sums, same two planes
-  T tvC[18], knC[18], tvN[18], tvR[18];
-  // FP32: the next step's coefficients are loaded a full step ahead; FP64 has no registers for a second set and
-  // loads them after the output phase
+  // FP32: the coefficients of the next steps are loaded a full step ahead; FP64 has no registers for a second set
+  // and loads them after the output phase
   constexpr bool DEEP = sizeof(T) == 4;
-  constexpr int ND = DEEP ? 18 : 1;
-  T tvNN[ND], knN[ND], tvRN[ND];
+  // register banks whose roles rotate with the step (the z loop is unrolled by the rotation period, so the
+  // rotation is renaming): own elongations / shell sums of the output plane and of the next plane; own Tv of the
+  // output plane, the next plane and (FP32) the plane after; own Kn of the output plane and (FP32) the next; ring
+  // Tv of the next plane and (FP32) the plane after
+  constexpr int NTV = DEEP ? 3 : 2, NKB = DEEP ? 2 : 1;
+  T DD[2][18], SS[2][4], TV[NTV][18], KN[1][18], TR[NKB][18];
 
   auto fetch_u = [&](int z) {
-    const int zo = cl(z, nz) * nxy;
+    const T* uz = u + (int64_t)cl(z, nz) * nxy * 3;
 #pragma unroll
     for (int j = 0; j < NLD; ++j) {
-      const int idx = min(tid + j * NT, HX * HY - 1);
-      const int hx = idx % HX, hy = idx / HX;
-      const int p = zo + cl(y0 - 2 + hy, ny) * nx + cl(x0 - 2 + hx, nx);
-      const int o = p * (int)(3 * sizeof(T));
-      ureg[j][0] = ld_off(u, o);
-      ureg[j][1] = ld_off(u, o + (int)sizeof(T));
-      ureg[j][2] = ld_off(u, o + 2 * (int)sizeof(T));
+      const Vec3<T> v = ld_off((const Vec3<T>*)uz, uoff[j]);
+      ureg[j][0] = v.x;
+      ureg[j][1] = v.y;
+      ureg[j][2] = v.z;
     }
   };
-  auto store_u = [&](int z) {
-    T(*dst)[HY][HX] = su[(z + 10) % 5];
+  auto store_u = [&](int slot) {
+    T(*dst)[HY][HX] = su[slot];
 #pragma unroll
     for (int j = 0; j < NLD; ++j) {
       const int idx = tid + j * NT;
@@ -209,16 +231,18 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 2 : 1) void ku3d_kernel(Ku
     }
   };
 
-  // Shell sums of the site at shell coordinates (sx, sy) in plane z, in-plane flags, Tv of the site; optionally
-  // keeps the 18 elongations.  Writes S0, sqrt2 S1, TS0, sqrt2 TS1 to the LDS ring and returns them in s.
-  auto shell_site = [&](auto keep_tag, int z, int sx, int sy, bool mx, bool px, bool my, bool py,
+  // Shell sums of the site at shell coordinates (sx, sy) in plane z (R = (z - zb) mod 6), in-plane flags, Tv of
+  // the site; optionally keeps the 18 elongations.  Writes S0, sqrt2 S1, TS0, sqrt2 TS1 to the LDS ring and
+  // returns them in s.
+  auto shell_site = [&](auto keep_tag, auto rel_tag, int z, int sx, int sy, bool mx, bool px, bool my, bool py,
                         const T (&tvv)[18], T (&keep)[18], T (&s)[4]) {
     constexpr bool KEEP = decltype(keep_tag)::value;
+    constexpr int R = decltype(rel_tag)::value;
     const int zc_ = cl(z, nz);
     const bool mz = zc_ > 0, pz = zc_ < nz - 1;
-    const T(*u0)[HY][HX] = su[(z + 9) % 5];   // plane z-1
-    const T(*u1)[HY][HX] = su[(z + 10) % 5];  // plane z
-    const T(*u2)[HY][HX] = su[(z + 11) % 5];  // plane z+1
+    const T(*u0)[HY][HX] = su[(R + 5) % 6];  // plane z-1
+    const T(*u1)[HY][HX] = su[R];            // plane z
+    const T(*u2)[HY][HX] = su[(R + 1) % 6];  // plane z+1
     const int hx = sx + 1, hy = sy + 1;
     const T ux = u1[0][hy][hx], uy = u1[1][hy][hx], uz = u1[2][hy][hx];
     T s0 = T(0), s1 = T(0), t0 = T(0), t1 = T(0);
@@ -237,7 +261,7 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 2 : 1) void ku3d_kernel(Ku
         t1 += tvv[K] * d;
       }
     });
-    T(*dst)[SY][SX] = ss[(z + 9) % 3];
+    T(*dst)[SY][SX] = ss[R % 3];
     dst[0][sy][sx] = s0;
     dst[1][sy][sx] = s1;
     dst[2][sy][sx] = t0;
@@ -247,60 +271,76 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 2 : 1) void ku3d_kernel(Ku
     s[2] = t0;
     s[3] = t1;
   };
-  auto own_p = [&](int z) { return cl(z, nz) * nxy + own_xy_off; };
-  auto ring_p = [&](int z) { return cl(z, nz) * nxy + r_xy_off; };
 
   // ---- prologue: u planes zb-2 .. zb+2 in LDS; shell sums of planes zb-1 and zb
   for (int z = zb - 2; z <= zb + 2; ++z) {
     fetch_u(z);
-    store_u(z);
+    store_u((z - zb + 6) % 6);
   }
   __syncthreads();
   using Keep = std::integral_constant<bool, true>;
   using Drop = std::integral_constant<bool, false>;
   {
     T tmp[4];
-    load18<T, C, MODE>(tv, ps, own_p(zb - 1), nx, nxy, tvN);
-    load18<T, C, MODE>(tv, ps, ring_p(zb - 1), nx, nxy, tvR);
-    shell_site(Drop{}, zb - 1, tx + 1, ty + 1, omx, opx, omy, opy, tvN, dN, tmp);
-    if (has_r) shell_site(Drop{}, zb - 1, rsx, rsy, rmx, rpx, rmy, rpy, tvR, dN, tmp);
-    load18<T, C, MODE>(tv, ps, own_p(zb), nx, nxy, tvC);
-    load18<T, C, MODE>(tv, ps, ring_p(zb), nx, nxy, tvR);
-    shell_site(Keep{}, zb, tx + 1, ty + 1, omx, opx, omy, opy, tvC, dC, sC);
-    if (has_r) shell_site(Drop{}, zb, rsx, rsy, rmx, rpx, rmy, rpy, tvR, dN, tmp);
-    load18<T, C, MODE>(kn, ps, own_p(zb), nx, nxy, knC);
-    load18<T, C, MODE>(tv, ps, own_p(zb + 1), nx, nxy, tvN);
-    load18<T, C, MODE>(tv, ps, ring_p(zb + 1), nx, nxy, tvR);
+    load18<T, C, MODE>(tv_z(zb - 1), ps, own_xy_off, nx, nxy, TV[1]);
+    load18<T, C, MODE>(tv_z(zb - 1), ps, r_xy_off, nx, nxy, TR[0]);
+    using Rm1 = std::integral_constant<int, 5>;
+    using R0 = std::integral_constant<int, 0>;
+    shell_site(Drop{}, Rm1{}, zb - 1, tx + 1, ty + 1, omx, opx, omy, opy, TV[1], DD[1], tmp);
+    if (has_r) shell_site(Drop{}, Rm1{}, zb - 1, rsx, rsy, rmx, rpx, rmy, rpy, TR[0], DD[1], tmp);
+    load18<T, C, MODE>(tv_z(zb), ps, own_xy_off, nx, nxy, TV[0]);
+    load18<T, C, MODE>(tv_z(zb), ps, r_xy_off, nx, nxy, TR[0]);
+    shell_site(Keep{}, R0{}, zb, tx + 1, ty + 1, omx, opx, omy, opy, TV[0], DD[0], SS[0]);
+    if (has_r) shell_site(Drop{}, R0{}, zb, rsx, rsy, rmx, rpx, rmy, rpy, TR[0], DD[1], tmp);
+    if constexpr (DEEP) {
+      load18<T, C, MODE>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
+      load18<T, C, MODE>(tv_z(zb + 1), ps, r_xy_off, nx, nxy, TR[0]);
+    }
   }
   __syncthreads();
 
   T acc = T(0);
   const int hx = tx + 2, hy = ty + 2, sx = tx + 1, sy = ty + 1;
-  for (int z = zb; z < ze; ++z) {
+  const int64_t obase = a.bo.at(b, l), mbase = a.bm.at(b, l), rbase = a.brhs.at(b, l);
+  const unsigned oxy = (unsigned)(gyo * nx + gxo);
+
+  // one step of the march: shell sums of plane z+1, barrier, output of plane z.  J = step index modulo the
+  // rotation period selects the banks.
+  auto step = [&](auto jc, int z) __attribute__((always_inline)) {
+    constexpr int J = decltype(jc)::value;
+    T(&dC)[18] = DD[J % 2];
+    T(&dN)[18] = DD[(J + 1) % 2];
+    T(&sC)[4] = SS[J % 2];
+    T(&sN)[4] = SS[(J + 1) % 2];
+    T(&tvC)[18] = TV[J % NTV];
+    T(&tvN)[18] = TV[(J + 1) % NTV];
+    T(&knC)[18] = KN[0];
+    T(&tvR)[18] = TR[J % NKB];
     fetch_u(z + 3);  // lands during this step
     if constexpr (DEEP) {
-      load18<T, C, MODE>(tv, ps, own_p(z + 2), nx, nxy, tvNN);
-      load18<T, C, MODE>(kn, ps, own_p(z + 1), nx, nxy, knN);
-      load18<T, C, MODE>(tv, ps, ring_p(z + 2), nx, nxy, tvRN);
+      // Kn of the output plane lands during the shell phase; Tv a full step ahead (it feeds both phases)
+      load18<T, C, MODE>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
+      load18<T, C, MODE>(tv_z(z + 2), ps, own_xy_off, nx, nxy, TV[(J + 2) % NTV]);
+      load18<T, C, MODE>(tv_z(z + 2), ps, r_xy_off, nx, nxy, TR[(J + 1) % NKB]);
     }
     {
       T tmp[4];
-      if constexpr (!DEEP) load18<T, C, MODE>(tv, ps, own_p(z + 1), nx, nxy, tvN);
-      shell_site(Keep{}, z + 1, sx, sy, omx, opx, omy, opy, tvN, dN, sN);
-      if constexpr (!DEEP) load18<T, C, MODE>(tv, ps, ring_p(z + 1), nx, nxy, tvR);
-      if (has_r) shell_site(Drop{}, z + 1, rsx, rsy, rmx, rpx, rmy, rpy, tvR, dN, tmp);
+      if constexpr (!DEEP) load18<T, C, MODE>(tv_z(z + 1), ps, own_xy_off, nx, nxy, tvN);
+      using RN = std::integral_constant<int, (J + 1) % 6>;
+      shell_site(Keep{}, RN{}, z + 1, sx, sy, omx, opx, omy, opy, tvN, dN, sN);
+      if constexpr (!DEEP) load18<T, C, MODE>(tv_z(z + 1), ps, r_xy_off, nx, nxy, tvR);
+      if (has_r) shell_site(Drop{}, RN{}, z + 1, rsx, rsy, rmx, rpx, rmy, rpy, tvR, dN, tmp);
     }
     __syncthreads();
     if constexpr (!DEEP) {
-      load18<T, C, MODE>(kn, ps, own_p(z), nx, nxy, knC);
-      load18<T, C, MODE>(tv, ps, own_p(z), nx, nxy, tvC);
+      load18<T, C, MODE>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
+      load18<T, C, MODE>(tv_z(z), ps, own_xy_off, nx, nxy, tvC);
     }
     if (own_xy) {
-      const int p = (z * ny + gyo) * nx + gxo;
       const bool mx = omx, px = opx, my = omy, py = opy, mz = z > 0, pz = z < nz - 1;
-      const T(*s0p)[SY][SX] = ss[(z + 8) % 3];
-      const T(*s1p)[SY][SX] = ss[(z + 9) % 3];
-      const T(*s2p)[SY][SX] = ss[(z + 10) % 3];
+      const T(*s0p)[SY][SX] = ss[(J + 2) % 3];  // plane z-1
+      const T(*s1p)[SY][SX] = ss[J % 3];        // plane z
+      const T(*s2p)[SY][SX] = ss[(J + 1) % 3];  // plane z+1
       T fx = T(0), fy = T(0), fz = T(0);
       for_slots(Slots18{}, [&](auto kc) {
         constexpr int K = decltype(kc)::value;
@@ -322,20 +362,22 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 2 : 1) void ku3d_kernel(Ku
         if constexpr (oy != 0) fy += (oy > 0 ? -g : g);
         if constexpr (oz != 0) fz += (oz > 0 ? -g : g);
       });
-      T* __restrict__ o = a.out + a.bo.at(b, l) + 3 * (int64_t)p;
+      const int64_t pz3 = (int64_t)z * nxy * 3;
+      T* __restrict__ o = a.out + obase + pz3;
+      const unsigned ob = oxy * 3u * (unsigned)sizeof(T);
       if constexpr (EPI == EPI_PLAIN) {
-        o[0] = fx;
-        o[1] = fy;
-        o[2] = fz;
+        *(T*)((char*)o + ob) = fx;
+        *(T*)((char*)o + ob + sizeof(T)) = fy;
+        *(T*)((char*)o + ob + 2 * sizeof(T)) = fz;
       } else {
-        const T(*u1)[HY][HX] = su[(z + 10) % 5];
+        const T(*u1)[HY][HX] = su[J % 6];
         const T ux = u1[0][hy][hx], uy = u1[1][hy][hx], uz = u1[2][hy][hx];
         T m0 = T(1), m1 = T(1), m2 = T(1);
         if (a.mask) {
-          const uint8_t* mk = a.mask + a.bm.at(b, l) + 3 * (int64_t)p;
-          m0 = mk[0] ? T(1) : T(0);
-          m1 = mk[1] ? T(1) : T(0);
-          m2 = mk[2] ? T(1) : T(0);
+          const uint8_t* mk = a.mask + mbase + pz3;
+          m0 = ld_off(mk, oxy * 3u) ? T(1) : T(0);
+          m1 = ld_off(mk, oxy * 3u + 1u) ? T(1) : T(0);
+          m2 = ld_off(mk, oxy * 3u + 2u) ? T(1) : T(0);
         }
         if constexpr (EPI == EPI_MASKDOT) {
           fx *= m0;
@@ -343,33 +385,27 @@ __global__ __launch_bounds__(TX* TY, sizeof(T) == 4 ? 2 : 1) void ku3d_kernel(Ku
           fz *= m2;
           acc += ux * fx + uy * fy + uz * fz;
         } else {
-          const T* rh = a.rhs + a.brhs.at(b, l) + 3 * (int64_t)p;
-          fx = m0 * (rh[0] - fx);
-          fy = m1 * (rh[1] - fy);
-          fz = m2 * (rh[2] - fz);
+          const T* rh = a.rhs + rbase + pz3;
+          fx = m0 * (ld_off(rh, ob) - fx);
+          fy = m1 * (ld_off(rh, ob + (unsigned)sizeof(T)) - fy);
+          fz = m2 * (ld_off(rh, ob + 2u * (unsigned)sizeof(T)) - fz);
           acc += fx * fx + fy * fy + fz * fz;
         }
-        o[0] = fx;
-        o[1] = fy;
-        o[2] = fz;
+        *(T*)((char*)o + ob) = fx;
+        *(T*)((char*)o + ob + sizeof(T)) = fy;
+        *(T*)((char*)o + ob + 2 * sizeof(T)) = fz;
       }
     }
-    // registers of plane z+1 become the output plane; coefficients of the next steps
-#pragma unroll
-    for (int k = 0; k < 18; ++k) dC[k] = dN[k];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) sC[k] = sN[k];
-    if constexpr (DEEP) {
-#pragma unroll
-      for (int k = 0; k < 18; ++k) {
-        tvC[k] = tvN[k];
-        tvN[k] = tvNN[k];
-        knC[k] = knN[k];
-        tvR[k] = tvRN[k];
-      }
-    }
-    store_u(z + 3);  // into the slot of plane z-2 (no longer read)
+    store_u((J + 3) % 6);  // plane z+3, into the slot of plane z-3 (no longer read)
     __syncthreads();
+  };
+
+  constexpr int U = 6;  // lcm of the bank counts and of the LDS ring lengths
+  for (int z = zb; z < ze; z += U) {
+    [&]<int... Js>(std::integer_sequence<int, Js...>) __attribute__((always_inline)) {
+      bool go = true;
+      ((go = go && (z + Js < ze), go ? (step(std::integral_constant<int, Js>{}, z + Js), 0) : 0), ...);
+    }(std::make_integer_sequence<int, U>{});
   }
   if constexpr (EPI != EPI_PLAIN) {
     const T s = block_sum(acc, red);

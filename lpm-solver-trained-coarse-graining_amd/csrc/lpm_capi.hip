@@ -115,7 +115,7 @@ KuGeom ku_geom(const lpm_grid* g, int nb, int nl) {
     const bool wide = g_tile_wide ? g_tile_wide > 0 : g->nx >= 48;
     if (sizeof(T) == 4) {
       k.tx = wide ? 64 : 32;
-      k.ty = wide ? 4 : 8;
+      k.ty = g_tile_wide == 2 ? 8 : (wide ? 4 : 8);
     } else {
       k.tx = wide ? 32 : 16;
       k.ty = wide ? 4 : 8;
@@ -153,7 +153,9 @@ int launch_ku3(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const Ku3Arg
   a.nty = kg.nty;
   a.nzc = kg.nzc;
   if constexpr (sizeof(T) == 4) {
-    if (kg.tx == 64)
+    if (kg.tx == 64 && kg.ty == 8)
+      hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 64, 8, EPI>), kg.grid, kg.block, 0, st, a);
+    else if (kg.tx == 64)
       hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 64, 4, EPI>), kg.grid, kg.block, 0, st, a);
     else
       hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 32, 8, EPI>), kg.grid, kg.block, 0, st, a);
