@@ -204,8 +204,8 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
   // rotation is renaming): own elongations / shell sums of the output plane and of the next plane; own Tv of the
   // output plane, the next plane and (FP32) the plane after; own Kn of the output plane and (FP32) the next; ring
   // Tv of the next plane and (FP32) the plane after
-  constexpr int NTV = DEEP ? 3 : 2, NKB = DEEP ? 2 : 1;
-  T DD[2][18], SS[2][4], TV[NTV][18], KN[1][18], TR[NKB][18];
+  constexpr int NTV = DEEP ? 3 : 2;
+  T DD[2][18], SS[2][4], TV[NTV][18], KN[1][18], TR[1][18];
 
   auto fetch_u = [&](int z) {
     const T* uz = u + (int64_t)cl(z, nz) * nxy * 3;
@@ -292,10 +292,7 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
     load18<T, C, MODE>(tv_z(zb), ps, r_xy_off, nx, nxy, TR[0]);
     shell_site(Keep{}, R0{}, zb, tx + 1, ty + 1, omx, opx, omy, opy, TV[0], DD[0], SS[0]);
     if (has_r) shell_site(Drop{}, R0{}, zb, rsx, rsy, rmx, rpx, rmy, rpy, TR[0], DD[1], tmp);
-    if constexpr (DEEP) {
-      load18<T, C, MODE>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
-      load18<T, C, MODE>(tv_z(zb + 1), ps, r_xy_off, nx, nxy, TR[0]);
-    }
+    if constexpr (DEEP) load18<T, C, MODE>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
   }
   __syncthreads();
 
@@ -315,13 +312,14 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
     T(&tvC)[18] = TV[J % NTV];
     T(&tvN)[18] = TV[(J + 1) % NTV];
     T(&knC)[18] = KN[0];
-    T(&tvR)[18] = TR[J % NKB];
+    T(&tvR)[18] = TR[0];
     fetch_u(z + 3);  // lands during this step
     if constexpr (DEEP) {
-      // Kn of the output plane lands during the shell phase; Tv a full step ahead (it feeds both phases)
-      load18<T, C, MODE>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
+      // own Tv a full step ahead (it feeds both phases); the ring's Tv lands during the own shell sums, Kn of the
+      // output plane during the shell phase
       load18<T, C, MODE>(tv_z(z + 2), ps, own_xy_off, nx, nxy, TV[(J + 2) % NTV]);
-      load18<T, C, MODE>(tv_z(z + 2), ps, r_xy_off, nx, nxy, TR[(J + 1) % NKB]);
+      load18<T, C, MODE>(tv_z(z + 1), ps, r_xy_off, nx, nxy, tvR);
+      load18<T, C, MODE>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
     }
     {
       T tmp[4];
