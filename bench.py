@@ -141,11 +141,62 @@ def cpu_baseline(P, gpu_iters, budget_s=20.0):
     done = lib.orc_pcg3(N, ptr(nbr), ptr(Kn), ptr(Tv), ptr(M), ptr(fr), ptr(b), ptr(x), 0.0, its, ptr(rel))
     t_it = (time.perf_counter() - t0) / max(done, 1)
     t_solve = t_it * gpu_iters
-    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": int(lib.orc_threads()), "kind": "port",
+    out = {"value": 1.0 / t_solve, "unit": "solves/s", "cores": int(lib.orc_threads()), "kind": "port",
             "sample": "%d block-Jacobi PCG iterations (FP64, C/OpenMP restatement of coarse3d/lin3d.py:464-518) on "
                       "the same 256^3 problem: %.3f s per iteration, scaled by the GPU solve's %d PCG iterations"
                       % (done, t_it, gpu_iters),
             "sec_per_iter": t_it}
+    out["fp64_oracle_64cube"] = oracle_subvolume_check(lib, P["lab"], n)
+    return out
+
+
+def oracle_subvolume_check(lib, lab, n_full, n=64):
+    """Error of the GPU solve path against the FP64 oracle on the n^3 corner sub-volume of the same microstructure
+    (tension z): FP32 PCG + FP64 refinement to 1e-8 on the GPU vs the C oracle's FP64 block-Jacobi PCG to 1e-13
+    (a sparse direct factorisation of 786k DOF is out of budget)."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lpm_ref as R
+    import synth
+    import lpm_ops as ops
+    import _lpmlib as L
+    dev = torch.device("cuda:0")
+    sub = lab.reshape(n_full, n_full, n_full)[:n, :n, :n].ravel()
+    kn = PHASE_KN[sub]
+    ppar = np.stack([kn, kn, 0.3 * kn], 1)
+    dims = (n, n, n)
+    lat = R.Box(dims)
+    Kn, Tv = R.two_end_mean(lat, ppar)
+    z = lat.ijk[:, 2]
+    fixed = (z == 0) | (z == n - 1)
+    up = np.zeros((lat.N, 3))
+    up[z == n - 1, 2] = 1e-3 * (n - 1)
+    kp64, tp64 = synth.packed_from_particles(dims, ppar, dev, torch.float64)
+    upt = torch.as_tensor(up, device=dev)
+    free = torch.as_tensor((~fixed).astype(float), device=dev)
+    b = -ops.apply(dims, 1.0, upt, kp64, tp64, layout=L.PACKED) * free[:, None]
+    M = ops.block_jacobi(dims, 1.0, kp64, tp64, free=free, out_dtype=torch.float32, layout=L.PACKED)
+    x, info = ops.pcg_refined(dims, 1.0, b, kp64, tp64, kp64.float(), tp64.float(), minv32=M, free=free, tol=1e-8,
+                              maxit=100000, inner_tol=1e-4, layout64=L.PACKED, layout32=L.PACKED)
+    ug = up + x.cpu().numpy()
+    bo = np.ascontiguousarray(-R.apply(lat, up, Kn, Tv) * (~fixed)[:, None])
+    Mo = np.ascontiguousarray(R.precond(lat, Kn, Tv, ~fixed))
+    nbr = np.ascontiguousarray(lat.nbr.astype(np.int64))
+    fr = np.ascontiguousarray((~fixed).astype(np.uint8))
+    xo = np.zeros_like(bo)
+    rel = np.zeros(1)
+    pt = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    it = lib.orc_pcg3(lat.N, pt(nbr), pt(np.ascontiguousarray(Kn)), pt(np.ascontiguousarray(Tv)), pt(Mo), pt(fr),
+                      pt(bo), pt(xo), 1e-13, 100000, pt(rel))
+    uo = up + xo
+    so, sg = R.stress(lat, uo, Kn, Tv), R.stress(lat, ug, Kn, Tv)
+    top = z == n - 1
+    Fo, Fg = R.apply(lat, uo, Kn, Tv)[top, 2].sum(), R.apply(lat, ug, Kn, Tv)[top, 2].sum()
+    return {"subvolume": "%d^3 corner, tension z" % n, "gpu_relres_fp64": float(info.relmax),
+            "gpu_inner_iters": info.it, "oracle_iters": int(it), "oracle_relres": float(rel[0]),
+            "rel_err_u": float(np.abs(ug - uo).max() / np.abs(uo).max()),
+            "rel_err_stress": float(np.abs(sg - so).max() / np.abs(so).max()),
+            "rel_err_reaction": float(abs(Fg - Fo) / abs(Fo))}
 
 
 # ------------------------------------------------------------------------------------------------ other workloads

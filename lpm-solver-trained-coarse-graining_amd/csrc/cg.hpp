@@ -50,16 +50,17 @@ __global__ __launch_bounds__(256) void cg_init_kernel(CgVec<T> v, const T* __res
   T rz = T(0), rr = T(0), bb = T(0);
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < v.N; p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t o = (s * v.N + p) * D;
-    T r[3], z[3];
+    T r[3], z[3], m[3];
     for (int a = 0; a < D; ++a) {
-      const T m = v.mask ? (v.mask[v.bm.at(b, l) + D * p + a] ? T(1) : T(0)) : T(1);
-      const T bv = m * bvec[o + a];
+      m[a] = v.mask ? (v.mask[v.bm.at(b, l) + D * p + a] ? T(1) : T(0)) : T(1);
+      const T bv = m[a] * bvec[o + a];
       bb += bv * bv;
       r[a] = r_ready ? v.r[o + a] : bv;
     }
     if (v.minv) {
       constexpr int NS = D == 3 ? 6 : 3;
       apply_minv<D, T>(v.minv + v.bminv.at(b, l) + NS * p, r, z);
+      for (int a = 0; a < D; ++a) z[a] *= m[a];
     } else {
       for (int a = 0; a < D; ++a) z[a] = r[a];
     }
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(256) void cg_fin_alpha_kernel(const T* partial, int
   }
 }
 
-// x += alpha p, r -= alpha Ap, z = M r; partials (r.z, r.r)
+// x += alpha p, r -= alpha mask Ap, z = M r; partials (r.z, r.r)
 template <int D, typename T>
 __global__ __launch_bounds__(256) void cg_update_kernel(CgVec<T> v) {
   __shared__ T red[4];
@@ -142,16 +143,19 @@ __global__ __launch_bounds__(256) void cg_update_kernel(CgVec<T> v) {
   T rz = T(0), rr = T(0);
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < v.N; p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t o = (s * v.N + p) * D;
-    T r[3], z[3];
+    T r[3], z[3], m[3];
     for (int a = 0; a < D; ++a) {
+      // the K p kernel leaves fixed DOFs unmasked (p vanishes there, so p.Ap is unaffected); mask the product here
+      m[a] = v.mask ? (v.mask[v.bm.at(b, l) + D * p + a] ? T(1) : T(0)) : T(1);
       v.x[o + a] += alpha * v.p[o + a];
-      r[a] = v.r[o + a] - alpha * v.ap[o + a];
+      r[a] = v.r[o + a] - alpha * (m[a] * v.ap[o + a]);
       v.r[o + a] = r[a];
     }
     if (v.minv) {
       constexpr int NS = D == 3 ? 6 : 3;
       apply_minv<D, T>(v.minv + v.bminv.at(b, l) + NS * p, r, z);
-      for (int a = 0; a < D; ++a) v.z[o + a] = z[a];
+      // z (hence p) vanishes on fixed DOFs whatever the preconditioner holds there
+      for (int a = 0; a < D; ++a) v.z[o + a] = z[a] = m[a] * z[a];
     } else {
       for (int a = 0; a < D; ++a) z[a] = r[a];
     }

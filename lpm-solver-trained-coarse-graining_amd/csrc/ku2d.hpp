@@ -149,17 +149,15 @@ __global__ __launch_bounds__(TX* TY) void ku2d_kernel(Ku2Args<T, C> a) {
       o[0] = fx;
       o[1] = fy;
     } else {
-      T m0 = T(1), m1 = T(1);
-      if (a.mask) {
-        const uint8_t* mk = a.mask + a.bm.at(b, l) + 2 * p;
-        m0 = mk[0] ? T(1) : T(0);
-        m1 = mk[1] ? T(1) : T(0);
-      }
       if constexpr (EPI == EPI_MASKDOT) {
-        fx *= m0;
-        fy *= m1;
-        acc = ux * fx + uy * fy;
+        acc = ux * fx + uy * fy;  // u vanishes on fixed DOFs: cg_update masks the product
       } else {
+        T m0 = T(1), m1 = T(1);
+        if (a.mask) {
+          const uint8_t* mk = a.mask + a.bm.at(b, l) + 2 * p;
+          m0 = mk[0] ? T(1) : T(0);
+          m1 = mk[1] ? T(1) : T(0);
+        }
         const T* rh = a.rhs + a.brhs.at(b, l) + 2 * p;
         fx = m0 * (rh[0] - fx);
         fy = m1 * (rh[1] - fy);

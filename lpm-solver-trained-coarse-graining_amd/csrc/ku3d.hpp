@@ -28,7 +28,8 @@ namespace lpm {
 
 enum Ku3Epi : int {
   EPI_PLAIN = 0,    // out = K u
-  EPI_MASKDOT = 1,  // out = mask * (K u), partial dot(u, out) per block
+  EPI_MASKDOT = 1,  // out = K u, partial dot(u, K u) per block; u vanishes on fixed DOFs inside (P)CG, so the dot
+                    // equals dot(u, mask * K u) and the mask is applied where the product is consumed (cg_update)
   EPI_RESID = 2,    // out = mask * (rhs - K u), partial dot(out, out) per block
 };
 
@@ -106,12 +107,15 @@ __device__ __forceinline__ void for_slots(Seq<Ks...>, F&& f) {
 // directed: row 0 of the plane; packed: plane-0 entry of the plane's first particle) and `xy` the site's in-plane
 // index, which stays the same for every plane of a z march: the loads take one loop-invariant lane offset and a
 // uniform per-slot base (packed odd slots: the neighbour's forward-plane entry at p + delta, inside the zero guards).
-template <typename T, typename C, int MODE>
+// SKIP5 leaves slot 5 (-z) alone: in the packed layout it is slot 4 of the same column one plane below, which the
+// z march already holds in a register.
+template <typename T, typename C, int MODE, bool SKIP5 = false>
 __device__ __forceinline__ void load18(const C* __restrict__ zbase, int64_t ps, unsigned xy, int nx, int nxy,
                                        T (&out)[18]) {
   for_slots(Slots18{}, [&](auto kc) {
     constexpr int K = decltype(kc)::value;
-    if constexpr (MODE == COEF_DIRECTED) {
+    if constexpr (SKIP5 && K == 5) {
+    } else if constexpr (MODE == COEF_DIRECTED) {
       out[K] = (T)ld_off(zbase, xy * (18u * (unsigned)sizeof(C)) + K * (unsigned)sizeof(C));  // < 4 GiB (host check)
     } else {
       const int64_t sh = (int64_t)(K >> 1) * ps + ((K & 1) ? off3(K, 0) + off3(K, 1) * nx + off3(K, 2) * nxy : 0);
@@ -205,6 +209,8 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
   // output plane, the next plane and (FP32) the plane after; own Kn of the output plane and (FP32) the next; ring
   // Tv of the next plane and (FP32) the plane after
   constexpr int NTV = DEEP ? 3 : 2;
+  // packed FP32: the -z coefficients come from the previous plane's registers instead of a load
+  constexpr bool REUSE5 = DEEP && MODE == COEF_PACKED;
   T DD[2][18], SS[2][4], TV[NTV][18], KN[1][18], TR[1][18];
 
   auto fetch_u = [&](int z) {
@@ -292,7 +298,15 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
     load18<T, C, MODE>(tv_z(zb), ps, r_xy_off, nx, nxy, TR[0]);
     shell_site(Keep{}, R0{}, zb, tx + 1, ty + 1, omx, opx, omy, opy, TV[0], DD[0], SS[0]);
     if (has_r) shell_site(Drop{}, R0{}, zb, rsx, rsy, rmx, rpx, rmy, rpy, TR[0], DD[1], tmp);
-    if constexpr (DEEP) load18<T, C, MODE>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
+    if constexpr (DEEP) {
+      if constexpr (REUSE5) {
+        load18<T, C, MODE, true>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
+        TV[1][5] = TV[0][4];
+        KN[0][4] = (T)ld_off(kn_z(zb - 1) + 2 * ps, own_xy_off * (unsigned)sizeof(C));  // slot 4 below zb
+      } else {
+        load18<T, C, MODE>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
+      }
+    }
   }
   __syncthreads();
 
@@ -317,9 +331,19 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
     if constexpr (DEEP) {
       // own Tv a full step ahead (it feeds both phases); the ring's Tv lands during the own shell sums, Kn of the
       // output plane during the shell phase
-      load18<T, C, MODE>(tv_z(z + 2), ps, own_xy_off, nx, nxy, TV[(J + 2) % NTV]);
-      load18<T, C, MODE>(tv_z(z + 1), ps, r_xy_off, nx, nxy, tvR);
-      load18<T, C, MODE>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
+      if constexpr (REUSE5) {
+        const T r4 = tvR[4], k4 = knC[4];  // ring Tv of plane z, own Kn of plane z-1
+        load18<T, C, MODE, true>(tv_z(z + 2), ps, own_xy_off, nx, nxy, TV[(J + 2) % NTV]);
+        TV[(J + 2) % NTV][5] = TV[(J + 1) % NTV][4];
+        load18<T, C, MODE, true>(tv_z(z + 1), ps, r_xy_off, nx, nxy, tvR);
+        tvR[5] = r4;
+        load18<T, C, MODE, true>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
+        knC[5] = k4;
+      } else {
+        load18<T, C, MODE>(tv_z(z + 2), ps, own_xy_off, nx, nxy, TV[(J + 2) % NTV]);
+        load18<T, C, MODE>(tv_z(z + 1), ps, r_xy_off, nx, nxy, tvR);
+        load18<T, C, MODE>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
+      }
     }
     {
       T tmp[4];
@@ -370,19 +394,16 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
       } else {
         const T(*u1)[HY][HX] = su[J % 6];
         const T ux = u1[0][hy][hx], uy = u1[1][hy][hx], uz = u1[2][hy][hx];
-        T m0 = T(1), m1 = T(1), m2 = T(1);
-        if (a.mask) {
-          const uint8_t* mk = a.mask + mbase + pz3;
-          m0 = ld_off(mk, oxy * 3u) ? T(1) : T(0);
-          m1 = ld_off(mk, oxy * 3u + 1u) ? T(1) : T(0);
-          m2 = ld_off(mk, oxy * 3u + 2u) ? T(1) : T(0);
-        }
         if constexpr (EPI == EPI_MASKDOT) {
-          fx *= m0;
-          fy *= m1;
-          fz *= m2;
           acc += ux * fx + uy * fy + uz * fz;
         } else {
+          T m0 = T(1), m1 = T(1), m2 = T(1);
+          if (a.mask) {
+            const uint8_t* mk = a.mask + mbase + pz3;
+            m0 = ld_off(mk, oxy * 3u) ? T(1) : T(0);
+            m1 = ld_off(mk, oxy * 3u + 1u) ? T(1) : T(0);
+            m2 = ld_off(mk, oxy * 3u + 2u) ? T(1) : T(0);
+          }
           const T* rh = a.rhs + rbase + pz3;
           fx = m0 * (ld_off(rh, ob) - fx);
           fy = m1 * (ld_off(rh, ob + (unsigned)sizeof(T)) - fy);
