@@ -206,12 +206,12 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
   constexpr bool DEEP = sizeof(T) == 4;
   // register banks whose roles rotate with the step (the z loop is unrolled by the rotation period, so the
   // rotation is renaming): own elongations / shell sums of the output plane and of the next plane; own Tv of the
-  // output plane, the next plane and (FP32) the plane after; own Kn of the output plane and (FP32) the next; ring
-  // Tv of the next plane and (FP32) the plane after
+  // output plane, the next plane and (FP32) the plane after; own Kn of the output plane (loaded at the start of
+  // its step); ring Tv of the next plane and the plane after
   constexpr int NTV = DEEP ? 3 : 2;
   // packed FP32: the -z coefficients come from the previous plane's registers instead of a load
   constexpr bool REUSE5 = DEEP && MODE == COEF_PACKED;
-  T DD[2][18], SS[2][4], TV[NTV][18], KN[1][18], TR[1][18];
+  T DD[2][18], SS[2][4], TV[NTV][18], KN[1][18], TR[2][18];
 
   auto fetch_u = [&](int z) {
     const T* uz = u + (int64_t)cl(z, nz) * nxy * 3;
@@ -303,6 +303,8 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
         load18<T, C, MODE, true>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
         TV[1][5] = TV[0][4];
         KN[0][4] = (T)ld_off(kn_z(zb - 1) + 2 * ps, own_xy_off * (unsigned)sizeof(C));  // slot 4 below zb
+        load18<T, C, MODE, true>(tv_z(zb + 1), ps, r_xy_off, nx, nxy, TR[1]);
+        TR[1][5] = TR[0][4];
       } else {
         load18<T, C, MODE>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
       }
@@ -326,17 +328,17 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
     T(&tvC)[18] = TV[J % NTV];
     T(&tvN)[18] = TV[(J + 1) % NTV];
     T(&knC)[18] = KN[0];
-    T(&tvR)[18] = TR[0];
+    T(&tvR)[18] = TR[(J + 1) % 2];
     fetch_u(z + 3);  // lands during this step
     if constexpr (DEEP) {
       // own Tv a full step ahead (it feeds both phases); the ring's Tv lands during the own shell sums, Kn of the
       // output plane during the shell phase
       if constexpr (REUSE5) {
-        const T r4 = tvR[4], k4 = knC[4];  // ring Tv of plane z, own Kn of plane z-1
+        const T k4 = knC[4];  // own Kn of plane z-1
         load18<T, C, MODE, true>(tv_z(z + 2), ps, own_xy_off, nx, nxy, TV[(J + 2) % NTV]);
         TV[(J + 2) % NTV][5] = TV[(J + 1) % NTV][4];
-        load18<T, C, MODE, true>(tv_z(z + 1), ps, r_xy_off, nx, nxy, tvR);
-        tvR[5] = r4;
+        load18<T, C, MODE, true>(tv_z(z + 2), ps, r_xy_off, nx, nxy, TR[J % 2]);
+        TR[J % 2][5] = tvR[4];
         load18<T, C, MODE, true>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
         knC[5] = k4;
       } else {

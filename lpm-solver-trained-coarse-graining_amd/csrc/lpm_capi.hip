@@ -123,8 +123,9 @@ KuGeom ku_geom(const lpm_grid* g, int nb, int nl) {
     k.ntx = (g->nx + k.tx - 1) / k.tx;
     k.nty = (g->ny + k.ty - 1) / k.ty;
     const int64_t xy = (int64_t)k.ntx * k.nty * nb * nl;
-    // about 1024 blocks (two resident per CU): long z marches amortise the 5-plane prologue
-    const int64_t want = std::max<int64_t>(1, (1024 + xy - 1) / xy);
+    // about 512 blocks, i.e. one round of two resident blocks per CU: no tail round, and the longest z marches
+    // amortise the 5-plane prologue
+    const int64_t want = std::max<int64_t>(1, (512 + xy - 1) / xy);
     k.zc = g_zc > 0 ? g_zc : std::max<int>(8, (int)((g->nz + want - 1) / want));
     k.nzc = (g->nz + k.zc - 1) / k.zc;
     k.grid = dim3(k.ntx * k.nty * k.nzc, nl, nb);
