@@ -9,7 +9,9 @@ namespace lpm {
 
 // per-system scalars (type T) and integer state
 enum CgScal : int { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_BN, S_TOL, S_REL, NSCAL };
-enum CgInt : int { I_IT = 0, I_ACTIVE, I_MAXIT, I_CONV, NINT };
+// I_XUPD: the x += alpha p of the current iteration is still to be applied (set by cg_fin_alpha, consumed by
+// cg_pupdate, which folds the x update into its pass over p)
+enum CgInt : int { I_IT = 0, I_ACTIVE, I_MAXIT, I_CONV, I_XUPD, NINT };
 
 template <typename T>
 struct CgVec {
@@ -120,10 +122,14 @@ __global__ __launch_bounds__(256) void cg_fin_init_kernel(T* partial, int nblk, 
 
 // alpha = rz / pAp from the partials of the masked K p kernel (nblk per system)
 template <typename T>
-__global__ __launch_bounds__(256) void cg_fin_alpha_kernel(const T* partial, int nblk, T* scal, const int* ist) {
+__global__ __launch_bounds__(256) void cg_fin_alpha_kernel(const T* partial, int nblk, T* scal, int* ist) {
   __shared__ T red[4];
   const int s = blockIdx.x;
-  if (!ist[(int64_t)s * NINT + I_ACTIVE]) return;
+  int* is = ist + (int64_t)s * NINT;
+  const int active = is[I_ACTIVE];
+  __syncthreads();
+  if (threadIdx.x == 0) is[I_XUPD] = active;
+  if (!active) return;
   const T pap = sum_partials(partial + (int64_t)s * nblk, nblk, 1, 0, red);
   if (threadIdx.x == 0) {
     T* sc = scal + (int64_t)s * NSCAL;
@@ -132,7 +138,7 @@ __global__ __launch_bounds__(256) void cg_fin_alpha_kernel(const T* partial, int
   }
 }
 
-// x += alpha p, r -= alpha mask Ap, z = M r; partials (r.z, r.r)
+// r -= alpha mask Ap, z = M r; partials (r.z, r.r).  x += alpha p is applied by cg_pupdate.
 template <int D, typename T>
 __global__ __launch_bounds__(256) void cg_update_kernel(CgVec<T> v) {
   __shared__ T red[4];
@@ -147,7 +153,6 @@ __global__ __launch_bounds__(256) void cg_update_kernel(CgVec<T> v) {
     for (int a = 0; a < D; ++a) {
       // the K p kernel leaves fixed DOFs unmasked (p vanishes there, so p.Ap is unaffected); mask the product here
       m[a] = v.mask ? (v.mask[v.bm.at(b, l) + D * p + a] ? T(1) : T(0)) : T(1);
-      v.x[o + a] += alpha * v.p[o + a];
       r[a] = v.r[o + a] - alpha * (m[a] * v.ap[o + a]);
       v.r[o + a] = r[a];
     }
@@ -202,17 +207,24 @@ __global__ __launch_bounds__(256) void cg_fin_beta_kernel(const T* partial, int 
   }
 }
 
-// p = z + beta p
+// x += alpha p (the iteration's update, also in the iteration that converged), then p = z + beta p while the
+// system is still active
 template <int D, typename T>
 __global__ __launch_bounds__(256) void cg_pupdate_kernel(CgVec<T> v) {
   const int l = blockIdx.y, b = blockIdx.z;
   const int64_t s = (int64_t)b * v.nl + l;
-  if (!v.ist[s * NINT + I_ACTIVE]) return;
+  if (!v.ist[s * NINT + I_XUPD]) return;
+  const bool active = v.ist[s * NINT + I_ACTIVE] != 0;
+  const T alpha = v.scal[s * NSCAL + S_ALPHA];
   const T beta = v.scal[s * NSCAL + S_BETA];
   const T* z = v.z ? v.z : v.r;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < v.N; p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t o = (s * v.N + p) * D;
-    for (int a = 0; a < D; ++a) v.p[o + a] = z[o + a] + beta * v.p[o + a];
+    for (int a = 0; a < D; ++a) {
+      const T pv = v.p[o + a];
+      v.x[o + a] += alpha * pv;
+      if (active) v.p[o + a] = z[o + a] + beta * pv;
+    }
   }
 }
 

@@ -261,6 +261,7 @@ __global__ void cg_setup_kernel(T* scal, int* ist, int nsys, T tol, int maxit) {
   ist[(int64_t)s * NINT + I_MAXIT] = maxit;
   ist[(int64_t)s * NINT + I_ACTIVE] = 0;
   ist[(int64_t)s * NINT + I_IT] = 0;
+  ist[(int64_t)s * NINT + I_XUPD] = 0;
 }
 // per-system tolerance from a device array (refinement inner solves)
 template <typename T>
