@@ -138,6 +138,21 @@ __global__ __launch_bounds__(256) void cg_fin_alpha_kernel(const T* partial, int
   }
 }
 
+// D components of one site moved by one vector access
+template <int D, typename T>
+struct Site {
+  T c[D];
+};
+template <int D, typename T>
+__device__ __forceinline__ Site<D, T> ld_site(const T* __restrict__ base, int64_t o) {
+  return *(const Site<D, T>*)(base + o);
+}
+template <int D, typename T>
+__device__ __forceinline__ void st_site(T* __restrict__ base, int64_t o, const Site<D, T>& v) {
+  *(Site<D, T>*)(base + o) = v;
+}
+constexpr int VU = 4;  // independent sites per thread and pass of the vector kernels (memory-level parallelism)
+
 // r -= alpha mask Ap, z = M r; partials (r.z, r.r).  x += alpha p is applied by cg_pupdate.
 template <int D, typename T>
 __global__ __launch_bounds__(256) void cg_update_kernel(CgVec<T> v) {
@@ -145,28 +160,52 @@ __global__ __launch_bounds__(256) void cg_update_kernel(CgVec<T> v) {
   const int l = blockIdx.y, b = blockIdx.z;
   const int64_t s = (int64_t)b * v.nl + l;
   if (!v.ist[s * NINT + I_ACTIVE]) return;
+  constexpr int NS = D == 3 ? 6 : 3;
   const T alpha = v.scal[s * NSCAL + S_ALPHA];
+  T* __restrict__ r_ = v.r + s * v.N * D;
+  T* __restrict__ z_ = v.z ? v.z + s * v.N * D : nullptr;
+  const T* __restrict__ ap_ = v.ap + s * v.N * D;
+  const T* __restrict__ mi_ = v.minv ? v.minv + v.bminv.at(b, l) : nullptr;
+  const uint8_t* __restrict__ mk_ = v.mask ? v.mask + v.bm.at(b, l) : nullptr;
   T rz = T(0), rr = T(0);
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < v.N; p += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t o = (s * v.N + p) * D;
-    T r[3], z[3], m[3];
-    for (int a = 0; a < D; ++a) {
-      // the K p kernel leaves fixed DOFs unmasked (p vanishes there, so p.Ap is unaffected); mask the product here
-      m[a] = v.mask ? (v.mask[v.bm.at(b, l) + D * p + a] ? T(1) : T(0)) : T(1);
-      r[a] = v.r[o + a] - alpha * (m[a] * v.ap[o + a]);
-      v.r[o + a] = r[a];
+  const int64_t step = (int64_t)gridDim.x * blockDim.x * VU;
+  for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x * VU + threadIdx.x; p0 < v.N; p0 += step) {
+    Site<D, T> rv[VU], av[VU];
+    T m[VU][3];
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+      const int64_t p = p0 + (int64_t)u * blockDim.x;
+      if (p < v.N) {
+        rv[u] = ld_site<D>(r_, p * D);
+        av[u] = ld_site<D>(ap_, p * D);
+        for (int a = 0; a < D; ++a) m[u][a] = mk_ ? (mk_[D * p + a] ? T(1) : T(0)) : T(1);
+      }
     }
-    if (v.minv) {
-      constexpr int NS = D == 3 ? 6 : 3;
-      apply_minv<D, T>(v.minv + v.bminv.at(b, l) + NS * p, r, z);
-      // z (hence p) vanishes on fixed DOFs whatever the preconditioner holds there
-      for (int a = 0; a < D; ++a) v.z[o + a] = z[a] = m[a] * z[a];
-    } else {
-      for (int a = 0; a < D; ++a) z[a] = r[a];
-    }
-    for (int a = 0; a < D; ++a) {
-      rz += r[a] * z[a];
-      rr += r[a] * r[a];
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+      const int64_t p = p0 + (int64_t)u * blockDim.x;
+      if (p < v.N) {
+        T r[3], z[3];
+        for (int a = 0; a < D; ++a) {
+          // the K p kernel leaves fixed DOFs unmasked (p vanishes there, so p.Ap is unaffected); mask here
+          r[a] = rv[u].c[a] - alpha * (m[u][a] * av[u].c[a]);
+          rv[u].c[a] = r[a];
+        }
+        st_site<D>(r_, p * D, rv[u]);
+        if (mi_) {
+          apply_minv<D, T>(mi_ + NS * p, r, z);
+          // z (hence p) vanishes on fixed DOFs whatever the preconditioner holds there
+          Site<D, T> zv;
+          for (int a = 0; a < D; ++a) zv.c[a] = z[a] = m[u][a] * z[a];
+          st_site<D>(z_, p * D, zv);
+        } else {
+          for (int a = 0; a < D; ++a) z[a] = r[a];
+        }
+        for (int a = 0; a < D; ++a) {
+          rz += r[a] * z[a];
+          rr += r[a] * r[a];
+        }
+      }
     }
   }
   const T s0 = block_sum(rz, red);
@@ -217,13 +256,32 @@ __global__ __launch_bounds__(256) void cg_pupdate_kernel(CgVec<T> v) {
   const bool active = v.ist[s * NINT + I_ACTIVE] != 0;
   const T alpha = v.scal[s * NSCAL + S_ALPHA];
   const T beta = v.scal[s * NSCAL + S_BETA];
-  const T* z = v.z ? v.z : v.r;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < v.N; p += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t o = (s * v.N + p) * D;
-    for (int a = 0; a < D; ++a) {
-      const T pv = v.p[o + a];
-      v.x[o + a] += alpha * pv;
-      if (active) v.p[o + a] = z[o + a] + beta * pv;
+  T* __restrict__ x_ = v.x + s * v.N * D;
+  T* __restrict__ p_ = v.p + s * v.N * D;
+  const T* __restrict__ z_ = (v.z ? v.z : v.r) + s * v.N * D;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x * VU;
+  for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x * VU + threadIdx.x; p0 < v.N; p0 += step) {
+    Site<D, T> pv[VU], xv[VU], zv[VU];
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+      const int64_t q = p0 + (int64_t)u * blockDim.x;
+      if (q < v.N) {
+        pv[u] = ld_site<D>(p_, q * D);
+        xv[u] = ld_site<D>(x_, q * D);
+        if (active) zv[u] = ld_site<D>(z_, q * D);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+      const int64_t q = p0 + (int64_t)u * blockDim.x;
+      if (q < v.N) {
+        for (int a = 0; a < D; ++a) xv[u].c[a] += alpha * pv[u].c[a];
+        st_site<D>(x_, q * D, xv[u]);
+        if (active) {
+          for (int a = 0; a < D; ++a) pv[u].c[a] = zv[u].c[a] + beta * pv[u].c[a];
+          st_site<D>(p_, q * D, pv[u]);
+        }
+      }
     }
   }
 }
