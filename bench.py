@@ -26,9 +26,9 @@ METRIC = "K·u HBM GB/s (% peak); CG solves/s + p99 latency at rel-res 1e-8"
 HBM_PEAK_GBS = 8000.0
 PHASE_KN = np.array([0.1, 1.0, 2.0, 5.0])
 # HBM bytes per launch of the masked K p kernel at 256^3 (FP32, packed), from rocprofv3 --pmc FETCH_SIZE and
-# --pmc WRITE_SIZE runs of `bench.py --probe 256`, scaled by the same counters of a 1.61 GB copy
-# (profiles/ku3d_256cube_fp32_packed.md)
-KU_TRAFFIC_256 = 812452 * 1024 * 2.0 + 196613 * 1024 * 1.0
+# --pmc WRITE_SIZE runs of `bench.py --probe 256` (KiB per dispatch), scaled by the same counters of a 1.61 GB copy
+# (factors 2.0 and 1.0; profiles/ku3d_256cube_fp32_packed.md)
+KU_TRAFFIC_256 = 936727 * 1024 * 2.0 + 196612 * 1024 * 1.0
 
 
 def probe(n, reps):
