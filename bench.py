@@ -240,7 +240,7 @@ def bench_plain2d(a):
 
     def gpu_solve():
         return ops.pcg(dims, 1.0, b, kp, tp, minv=None, free=mfree, tol=a.tol, maxit=200000, check=50,
-                       layout=L.PACKED)
+                       graph=True, layout=L.PACKED)
     for _ in range(a.warmup):
         gpu_solve()
     torch.cuda.synchronize()
@@ -292,22 +292,26 @@ def bench_kubc3d(a):
     dims = P["dims"]
     N = 48 ** 3
 
-    def step():
+    def step(graph=True):
         b = -ops.apply(dims, 1.0, P["up"], P["kp64"], P["tp64"], layout=L.PACKED) * P["free"][:, None]
         M = ops.block_jacobi(dims, 1.0, P["kp64"], P["tp64"], free=P["free"], out_dtype=torch.float32,
                              layout=L.PACKED)
         return ops.pcg_refined(dims, 1.0, b, P["kp64"], P["tp64"], P["kp32"], P["tp32"], minv32=M, free=P["free"],
-                               tol=a.tol, maxit=100000, inner_tol=1e-4, layout64=L.PACKED, layout32=L.PACKED)
+                               tol=a.tol, maxit=100000, inner_tol=1e-4, check=25, graph=graph, layout64=L.PACKED,
+                               layout32=L.PACKED)
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
-    L.lib().lpm_profile_ku(1)
     lat = []
     for _ in range(a.steps):
         t0 = time.perf_counter()
         x, info = step()
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0)
+    # K u launch times from one more solve without graph replays (HIP events around each launch)
+    L.lib().lpm_profile_ku(1)
+    step(graph=False)
+    torch.cuda.synchronize()
     ms = ctypes.c_double()
     cnt = ctypes.c_int64()
     L.lib().lpm_profile_ku_read(ctypes.byref(ms), ctypes.byref(cnt))
@@ -371,7 +375,7 @@ def bench_serve(a):
     torch.cuda.synchronize()
     plain_rate = nplain / (time.perf_counter() - t0)
     print("[serve] plain GPU CG: %.1f solves/s on %d requests" % (plain_rate, nplain), file=sys.stderr, flush=True)
-    srv = serve.Server(max_slots=a.slots, device=dev, budget=a.budget)
+    srv = serve.Server(max_slots=a.slots, device=dev, budget=a.budget, threads=a.threads)
     serve.run_stream(srv, reqs[:min(64, len(reqs))])                       # warm-up (graph capture, clocks)
     sat = serve.run_stream(srv, reqs)
     print("[serve] saturation: %.1f solves/s" % sat["solves_per_s"], file=sys.stderr, flush=True)
@@ -391,7 +395,7 @@ def bench_serve(a):
            "saturation": sat, "poisson": pois,
            "plain_gpu_cg": {"solves_per_s": plain_rate, "sample": nplain,
                             "speedup": sat["solves_per_s"] / plain_rate},
-           "slots": a.slots, "budget": a.budget})
+           "slots": a.slots, "budget": a.budget, "threads": a.threads})
 
 
 def bench_train3d(a):
@@ -419,7 +423,7 @@ def bench_train3d(a):
             ((o["sig"] - tgt["sig"]) ** 2).mean() / (tgt["sig"] ** 2).mean()
         loss.backward()
         opt.step()
-        return float(loss)
+        return float(loss.detach())
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
@@ -456,6 +460,7 @@ def main():
     ap.add_argument("--budget", type=int, default=400000)
     ap.add_argument("--load", type=float, default=0.8)
     ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--threads", type=int, default=1024, help="serving: workgroup size per request slot")
     a = ap.parse_args()
     if a.probe:
         probe(a.probe, a.probe_reps)

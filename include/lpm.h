@@ -139,7 +139,7 @@ int lpm_pcg_refined(const lpm_grid* g, int32_t nb, int32_t nl, int32_t layout64,
                     int32_t* outer, int32_t* conv, void* stream);
 
 /* ---- Serving: independent 2D square-lattice solves of different sizes, continuously batched.
- * A server owns a device arena and max_slots slots.  lpm_serve_admit copies one request (kn, tv (N, 8) FP32
+ * A server owns a device arena and max_slots slots, each advanced by one workgroup of `threads` (256, 512, 1024).  lpm_serve_admit copies one request (kn, tv (N, 8) FP32
  * directed symmetric, mask (N, 2) 1 = free DOF, up (N, 2) FP64 prescribed displacement, zero on free DOFs) into
  * a slot, or returns LPM_E_FULL when no slot or arena space is free.  lpm_serve_step launches (optionally as a
  * replayed hipGraph) one persistent kernel in which every active slot advances its own FP32 block-Jacobi PCG
@@ -150,7 +150,7 @@ int lpm_pcg_refined(const lpm_grid* g, int32_t nb, int32_t nl, int32_t layout64,
  * not thread-safe.  The solve replaces CoarseT.solve's dense Cholesky (coarse2d/torch_lpm.py:216-229) for
  * request streams; the slot arena follows the survey's proposed lpm_serving_* interface (admit, step, retire). */
 size_t lpm_serve_footprint(int32_t nx, int32_t ny);
-int lpm_serve_create(int32_t max_slots, size_t arena_bytes, void** server);
+int lpm_serve_create(int32_t max_slots, size_t arena_bytes, int32_t threads, void** server);
 void lpm_serve_destroy(void* server);
 int lpm_serve_admit(void* server, int32_t nx, int32_t ny, double h, const float* kn, const float* tv,
                     const uint8_t* mask, const double* up, double tol, double inner_tol, int32_t maxit, void* stream,

@@ -52,7 +52,7 @@ SIGS = {
     "lpm_pcg_solve": (I32, [GP, I32, I32, I32, I32, P, P, Bcast, P, Bcast, P, Bcast, P, P, I32,
                             ctypes.POINTER(CgOpts), P, P, P, P, P]),
     "lpm_serve_footprint": (ctypes.c_size_t, [I32, I32]),
-    "lpm_serve_create": (I32, [I32, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "lpm_serve_create": (I32, [I32, ctypes.c_size_t, I32, ctypes.POINTER(ctypes.c_void_p)]),
     "lpm_serve_destroy": (None, [P]),
     "lpm_serve_admit": (I32, [P, I32, I32, ctypes.c_double, P, P, P, P, ctypes.c_double, ctypes.c_double, I32, P,
                               ctypes.POINTER(I32)]),

@@ -880,6 +880,7 @@ namespace {
 
 struct Server {
   int max_slots = 0;
+  int threads = 256;  // workgroup size of the slot kernel
   char* arena = nullptr;
   size_t arena_bytes = 0;
   std::map<size_t, size_t> free_blocks;  // offset -> size, coalesced
@@ -938,11 +939,13 @@ size_t lpm_serve_footprint(int32_t nx, int32_t ny) {
   return slot_bytes(nx, ny);
 }
 
-int lpm_serve_create(int32_t max_slots, size_t arena_bytes, void** server) {
-  if (!server || max_slots < 1 || max_slots > 65535 || arena_bytes < 256)
+int lpm_serve_create(int32_t max_slots, size_t arena_bytes, int32_t threads, void** server) {
+  if (!server || max_slots < 1 || max_slots > 65535 || arena_bytes < 256 ||
+      (threads != 256 && threads != 512 && threads != 1024))
     return fail(LPM_E_BADARG, "lpm_serve_create: bad arguments");
   Server* s = new Server();
   s->max_slots = max_slots;
+  s->threads = threads;
   s->arena_bytes = arena_bytes & ~(size_t)255;
   hipError_t e = hipMalloc((void**)&s->arena, s->arena_bytes);
   if (e == hipSuccess) e = hipMalloc((void**)&s->d_slots, sizeof(SlotDesc) * max_slots);
@@ -1046,7 +1049,7 @@ int lpm_serve_step(void* server, int64_t budget, int32_t use_graph, void* stream
   Server* s = (Server*)server;
   if (!s || budget < 1) return fail(LPM_E_BADARG, "lpm_serve_step: bad arguments");
   hipStream_t st = (hipStream_t)stream;
-  const dim3 grid((unsigned)s->max_slots), block(srv::NT);
+  const dim3 grid((unsigned)s->max_slots), block((unsigned)s->threads);
   if (!use_graph) {
     hipLaunchKernelGGL(serve_kernel, grid, block, 0, st, s->d_slots, s->max_slots, (long long)budget);
     LAUNCHCHK();

@@ -48,7 +48,7 @@ struct SlotDesc {
 
 namespace srv {
 
-constexpr int NT = 256;  // threads per slot workgroup
+constexpr int NT_MAX = 1024;  // threads per slot workgroup: 256, 512 or 1024 (blockDim.x), fixed per server
 
 // directed slot geometry of the 2D square lattice
 __device__ __forceinline__ int nbr2(int x, int y, int nx, int ny, int k) {
@@ -69,8 +69,8 @@ __device__ __forceinline__ void wg_sum2(T& a, T& b, T* red) {
   }
   __syncthreads();
   T sa = T(0), sb = T(0);
-#pragma unroll
-  for (int i = 0; i < NT / 64; ++i) {
+  const int nw = blockDim.x >> 6;
+  for (int i = 0; i < nw; ++i) {
     sa += red[2 * i];
     sb += red[2 * i + 1];
   }
@@ -83,7 +83,7 @@ __device__ __forceinline__ void wg_sum2(T& a, T& b, T* red) {
 template <typename T>
 __device__ void shell_pass(const SlotDesc& s, const T* __restrict__ u, T* __restrict__ w) {
   const int N = s.nx * s.ny;
-  for (int p = threadIdx.x; p < N; p += NT) {
+  for (int p = threadIdx.x; p < N; p += blockDim.x) {
     const int x = p % s.nx, y = p / s.nx;
     T a[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
@@ -129,7 +129,7 @@ __device__ void phase_init(SlotDesc& s, double* red) {
   shell_pass<double>(s, s.up, s.w64);
   __syncthreads();
   double bb = 0.0, unused = 0.0;
-  for (int p = threadIdx.x; p < N; p += NT) {
+  for (int p = threadIdx.x; p < N; p += blockDim.x) {
     double f0, f1;
     ku_site<double>(s, s.up, s.w64, p, f0, f1);
     const double b0 = -(double)mfree(s, 2 * p) * f0, b1 = -(double)mfree(s, 2 * p + 1) * f1;
@@ -190,7 +190,7 @@ __device__ void phase_outer(SlotDesc& s, double* red) {
   shell_pass<double>(s, s.x, s.w64);
   __syncthreads();
   double rr = 0.0, unused = 0.0;
-  for (int p = threadIdx.x; p < N; p += NT) {
+  for (int p = threadIdx.x; p < N; p += blockDim.x) {
     double f0, f1;
     ku_site<double>(s, s.x, s.w64, p, f0, f1);
     const double r0 = (double)mfree(s, 2 * p) * (s.b[2 * p] - f0);
@@ -209,7 +209,7 @@ __device__ void phase_outer(SlotDesc& s, double* red) {
   s.thr2 = itol * itol * rr;
   double rz = 0.0;
   unused = 0.0;
-  for (int p = threadIdx.x; p < N; p += NT) {
+  for (int p = threadIdx.x; p < N; p += blockDim.x) {
     const float* m = s.minv + 3 * p;
     const float r0 = s.r[2 * p], r1 = s.r[2 * p + 1];
     const float z0 = m[0] * r0 + m[2] * r1, z1 = m[2] * r0 + m[1] * r1;
@@ -234,7 +234,7 @@ __device__ void inner_iteration(SlotDesc& s, float* red) {
   shell_pass<float>(s, s.p, s.w32);
   __syncthreads();
   float pap = 0.f, unused = 0.f;
-  for (int p = threadIdx.x; p < N; p += NT) {
+  for (int p = threadIdx.x; p < N; p += blockDim.x) {
     float f0, f1;
     ku_site<float>(s, s.p, s.w32, p, f0, f1);
     f0 *= mfree(s, 2 * p);
@@ -250,7 +250,7 @@ __device__ void inner_iteration(SlotDesc& s, float* red) {
   }
   const float alpha = s.rz / pap;
   float rz = 0.f, rr = 0.f;
-  for (int p = threadIdx.x; p < N; p += NT) {
+  for (int p = threadIdx.x; p < N; p += blockDim.x) {
     const float* m = s.minv + 3 * p;
     const int i = 2 * p;
     s.d[i] += alpha * s.p[i];
@@ -268,24 +268,26 @@ __device__ void inner_iteration(SlotDesc& s, float* red) {
   s.it += 1;
   s.total += 1;
   if ((double)rr <= s.thr2 || s.total >= s.maxit) {
-    for (int p = threadIdx.x; p < 2 * N; p += NT) s.x[p] += (double)s.d[p];
+    for (int p = threadIdx.x; p < 2 * N; p += blockDim.x) s.x[p] += (double)s.d[p];
     __syncthreads();
     s.state = SLOT_OUTER;
     return;
   }
   const float beta = rz / s.rz;
   s.rz = rz;
-  for (int i = threadIdx.x; i < 2 * N; i += NT) s.p[i] = s.z[i] + beta * s.p[i];
+  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) s.p[i] = s.z[i] + beta * s.p[i];
   __syncthreads();
 }
 
 }  // namespace srv
 
-// One workgroup per slot; each advances its slot by at most `budget` site-iterations.  Slot descriptors live in
+// One workgroup per slot (blockDim.x a multiple of 64, at most 1024); each advances its slot by at most `budget`
+// site-iterations.  Slot descriptors live in
 // device memory; the workgroup keeps a private copy (identical in all threads) and thread 0 writes it back.
-__global__ __launch_bounds__(srv::NT) void serve_kernel(SlotDesc* __restrict__ slots, int nslots, long long budget) {
-  __shared__ double red64[2 * srv::NT / 64];
-  __shared__ float red32[2 * srv::NT / 64];
+__global__ __launch_bounds__(srv::NT_MAX) void serve_kernel(SlotDesc* __restrict__ slots, int nslots,
+                                                             long long budget) {
+  __shared__ double red64[2 * srv::NT_MAX / 64];
+  __shared__ float red32[2 * srv::NT_MAX / 64];
   const int id = blockIdx.x;
   if (id >= nslots) return;
   SlotDesc s = slots[id];
@@ -307,7 +309,7 @@ __global__ __launch_bounds__(srv::NT) void serve_kernel(SlotDesc* __restrict__ s
     }
   }
   if (s.state == SLOT_DONE) {
-    for (int i = threadIdx.x; i < 2 * N; i += srv::NT) s.up[i] += s.x[i];
+    for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) s.up[i] += s.x[i];
     __syncthreads();
   }
   if (threadIdx.x == 0) {

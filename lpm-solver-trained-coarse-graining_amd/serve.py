@@ -43,14 +43,14 @@ class Request:
 
 
 class Server:
-    def __init__(self, max_slots=512, arena_bytes=None, device="cuda", budget=None, use_graph=True):
+    def __init__(self, max_slots=512, arena_bytes=None, device="cuda", budget=None, use_graph=True, threads=1024):
         L.require_gpu(torch.empty(0, device=device))
         self.dev = torch.device(device)
         self.lib = L.lib()
         if arena_bytes is None:
             arena_bytes = 4 << 30
         h = ctypes.c_void_p()
-        L.check(self.lib.lpm_serve_create(int(max_slots), int(arena_bytes), ctypes.byref(h)))
+        L.check(self.lib.lpm_serve_create(int(max_slots), int(arena_bytes), int(threads), ctypes.byref(h)))
         self.h = h
         self.max_slots = int(max_slots)
         self.budget = int(budget or 64 * 20736)
