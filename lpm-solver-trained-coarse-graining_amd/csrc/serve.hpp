@@ -79,25 +79,38 @@ __device__ __forceinline__ void wg_sum2(T& a, T& b, T* red) {
   __syncthreads();
 }
 
+// aligned short vectors for the per-site accesses (the arena arrays are 256-byte aligned)
+template <typename T, int N>
+struct alignas(sizeof(T) * N) Vec {
+  T v[N];
+};
+template <typename T, int N>
+__device__ __forceinline__ Vec<T, N> vld(const T* __restrict__ p) {
+  return *(const Vec<T, N>*)p;
+}
+
 // shell sums (S0, S1, TS0, TS1) of u at every site into w
 template <typename T>
 __device__ void shell_pass(const SlotDesc& s, const T* __restrict__ u, T* __restrict__ w) {
-  const int N = s.nx * s.ny;
+  const int N = s.nx * s.ny, nx = s.nx, ny = s.ny;
+  const float* __restrict__ tvp = s.tv;
   for (int p = threadIdx.x; p < N; p += blockDim.x) {
-    const int x = p % s.nx, y = p / s.nx;
-    T a[4] = {T(0), T(0), T(0), T(0)};
+    const int x = p % nx, y = p / nx;
+    const Vec<T, 2> ui = vld<T, 2>(u + 2 * p);
+    const Vec<float, 4> t0 = vld<float, 4>(tvp + 8 * p), t1 = vld<float, 4>(tvp + 8 * p + 4);
+    const float tvk[8] = {t0.v[0], t0.v[1], t0.v[2], t0.v[3], t1.v[0], t1.v[1], t1.v[2], t1.v[3]};
+    Vec<T, 4> a;
+    a.v[0] = a.v[1] = a.v[2] = a.v[3] = T(0);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int q = nbr2(x, y, s.nx, s.ny, k);
+      const int q = nbr2(x, y, nx, ny, k);
       if (q < 0) continue;
-      const T dl = nd2<T>(k, 0) * (u[2 * p] - u[2 * q]) + nd2<T>(k, 1) * (u[2 * p + 1] - u[2 * q + 1]);
-      a[shell2(k)] += dl;
-      a[2 + shell2(k)] += (T)s.tv[8 * p + k] * dl;
+      const Vec<T, 2> uq = vld<T, 2>(u + 2 * q);
+      const T dl = nd2<T>(k, 0) * (ui.v[0] - uq.v[0]) + nd2<T>(k, 1) * (ui.v[1] - uq.v[1]);
+      a.v[shell2(k)] += dl;
+      a.v[2 + shell2(k)] += (T)tvk[k] * dl;
     }
-    w[4 * p] = a[0];
-    w[4 * p + 1] = a[1];
-    w[4 * p + 2] = a[2];
-    w[4 * p + 3] = a[3];
+    *(Vec<T, 4>*)(w + 4 * p) = a;
   }
 }
 
@@ -105,17 +118,27 @@ __device__ void shell_pass(const SlotDesc& s, const T* __restrict__ u, T* __rest
 template <typename T>
 __device__ __forceinline__ void ku_site(const SlotDesc& s, const T* __restrict__ u, const T* __restrict__ w, int p,
                                         T& f0, T& f1) {
-  const int x = p % s.nx, y = p / s.nx;
+  const int nx = s.nx, ny = s.ny;
+  const int x = p % nx, y = p / nx;
+  const float* __restrict__ knp = s.kn;
+  const float* __restrict__ tvp = s.tv;
+  const Vec<T, 2> ui = vld<T, 2>(u + 2 * p);
+  const Vec<T, 4> wi = vld<T, 4>(w + 4 * p);
+  const Vec<float, 4> k0 = vld<float, 4>(knp + 8 * p), k1 = vld<float, 4>(knp + 8 * p + 4);
+  const Vec<float, 4> t0 = vld<float, 4>(tvp + 8 * p), t1 = vld<float, 4>(tvp + 8 * p + 4);
+  const float knk[8] = {k0.v[0], k0.v[1], k0.v[2], k0.v[3], k1.v[0], k1.v[1], k1.v[2], k1.v[3]};
+  const float tvk[8] = {t0.v[0], t0.v[1], t0.v[2], t0.v[3], t1.v[0], t1.v[1], t1.v[2], t1.v[3]};
   f0 = f1 = T(0);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const int q = nbr2(x, y, s.nx, s.ny, k);
+    const int q = nbr2(x, y, nx, ny, k);
     if (q < 0) continue;
     const int c = shell2(k);
-    const T dl = nd2<T>(k, 0) * (u[2 * p] - u[2 * q]) + nd2<T>(k, 1) * (u[2 * p + 1] - u[2 * q + 1]);
-    const T tv = (T)s.tv[8 * p + k];
-    const T F = T(2) * (T)s.kn[8 * p + k] * dl + T(0.5) * (w[4 * p + 2 + c] + w[4 * q + 2 + c]) +
-                T(0.5) * tv * (w[4 * p + c] + w[4 * q + c]);
+    const Vec<T, 2> uq = vld<T, 2>(u + 2 * q);
+    const Vec<T, 4> wq = vld<T, 4>(w + 4 * q);
+    const T dl = nd2<T>(k, 0) * (ui.v[0] - uq.v[0]) + nd2<T>(k, 1) * (ui.v[1] - uq.v[1]);
+    const T tv = (T)tvk[k];
+    const T F = T(2) * (T)knk[k] * dl + T(0.5) * (wi.v[2 + c] + wq.v[2 + c]) + T(0.5) * tv * (wi.v[c] + wq.v[c]);
     f0 += F * nd2<T>(k, 0);
     f1 += F * nd2<T>(k, 1);
   }
@@ -231,17 +254,25 @@ __device__ void phase_outer(SlotDesc& s, double* red) {
 // one inner PCG iteration (FP32)
 __device__ void inner_iteration(SlotDesc& s, float* red) {
   const int N = s.nx * s.ny;
-  shell_pass<float>(s, s.p, s.w32);
+  float* __restrict__ pv = s.p;
+  float* __restrict__ apv = s.ap;
+  float* __restrict__ rv = s.r;
+  float* __restrict__ zv = s.z;
+  float* __restrict__ dv = s.d;
+  const float* __restrict__ mi = s.minv;
+  const uint8_t* __restrict__ mk = s.mask;
+  shell_pass<float>(s, pv, s.w32);
   __syncthreads();
   float pap = 0.f, unused = 0.f;
   for (int p = threadIdx.x; p < N; p += blockDim.x) {
     float f0, f1;
-    ku_site<float>(s, s.p, s.w32, p, f0, f1);
-    f0 *= mfree(s, 2 * p);
-    f1 *= mfree(s, 2 * p + 1);
-    s.ap[2 * p] = f0;
-    s.ap[2 * p + 1] = f1;
-    pap += s.p[2 * p] * f0 + s.p[2 * p + 1] * f1;
+    ku_site<float>(s, pv, s.w32, p, f0, f1);
+    Vec<float, 2> o;
+    o.v[0] = mk[2 * p] ? f0 : 0.f;
+    o.v[1] = mk[2 * p + 1] ? f1 : 0.f;
+    *(Vec<float, 2>*)(apv + 2 * p) = o;
+    const Vec<float, 2> pp = vld<float, 2>(pv + 2 * p);
+    pap += pp.v[0] * o.v[0] + pp.v[1] * o.v[1];
   }
   wg_sum2(pap, unused, red);
   if (!(pap > 0.f)) {
@@ -251,31 +282,41 @@ __device__ void inner_iteration(SlotDesc& s, float* red) {
   const float alpha = s.rz / pap;
   float rz = 0.f, rr = 0.f;
   for (int p = threadIdx.x; p < N; p += blockDim.x) {
-    const float* m = s.minv + 3 * p;
     const int i = 2 * p;
-    s.d[i] += alpha * s.p[i];
-    s.d[i + 1] += alpha * s.p[i + 1];
-    const float r0 = s.r[i] - alpha * s.ap[i], r1 = s.r[i + 1] - alpha * s.ap[i + 1];
-    s.r[i] = r0;
-    s.r[i + 1] = r1;
-    const float z0 = m[0] * r0 + m[2] * r1, z1 = m[2] * r0 + m[1] * r1;
-    s.z[i] = z0;
-    s.z[i + 1] = z1;
-    rz += r0 * z0 + r1 * z1;
-    rr += r0 * r0 + r1 * r1;
+    const Vec<float, 2> pp = vld<float, 2>(pv + i), aa = vld<float, 2>(apv + i), r_ = vld<float, 2>(rv + i);
+    Vec<float, 2> dd = vld<float, 2>(dv + i);
+    const float m0 = mi[3 * p], m1 = mi[3 * p + 1], m2 = mi[3 * p + 2];
+    dd.v[0] += alpha * pp.v[0];
+    dd.v[1] += alpha * pp.v[1];
+    *(Vec<float, 2>*)(dv + i) = dd;
+    Vec<float, 2> r2, z2;
+    r2.v[0] = r_.v[0] - alpha * aa.v[0];
+    r2.v[1] = r_.v[1] - alpha * aa.v[1];
+    *(Vec<float, 2>*)(rv + i) = r2;
+    z2.v[0] = m0 * r2.v[0] + m2 * r2.v[1];
+    z2.v[1] = m2 * r2.v[0] + m1 * r2.v[1];
+    *(Vec<float, 2>*)(zv + i) = z2;
+    rz += r2.v[0] * z2.v[0] + r2.v[1] * z2.v[1];
+    rr += r2.v[0] * r2.v[0] + r2.v[1] * r2.v[1];
   }
   wg_sum2(rz, rr, red);
   s.it += 1;
   s.total += 1;
   if ((double)rr <= s.thr2 || s.total >= s.maxit) {
-    for (int p = threadIdx.x; p < 2 * N; p += blockDim.x) s.x[p] += (double)s.d[p];
+    for (int p = threadIdx.x; p < 2 * N; p += blockDim.x) s.x[p] += (double)dv[p];
     __syncthreads();
     s.state = SLOT_OUTER;
     return;
   }
   const float beta = rz / s.rz;
   s.rz = rz;
-  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) s.p[i] = s.z[i] + beta * s.p[i];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    Vec<float, 2> pp = vld<float, 2>(pv + 2 * i);
+    const Vec<float, 2> z2 = vld<float, 2>(zv + 2 * i);
+    pp.v[0] = z2.v[0] + beta * pp.v[0];
+    pp.v[1] = z2.v[1] + beta * pp.v[1];
+    *(Vec<float, 2>*)(pv + 2 * i) = pp;
+  }
   __syncthreads();
 }
 
