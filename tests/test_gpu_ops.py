@@ -247,3 +247,41 @@ def test_pcg_graph_replay_matches_eager(dev, dims):
                          graph=True)
         assert ig.it == ie.it and ie.it > 20
         assert torch.equal(xg, xe)
+
+
+_TILING_REF = {}
+
+
+def _tiling_case():
+    """The 70 x 11 x 40 case of the tiling test with its oracle K u and PCG solution (computed once)."""
+    if not _TILING_REF:
+        dims = (70, 11, 40)
+        lat, Kn, Tv, u = case(dims, seed=17, two_phase=True)
+        fixed, up, _ = R.prescribed3("tb_sx", lat.x, lat.ijk, dims, 1)
+        b = -R.apply(lat, up * fixed[:, None], Kn, Tv) * (~fixed)[:, None]
+        xr, _, _ = R.pcg(lat, b, Kn, Tv, R.precond(lat, Kn, Tv, ~fixed), ~fixed, tol=1e-11, maxit=5000)
+        _TILING_REF.update(dims=dims, lat=lat, Kn=Kn, Tv=Tv, u=u, ref=R.apply(lat, u, Kn, Tv), fixed=fixed, b=b, xr=xr)
+    return _TILING_REF
+
+
+@pytest.mark.parametrize("wide,zc", [(1, 7), (1, 13), (-1, 5), (2, 11), (1, 0)])
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_apply_tilings_and_z_chunks(dev, wide, zc, dt):
+    """Every tile shape and z-chunk length (chunks that end inside the 6-step unrolled group, a last chunk shorter
+    than the others) gives the oracle's K u, packed and directed, plain and inside PCG."""
+    print("tiling", wide, zc, dt, flush=True)
+    c = _tiling_case()
+    dims, Kn, Tv, u, ref = c["dims"], c["Kn"], c["Tv"], c["u"], c["ref"]
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=dt)
+    kp, tp, _ = ops.pack(dims, t(Kn), t(Tv))
+    try:
+        L.lib().lpm_set_ku_tiling(wide, zc)
+        assert rel(ops.apply(dims, 1.0, t(u), kp, tp, layout=L.PACKED), ref) < TOL[dt]
+        assert rel(ops.apply(dims, 1.0, t(u), t(Kn), t(Tv)), ref) < TOL[dt]
+        free = torch.as_tensor(~c["fixed"], device=dev)
+        if dt == torch.float64:
+            x, info = ops.pcg(dims, 1.0, t(c["b"]), kp, tp, minv=ops.block_jacobi(dims, 1.0, kp, tp, free=free,
+                              layout=L.PACKED), free=free, tol=1e-11, maxit=5000, layout=L.PACKED)
+            assert info.conv.all() and rel(x, c["xr"]) < 1e-8
+    finally:
+        L.lib().lpm_set_ku_tiling(0, 0)
