@@ -107,14 +107,14 @@ __device__ __forceinline__ void for_slots(Seq<Ks...>, F&& f) {
 // directed: row 0 of the plane; packed: plane-0 entry of the plane's first particle) and `xy` the site's in-plane
 // index, which stays the same for every plane of a z march: the loads take one loop-invariant lane offset and a
 // uniform per-slot base (packed odd slots: the neighbour's forward-plane entry at p + delta, inside the zero guards).
-// SKIP5 leaves slot 5 (-z) alone: in the packed layout it is slot 4 of the same column one plane below, which the
-// z march already holds in a register.
-template <typename T, typename C, int MODE, bool SKIP5 = false>
+// SKIP: slots (bit K) the caller fills from registers instead (packed layout: slot 5 (-z) is slot 4 of the same
+// column one plane below; slots 1 (-x) and 11 (-x, -z) are slots 0 and 10 of the lane to the left).
+template <typename T, typename C, int MODE, unsigned SKIP = 0u>
 __device__ __forceinline__ void load18(const C* __restrict__ zbase, int64_t ps, unsigned xy, int nx, int nxy,
                                        T (&out)[18]) {
   for_slots(Slots18{}, [&](auto kc) {
     constexpr int K = decltype(kc)::value;
-    if constexpr (SKIP5 && K == 5) {
+    if constexpr ((SKIP >> K) & 1u) {
     } else if constexpr (MODE == COEF_DIRECTED) {
       out[K] = (T)ld_off(zbase, xy * (18u * (unsigned)sizeof(C)) + K * (unsigned)sizeof(C));  // < 4 GiB (host check)
     } else {
@@ -122,6 +122,11 @@ __device__ __forceinline__ void load18(const C* __restrict__ zbase, int64_t ps, 
       out[K] = (T)ld_off(zbase + sh, xy * (unsigned)sizeof(C));
     }
   });
+}
+
+// v of the lane to the left (wave shift right by one lane); lane 0 takes `lane0`
+__device__ __forceinline__ float lane_left(float v, float lane0) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane0), __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
 
 template <typename T, typename C, int MODE, int TX, int TY, int EPI>
@@ -211,6 +216,9 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
   constexpr int NTV = DEEP ? 3 : 2;
   // packed FP32: the -z coefficients come from the previous plane's registers instead of a load
   constexpr bool REUSE5 = DEEP && MODE == COEF_PACKED;
+  // 64-wide tiles: a wave is one row of the tile, so the -x neighbour of a site is the lane to the left
+  constexpr bool LANEX = REUSE5 && TX == 64;
+  constexpr unsigned SKIPK = LANEX ? ((1u << 1) | (1u << 5) | (1u << 11)) : (REUSE5 ? (1u << 5) : 0u);
   T DD[2][18], SS[2][4], TV[NTV][18], KN[1][18], TR[2][18];
 
   auto fetch_u = [&](int z) {
@@ -278,6 +286,14 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
     s[3] = t1;
   };
 
+  // entry of forward plane `plane` at the site left of the wave's first site, shifted by `dz` sites (a uniform
+  // address: one scalar load per wave)
+  const unsigned row0 = __builtin_amdgcn_readfirstlane(own_xy_off);
+  auto lane0_entry = [&](const C* zbase, int plane, int dz) -> T {
+    return (T)zbase[(int64_t)plane * ps + (int64_t)row0 - 1 + dz];
+  };
+  T tl1 = T(0), tl11 = T(0), kl1 = T(0), kl11 = T(0), k10 = T(0);
+
   // ---- prologue: u planes zb-2 .. zb+2 in LDS; shell sums of planes zb-1 and zb
   for (int z = zb - 2; z <= zb + 2; ++z) {
     fetch_u(z);
@@ -300,10 +316,15 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
     if (has_r) shell_site(Drop{}, R0{}, zb, rsx, rsy, rmx, rpx, rmy, rpy, TR[0], DD[1], tmp);
     if constexpr (DEEP) {
       if constexpr (REUSE5) {
-        load18<T, C, MODE, true>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
+        load18<T, C, MODE, SKIPK>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
         TV[1][5] = TV[0][4];
+        if constexpr (LANEX) {
+          tl1 = lane0_entry(tv_z(zb + 1), 0, 0);
+          tl11 = lane0_entry(tv_z(zb + 1), 5, -nxy);
+          KN[0][10] = (T)ld_off(kn_z(zb - 1) + 5 * ps, own_xy_off * (unsigned)sizeof(C));  // slot 10 below zb
+        }
         KN[0][4] = (T)ld_off(kn_z(zb - 1) + 2 * ps, own_xy_off * (unsigned)sizeof(C));  // slot 4 below zb
-        load18<T, C, MODE, true>(tv_z(zb + 1), ps, r_xy_off, nx, nxy, TR[1]);
+        load18<T, C, MODE, (1u << 5)>(tv_z(zb + 1), ps, r_xy_off, nx, nxy, TR[1]);
         TR[1][5] = TR[0][4];
       } else {
         load18<T, C, MODE>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
@@ -335,11 +356,23 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
       // output plane during the shell phase
       if constexpr (REUSE5) {
         const T k4 = knC[4];  // own Kn of plane z-1
-        load18<T, C, MODE, true>(tv_z(z + 2), ps, own_xy_off, nx, nxy, TV[(J + 2) % NTV]);
+        if constexpr (LANEX) {
+          // slots 1 and 11 of the Tv plane that becomes the shell plane now: from the lane to the left
+          tvN[1] = lane_left(tvN[0], tl1);
+          tvN[11] = lane_left(tvC[10], tl11);
+          k10 = knC[10];  // own Kn of plane z-1, slot 10
+        }
+        load18<T, C, MODE, SKIPK>(tv_z(z + 2), ps, own_xy_off, nx, nxy, TV[(J + 2) % NTV]);
         TV[(J + 2) % NTV][5] = TV[(J + 1) % NTV][4];
-        load18<T, C, MODE, true>(tv_z(z + 2), ps, r_xy_off, nx, nxy, TR[J % 2]);
+        if constexpr (LANEX) {
+          tl1 = lane0_entry(tv_z(z + 2), 0, 0);
+          tl11 = lane0_entry(tv_z(z + 2), 5, -nxy);
+          kl1 = lane0_entry(kn_z(z), 0, 0);
+          kl11 = lane0_entry(kn_z(z), 5, -nxy);
+        }
+        load18<T, C, MODE, (1u << 5)>(tv_z(z + 2), ps, r_xy_off, nx, nxy, TR[J % 2]);
         TR[J % 2][5] = tvR[4];
-        load18<T, C, MODE, true>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
+        load18<T, C, MODE, SKIPK>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
         knC[5] = k4;
       } else {
         load18<T, C, MODE>(tv_z(z + 2), ps, own_xy_off, nx, nxy, TV[(J + 2) % NTV]);
@@ -359,6 +392,10 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
     if constexpr (!DEEP) {
       load18<T, C, MODE>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
       load18<T, C, MODE>(tv_z(z), ps, own_xy_off, nx, nxy, tvC);
+    }
+    if constexpr (LANEX) {
+      knC[1] = lane_left(knC[0], kl1);
+      knC[11] = lane_left(k10, kl11);
     }
     if (own_xy) {
       const bool mx = omx, px = opx, my = omy, py = opy, mz = z > 0, pz = z < nz - 1;
