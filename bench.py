@@ -359,6 +359,168 @@ def bench_kubc3d(a):
                             "inner iterations over the 9 RHS" % (done, t_it)}})
 
 
+def bench_porous256(a):
+    """256^3 cement-like microstructure with the pore particles removed (bond words), tension z, FP32 PCG + FP64
+    refinement to 1e-8; the masked K p kernel's roofline and the C oracle's PCG on the same reduced lattice."""
+    import torch
+    import lpm_ops as ops
+    import _lpmlib as L
+    import workloads
+    dev = torch.device("cuda:0")
+    n = a.n
+    t0 = time.perf_counter()
+    P = workloads.porous3d(n, a.seed, dev)
+    dims, w = P["dims"], P["words"]
+    N = n ** 3
+    print("[bench] porous %d^3 ready: solid %.3f, %.1f s" % (n, P["solid_fraction"], time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+
+    def step():
+        b = -ops.apply(dims, 1.0, P["up"], P["kp64"], P["tp64"], layout=L.PACKED, bonds=w) * P["free"][:, None]
+        M = ops.block_jacobi(dims, 1.0, P["kp64"], P["tp64"], free=P["free"], out_dtype=torch.float32,
+                             layout=L.PACKED, bonds=w)
+        return ops.pcg_refined(dims, 1.0, b, P["kp64"], P["tp64"], P["kp32"], P["tp32"], minv32=M, free=P["free"],
+                               tol=a.tol, maxit=100000, inner_tol=1e-4, layout64=L.PACKED, layout32=L.PACKED,
+                               bonds=w)
+    for wi in range(a.warmup):
+        _, info = step()
+        torch.cuda.synchronize()
+        print("[bench] warmup %d: %d its, %d outer, relres %.2e" % (wi, info.it, info.outer, info.relmax),
+              file=sys.stderr, flush=True)
+    L.lib().lpm_profile_ku(1)
+    lat, its = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s_ in range(a.steps):
+        ts = time.perf_counter()
+        x, info = step()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+        its.append(info.it)
+        print("[bench] step %d: %.3f s, %d its, relres %.2e" % (s_, lat[-1], info.it, info.relmax), file=sys.stderr,
+              flush=True)
+    total = time.perf_counter() - t0
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    L.lib().lpm_profile_ku_read(ctypes.byref(ms), ctypes.byref(cnt))
+    L.lib().lpm_profile_ku(0)
+    ku_ms = ms.value / max(cnt.value, 1)
+    ku_bytes = N * (2 * 3 * 4 + 18 * 4 + 4)                 # u in + Ku out + 18 coefficients + bond word, FP32
+    achieved = ku_bytes / (ku_ms * 1e-3) / 1e9
+    lat_ms = np.array(lat) * 1e3
+    extra = {"latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+             "solve": {"pcg_iters": int(np.median(its)), "outer": info.outer, "relres_fp64": info.relmax,
+                       "solid_fraction": P["solid_fraction"], "removed_by_cleaning": P["removed_by_cleaning"]},
+             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                          "kernel": "ku3d_kernel<float,float,PACKED,64,4,MASKDOT,BONDS>", "launches": cnt.value,
+                          "avg_us": ku_ms * 1e3, "alg_bytes_per_launch": ku_bytes}}
+    if not a.no_cpu:
+        extra["cpu_baseline"] = _porous_cpu(P, int(np.median(its)), a.cpu_budget)
+        extra["fp64_oracle_64cube"] = _porous_subvolume(P["lab"], n, dev)
+    _line(METRIC, a.steps / total, "solves/s", a.steps, a.warmup, total / a.steps * 1e3,
+          "f32 inner + f64 refinement",
+          "SC LPM %d^3 box, cement-like with pore particles removed (solid %.3f), tension z, block-Jacobi PCG FP32 "
+          "+ FP64 refinement to rel-res %g" % (n, P["solid_fraction"], a.tol), extra)
+
+
+def _masked_nbr(n, solid):
+    """(N, 18) int64 neighbour table of the box with -1 where a slot leaves the box or either end is removed."""
+    N = n ** 3
+    p = np.arange(N, dtype=np.int64)
+    ijk = np.stack([p % n, (p // n) % n, p // (n * n)], 1)
+    nbr = np.empty((N, 18), np.int64)
+    offs = np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, -1, 0),
+                     (1, -1, 0), (-1, 1, 0), (1, 0, 1), (-1, 0, -1), (1, 0, -1), (-1, 0, 1), (0, 1, 1), (0, -1, -1),
+                     (0, 1, -1), (0, -1, 1)])
+    for k, o in enumerate(offs):
+        q = ijk + o
+        ok = np.all((q >= 0) & (q < n), 1)
+        qq = np.where(ok, (q[:, 2] * n + q[:, 1]) * n + q[:, 0], 0)
+        nbr[:, k] = np.where(ok & solid & solid[qq], qq, -1)
+    return nbr
+
+
+def _porous_cpu(P, gpu_iters, budget_s):
+    """C oracle block-Jacobi PCG (FP64, OpenMP) on the same reduced lattice for a bounded number of iterations."""
+    import lpm_ops as ops
+    import _lpmlib as L
+    import synth
+    lib = _oracle_lib()
+    dims, w = P["dims"], P["words"]
+    n = dims[0]
+    N = n ** 3
+    Kn, Tv = synth.directed_from_packed(dims, P["kp64"], P["tp64"])
+    Kn = np.ascontiguousarray(Kn.cpu().numpy())
+    Tv = np.ascontiguousarray(Tv.cpu().numpy())
+    M = ops.full_from_sym(ops.block_jacobi(dims, 1.0, P["kp64"], P["tp64"], free=P["free"], layout=L.PACKED,
+                                           bonds=w))
+    M = np.ascontiguousarray(M.cpu().numpy())
+    b = -ops.apply(dims, 1.0, P["up"], P["kp64"], P["tp64"], layout=L.PACKED, bonds=w) * P["free"][:, None]
+    b = np.ascontiguousarray(b.cpu().numpy())
+    fr = np.ascontiguousarray((P["free"] > 0).cpu().numpy().astype(np.uint8))
+    nbr = _masked_nbr(n, P["solid"])
+    ptr = lambda v: v.ctypes.data_as(ctypes.c_void_p)
+    rel = np.zeros(1)
+    x = np.zeros_like(b)
+    t0 = time.perf_counter()
+    lib.orc_pcg3(N, ptr(nbr), ptr(Kn), ptr(Tv), ptr(M), ptr(fr), ptr(b), ptr(x), 0.0, 2, ptr(rel))
+    t_probe = (time.perf_counter() - t0) / 2
+    its = int(max(3, min(200, budget_s / max(t_probe, 1e-6))))
+    x[:] = 0
+    t0 = time.perf_counter()
+    done = lib.orc_pcg3(N, ptr(nbr), ptr(Kn), ptr(Tv), ptr(M), ptr(fr), ptr(b), ptr(x), 0.0, its, ptr(rel))
+    t_it = (time.perf_counter() - t0) / max(done, 1)
+    return {"value": 1.0 / (t_it * gpu_iters), "unit": "solves/s", "cores": int(lib.orc_threads()), "kind": "port",
+            "sample": "%d block-Jacobi PCG iterations (FP64, C/OpenMP restatement of coarse3d/lin3d.py:464-518) on "
+                      "the same reduced %d^3 lattice: %.3f s per iteration, scaled by the GPU solve's %d PCG "
+                      "iterations" % (done, n, t_it, gpu_iters)}
+
+
+def _porous_subvolume(lab, n_full, dev, n=64):
+    """GPU (bond words, FP32 + FP64 refinement to 1e-8) vs the C oracle's FP64 PCG to 1e-13 on the reduced lattice
+    (compact numbering) of the n^3 corner of the same labels, its solid set cleaned again."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lpm_ref as R
+    import lpm_ops as ops
+    import _lpmlib as L
+    import workloads
+    sub = lab.reshape(n_full, n_full, n_full)[:n, :n, :n].ravel()
+    P = workloads.porous3d(n, 0, dev, lab=sub)
+    dims, w = P["dims"], P["words"]
+    b = -ops.apply(dims, 1.0, P["up"], P["kp64"], P["tp64"], layout=L.PACKED, bonds=w) * P["free"][:, None]
+    M = ops.block_jacobi(dims, 1.0, P["kp64"], P["tp64"], free=P["free"], out_dtype=torch.float32, layout=L.PACKED,
+                         bonds=w)
+    x, info = ops.pcg_refined(dims, 1.0, b, P["kp64"], P["tp64"], P["kp32"], P["tp32"], minv32=M, free=P["free"],
+                              tol=1e-8, maxit=100000, inner_tol=1e-4, layout64=L.PACKED, layout32=L.PACKED, bonds=w)
+    ug = (P["up"] + x).cpu().numpy()
+    lat = R.ActiveBox(dims, P["solid"])
+    Kf, Tf = R.two_end_mean(R.Box(dims), P["ppar"])
+    Kn, Tv = Kf[lat.sites], Tf[lat.sites]
+    fixed = P["fixed"].cpu().numpy()[lat.sites]
+    up = P["up"].cpu().numpy()[lat.sites]
+    bo = np.ascontiguousarray(-R.apply(lat, up, Kn, Tv) * (~fixed)[:, None])
+    Mo = np.ascontiguousarray(R.precond(lat, Kn, Tv, ~fixed))
+    lib = _oracle_lib()
+    pt = lambda v: v.ctypes.data_as(ctypes.c_void_p)
+    xo = np.zeros_like(bo)
+    rel = np.zeros(1)
+    it = lib.orc_pcg3(lat.N, pt(np.ascontiguousarray(lat.nbr.astype(np.int64))), pt(np.ascontiguousarray(Kn)),
+                      pt(np.ascontiguousarray(Tv)), pt(Mo), pt(np.ascontiguousarray((~fixed).astype(np.uint8))),
+                      pt(bo), pt(xo), 1e-13, 100000, pt(rel))
+    uo = up + xo
+    ugc = ug[lat.sites]
+    so, sg = R.stress(lat, uo, Kn, Tv), R.stress(lat, ugc, Kn, Tv)
+    top = lat.ijk[:, 2] == n - 1
+    Fo, Fg = R.apply(lat, uo, Kn, Tv)[top, 2].sum(), R.apply(lat, ugc, Kn, Tv)[top, 2].sum()
+    return {"subvolume": "%d^3 corner, pores removed, tension z" % n, "solid_particles": int(lat.N),
+            "gpu_relres_fp64": float(info.relmax), "oracle_iters": int(it), "oracle_relres": float(rel[0]),
+            "rel_err_u": float(np.abs(ugc - uo).max() / np.abs(uo).max()),
+            "rel_err_stress": float(np.abs(sg - so).max() / np.abs(so).max()),
+            "rel_err_reaction": float(abs(Fg - Fo) / abs(Fo))}
+
+
 def bench_serve(a):
     """Continuous batching of 512 coarse 2D requests (24^2-144^2, FP32 PCG + FP64 refinement to 1e-8)."""
     import torch
@@ -454,7 +616,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--probe", type=int, default=0, help="run the profiler probe on an n^3 lattice and exit")
     ap.add_argument("--probe-reps", type=int, default=10)
-    ap.add_argument("--config", default="cube256", choices=["cube256", "plain2d", "kubc3d", "serve", "train3d"])
+    ap.add_argument("--config", default="cube256",
+                    choices=["cube256", "plain2d", "kubc3d", "serve", "train3d", "porous256"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--slots", type=int, default=512)
     ap.add_argument("--budget", type=int, default=400000)
@@ -470,7 +633,8 @@ def main():
               "single device (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if a.config != "cube256":
-        {"plain2d": bench_plain2d, "kubc3d": bench_kubc3d, "serve": bench_serve, "train3d": bench_train3d}[a.config](a)
+        {"plain2d": bench_plain2d, "kubc3d": bench_kubc3d, "serve": bench_serve, "train3d": bench_train3d,
+         "porous256": bench_porous256}[a.config](a)
         return
     import torch
     dev = torch.device("cuda:0")

@@ -37,14 +37,22 @@ enum { LPM_F32 = 0, LPM_F64 = 1 };
 enum { LPM_COEF_DIRECTED = 0, LPM_COEF_PACKED = 1 };
 
 typedef struct {
+  int64_t sb, sl; /* element strides of the (b, l) batch indices; 0 = broadcast */
+} lpm_bcast;
+
+typedef struct {
   int32_t dim;        /* 2 or 3 */
   int32_t nx, ny, nz; /* nz = 1 in 2D */
   double h;           /* lattice spacing */
+  /* Optional removed particles (voids), the per-specimen validity of cement3d/torch_v1.py:36-53: NULL, or bond
+   * words uint32 (.., N) per system at bonds + b*bb.sb + l*bb.sl (the (b, l) indices of the call's coefficient
+   * batch; lpm_bond_grad / lpm_post_vjp: (coefficient batch, load)).  Bit k of particle p set = slot k is a bond;
+   * a bond must be set at both ends (bit k at p, bit k^1 at its neighbour), bits of slots leaving the box are
+   * ignored.  Every operator, preconditioner, solver and post-processing call then uses only these bonds
+   * (shell sums, forces, stress opp-flag rule, block diagonal); particles with no bond must be fixed. */
+  const uint32_t* bonds;
+  lpm_bcast bb;
 } lpm_grid;
-
-typedef struct {
-  int64_t sb, sl; /* element strides of the (b, l) batch indices; 0 = broadcast */
-} lpm_bcast;
 
 typedef struct {
   int32_t precond; /* 0 = none (plain CG), 1 = particle block Jacobi */
@@ -67,6 +75,11 @@ void lpm_profile_ku(int32_t enable);
  * per block (0 automatic).  Process-wide; no counterpart in the reference (tuning knob). */
 void lpm_set_ku_tiling(int32_t wide, int32_t zc);
 void lpm_profile_ku_read(double* total_ms, int64_t* launches);
+
+/* Bond words from solid flags (nbatch, N) bytes, nonzero = solid: bit k of p set iff slot k stays in the box and
+ * p and its neighbour are both solid; words (nbatch, N) uint32.  Replaces SCTV.set_mask (cement3d/torch_v1.py:36-53),
+ * which builds the same per-specimen validity as a (B, 1, N, 18) tensor.  g->bonds is not used. */
+int lpm_bond_words(const lpm_grid* g, int32_t nbatch, const uint8_t* solid, uint32_t* words, void* stream);
 
 /* out (nb, nl, N, dim) = K u.  Replaces SCT.apply (coarse3d/torch_lpm3d.py:81-83), lin3d.k_apply
  * (coarse3d/lin3d.py:126-149) and the 2D _kmv (timing/t2d/solver2d.py:269-305). */

@@ -16,13 +16,13 @@ F32, F64 = 0, 1
 DIRECTED, PACKED = 0, 1
 
 
-class Grid(ctypes.Structure):
-    _fields_ = [("dim", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
-                ("h", ctypes.c_double)]
-
-
 class Bcast(ctypes.Structure):
     _fields_ = [("sb", ctypes.c_int64), ("sl", ctypes.c_int64)]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("h", ctypes.c_double), ("bonds", ctypes.c_void_p), ("bb", Bcast)]
 
 
 class CgOpts(ctypes.Structure):
@@ -42,6 +42,7 @@ SIGS = {
     "lpm_set_ku_tiling": (None, [I32, I32]),
     "lpm_packed_len": (ctypes.c_int64, [GP]),
     "lpm_profile_ku_read": (None, [P, P]),
+    "lpm_bond_words": (I32, [GP, I32, P, P, P]),
     "lpm_apply": (I32, [GP, I32, I32, I32, I32, P, Bcast, P, P, Bcast, P, P]),
     "lpm_pack_coef": (I32, [GP, I32, I32, I32, P, P, P, P, ctypes.c_double, P, P]),
     "lpm_block_jacobi": (I32, [GP, I32, I32, I32, I32, I32, P, P, Bcast, P, Bcast, P, I32, P]),
@@ -114,13 +115,16 @@ def require_gpu(*ts):
             raise RuntimeError("the LPM solve path runs on the GPU only (got a tensor on %s)" % t.device)
 
 
-def grid_of(dims, h):
+def grid_of(dims, h, bonds=None, bb=None):
+    """lpm_grid of a box; `bonds` (a device tensor of bond words, kept alive by the caller) with strides bb."""
     dims = tuple(int(d) for d in dims)
     g = Grid()
     g.dim = len(dims)
     g.nx, g.ny = dims[0], dims[1]
     g.nz = dims[2] if len(dims) == 3 else 1
     g.h = float(h)
+    g.bonds = bonds.data_ptr() if bonds is not None else None
+    g.bb = bb if bb is not None else Bcast(0, 0)
     return g
 
 

@@ -16,11 +16,13 @@ struct Ku2Smem {
 };
 
 // K u on the tile [x0, x0+TX) x [y0, y0+TY) of one system; the result for the thread's own site is returned in
-// (fx, fy) together with its u; `own` tells whether the thread owns a site of the lattice.
-template <typename T, typename C, int MODE, int TX, int TY>
+// (fx, fy) together with its u; `own` tells whether the thread owns a site of the lattice.  BONDS: bond words of
+// removed particles (bit k of particle p = slot k is a bond) restrict the bonds further.
+template <typename T, typename C, int MODE, int TX, int TY, bool BONDS = false>
 __device__ __forceinline__ void ku2d_tile(const T* __restrict__ u, const C* __restrict__ kn, const C* __restrict__ tv,
                                           int nx, int ny, int64_t N, int x0, int y0, Ku2Smem<T, TX, TY>& sm,
-                                          T& fx, T& fy, T& ux, T& uy, bool& own, int64_t& pown) {
+                                          T& fx, T& fy, T& ux, T& uy, bool& own, int64_t& pown,
+                                          const uint32_t* __restrict__ bwd = nullptr) {
   // N: plane stride of packed coefficients (unused for directed ones)
   constexpr int HX = TX + 4, HY = TY + 4, SX = TX + 2, SY = TY + 2, NT = TX * TY;
   const int tid = threadIdx.x;
@@ -45,11 +47,12 @@ __device__ __forceinline__ void ku2d_tile(const T* __restrict__ u, const C* __re
       const int64_t p = (int64_t)gy * nx + gx;
       const int hx = sx + 1, hy = sy + 1;
       const T vx = sm.u[0][hy][hx], vy = sm.u[1][hy][hx];
+      const uint32_t w = BONDS ? bwd[p] : 0xffu;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const int ox = off2(k, 0), oy = off2(k, 1);
         const int qx = gx + ox, qy = gy + oy;
-        if (qx < 0 || qx >= nx || qy < 0 || qy >= ny) continue;
+        if (qx < 0 || qx >= nx || qy < 0 || qy >= ny || !((w >> k) & 1u)) continue;
         T dl = T(0);
         if (ox) dl += nd2<T>(k, 0) * (vx - sm.u[0][hy + oy][hx + ox]);
         if (oy) dl += nd2<T>(k, 1) * (vy - sm.u[1][hy + oy][hx + ox]);
@@ -83,11 +86,12 @@ __device__ __forceinline__ void ku2d_tile(const T* __restrict__ u, const C* __re
     ux = sm.u[0][hy][hx];
     uy = sm.u[1][hy][hx];
     const T si0 = sm.s[0][sy][sx], si1 = sm.s[1][sy][sx], ti0 = sm.s[2][sy][sx], ti1 = sm.s[3][sy][sx];
+    const uint32_t w = BONDS ? bwd[p] : 0xffu;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int ox = off2(k, 0), oy = off2(k, 1);
       const int qx = gx + ox, qy = gy + oy;
-      if (qx < 0 || qx >= nx || qy < 0 || qy >= ny) continue;
+      if (qx < 0 || qx >= nx || qy < 0 || qy >= ny || !((w >> k) & 1u)) continue;
       T dl = T(0);
       if (ox) dl += nd2<T>(k, 0) * (ux - sm.u[0][hy + oy][hx + ox]);
       if (oy) dl += nd2<T>(k, 1) * (uy - sm.u[1][hy + oy][hx + ox]);
@@ -125,9 +129,11 @@ struct Ku2Args {
   T* partial;
   const int* active;  // per system flag at active[s * act_stride]; null = all active
   int act_stride;
+  const uint32_t* bonds;  // bond words (BONDS kernels)
+  Bcast bb;
 };
 
-template <typename T, typename C, int MODE, int TX, int TY, int EPI>
+template <typename T, typename C, int MODE, int TX, int TY, int EPI, bool BONDS = false>
 __global__ __launch_bounds__(TX* TY) void ku2d_kernel(Ku2Args<T, C> a) {
   __shared__ Ku2Smem<T, TX, TY> sm;
   __shared__ T red[TX * TY / 64];
@@ -140,8 +146,9 @@ __global__ __launch_bounds__(TX* TY) void ku2d_kernel(Ku2Args<T, C> a) {
   T fx, fy, ux, uy;
   bool own;
   int64_t p;
-  ku2d_tile<T, C, MODE, TX, TY>(a.u + a.bu.at(b, l), a.kn + a.bc.at(b, l), a.tv + a.bc.at(b, l), a.nx, a.ny, a.ps, x0, y0,
-                                sm, fx, fy, ux, uy, own, p);
+  ku2d_tile<T, C, MODE, TX, TY, BONDS>(a.u + a.bu.at(b, l), a.kn + a.bc.at(b, l), a.tv + a.bc.at(b, l), a.nx, a.ny,
+                                       a.ps, x0, y0, sm, fx, fy, ux, uy, own, p,
+                                       BONDS ? a.bonds + a.bb.at(b, l) : nullptr);
   T acc = T(0);
   if (own) {
     T* o = a.out + a.bo.at(b, l) + 2 * p;

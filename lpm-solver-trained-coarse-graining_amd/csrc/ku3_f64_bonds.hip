@@ -1,0 +1,4 @@
+// K u launchers and tile kernels: ku3 f64 bonds (a translation unit of its own so the library builds in parallel).
+#include "ku_launch.hpp"
+
+LPM_INSTANTIATE_KU3(double, true)

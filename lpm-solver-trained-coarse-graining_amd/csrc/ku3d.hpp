@@ -54,6 +54,8 @@ struct Ku3Args {
   T* partial;         // (systems, blocks per system)
   const int* active;  // per system flag at active[s * act_stride]; null = all active
   int act_stride;
+  const uint32_t* bonds;  // bond words (removed particles), per particle: bit k = slot k is a bond; BONDS kernels
+  Bcast bb;
 };
 
 // blockIdx.x -> logical tile id, contiguous per XCD (blocks are dispatched round-robin over the 8 XCDs).
@@ -129,7 +131,7 @@ __device__ __forceinline__ float lane_left(float v, float lane0) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane0), __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
 
-template <typename T, typename C, int MODE, int TX, int TY, int EPI>
+template <typename T, typename C, int MODE, int TX, int TY, int EPI, bool BONDS = false>
 __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1) void ku3d_kernel(Ku3Args<T, C> a) {
   constexpr int HX = TX + 4, HY = TY + 4;  // u tile with a 2-site halo
   constexpr int SX = TX + 2, SY = TY + 2;  // shell sums with a 1-site halo
@@ -160,6 +162,7 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
   const T* __restrict__ u = a.u + a.bu.at(b, l);
   const C* __restrict__ kn = a.kn + a.bc.at(b, l);
   const C* __restrict__ tv = a.tv + a.bc.at(b, l);
+  const uint32_t* __restrict__ bwd = BONDS ? a.bonds + a.bb.at(b, l) : nullptr;
 
   // Site coordinates are clamped into the box for addressing.  Results of halo sites outside the box are never
   // read through an existing bond, so they need no masking.
@@ -220,6 +223,14 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
   constexpr bool LANEX = REUSE5 && TX == 64;
   constexpr unsigned SKIPK = LANEX ? ((1u << 1) | (1u << 5) | (1u << 11)) : (REUSE5 ? (1u << 5) : 0u);
   T DD[2][18], SS[2][4], TV[NTV][18], KN[1][18], TR[2][18];
+  // bond words (BONDS): own site per plane in the Tv rotation, ring site per plane in the ring-Tv rotation
+  uint32_t WO[NTV], WR[2];
+  auto word = [&](int z, unsigned xy) -> uint32_t {
+    if constexpr (BONDS)
+      return ld_off(bwd + (int64_t)cl(z, nz) * nxy, xy * 4u);
+    else
+      return 0xffffffffu;
+  };
 
   auto fetch_u = [&](int z) {
     const T* uz = u + (int64_t)cl(z, nz) * nxy * 3;
@@ -249,7 +260,7 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
   // the site; optionally keeps the 18 elongations.  Writes S0, sqrt2 S1, TS0, sqrt2 TS1 to the LDS ring and
   // returns them in s.
   auto shell_site = [&](auto keep_tag, auto rel_tag, int z, int sx, int sy, bool mx, bool px, bool my, bool py,
-                        const T (&tvv)[18], T (&keep)[18], T (&s)[4]) {
+                        const T (&tvv)[18], uint32_t wv, T (&keep)[18], T (&s)[4]) {
     constexpr bool KEEP = decltype(keep_tag)::value;
     constexpr int R = decltype(rel_tag)::value;
     const int zc_ = cl(z, nz);
@@ -265,7 +276,9 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
       constexpr int oz = off3(K, 2);
       const T(*uq)[HY][HX] = oz < 0 ? u0 : (oz > 0 ? u2 : u1);
       T d = dle3<T, K, HY, HX>(uq, hy, hx, ux, uy, uz);
-      d = slot_ok<K>(mx, px, my, py, mz, pz) ? d : T(0);
+      bool ok = slot_ok<K>(mx, px, my, py, mz, pz);
+      if constexpr (BONDS) ok = ok && ((wv >> K) & 1u);
+      d = ok ? d : T(0);
       if constexpr (KEEP) keep[K] = d;
       if constexpr (shell3(K) == 0) {
         s0 += d;
@@ -306,14 +319,18 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
     T tmp[4];
     load18<T, C, MODE>(tv_z(zb - 1), ps, own_xy_off, nx, nxy, TV[1]);
     load18<T, C, MODE>(tv_z(zb - 1), ps, r_xy_off, nx, nxy, TR[0]);
+    WO[1] = word(zb - 1, own_xy_off);
+    WR[0] = word(zb - 1, r_xy_off);
     using Rm1 = std::integral_constant<int, 5>;
     using R0 = std::integral_constant<int, 0>;
-    shell_site(Drop{}, Rm1{}, zb - 1, tx + 1, ty + 1, omx, opx, omy, opy, TV[1], DD[1], tmp);
-    if (has_r) shell_site(Drop{}, Rm1{}, zb - 1, rsx, rsy, rmx, rpx, rmy, rpy, TR[0], DD[1], tmp);
+    shell_site(Drop{}, Rm1{}, zb - 1, tx + 1, ty + 1, omx, opx, omy, opy, TV[1], WO[1], DD[1], tmp);
+    if (has_r) shell_site(Drop{}, Rm1{}, zb - 1, rsx, rsy, rmx, rpx, rmy, rpy, TR[0], WR[0], DD[1], tmp);
     load18<T, C, MODE>(tv_z(zb), ps, own_xy_off, nx, nxy, TV[0]);
     load18<T, C, MODE>(tv_z(zb), ps, r_xy_off, nx, nxy, TR[0]);
-    shell_site(Keep{}, R0{}, zb, tx + 1, ty + 1, omx, opx, omy, opy, TV[0], DD[0], SS[0]);
-    if (has_r) shell_site(Drop{}, R0{}, zb, rsx, rsy, rmx, rpx, rmy, rpy, TR[0], DD[1], tmp);
+    WO[0] = word(zb, own_xy_off);
+    WR[0] = word(zb, r_xy_off);
+    shell_site(Keep{}, R0{}, zb, tx + 1, ty + 1, omx, opx, omy, opy, TV[0], WO[0], DD[0], SS[0]);
+    if (has_r) shell_site(Drop{}, R0{}, zb, rsx, rsy, rmx, rpx, rmy, rpy, TR[0], WR[0], DD[1], tmp);
     if constexpr (DEEP) {
       if constexpr (REUSE5) {
         load18<T, C, MODE, SKIPK>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
@@ -329,6 +346,8 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
       } else {
         load18<T, C, MODE>(tv_z(zb + 1), ps, own_xy_off, nx, nxy, TV[1]);
       }
+      WO[1] = word(zb + 1, own_xy_off);
+      WR[1] = word(zb + 1, r_xy_off);
     }
   }
   __syncthreads();
@@ -350,6 +369,9 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
     T(&tvN)[18] = TV[(J + 1) % NTV];
     T(&knC)[18] = KN[0];
     T(&tvR)[18] = TR[(J + 1) % 2];
+    uint32_t& wC = WO[J % NTV];
+    uint32_t& wN = WO[(J + 1) % NTV];
+    uint32_t& wR = WR[(J + 1) % 2];
     fetch_u(z + 3);  // lands during this step
     if constexpr (DEEP) {
       // own Tv a full step ahead (it feeds both phases); the ring's Tv lands during the own shell sums, Kn of the
@@ -379,14 +401,26 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
         load18<T, C, MODE>(tv_z(z + 1), ps, r_xy_off, nx, nxy, tvR);
         load18<T, C, MODE>(kn_z(z), ps, own_xy_off, nx, nxy, knC);
       }
+      WO[(J + 2) % NTV] = word(z + 2, own_xy_off);
+      // the non-REUSE5 branch loads the ring's Tv of plane z + 1 into tvR; REUSE5 loads plane z + 2 into TR[J % 2]
+      if constexpr (REUSE5)
+        WR[J % 2] = word(z + 2, r_xy_off);
+      else
+        wR = word(z + 1, r_xy_off);
     }
     {
       T tmp[4];
-      if constexpr (!DEEP) load18<T, C, MODE>(tv_z(z + 1), ps, own_xy_off, nx, nxy, tvN);
+      if constexpr (!DEEP) {
+        load18<T, C, MODE>(tv_z(z + 1), ps, own_xy_off, nx, nxy, tvN);
+        wN = word(z + 1, own_xy_off);
+      }
       using RN = std::integral_constant<int, (J + 1) % 6>;
-      shell_site(Keep{}, RN{}, z + 1, sx, sy, omx, opx, omy, opy, tvN, dN, sN);
-      if constexpr (!DEEP) load18<T, C, MODE>(tv_z(z + 1), ps, r_xy_off, nx, nxy, tvR);
-      if (has_r) shell_site(Drop{}, RN{}, z + 1, rsx, rsy, rmx, rpx, rmy, rpy, tvR, dN, tmp);
+      shell_site(Keep{}, RN{}, z + 1, sx, sy, omx, opx, omy, opy, tvN, wN, dN, sN);
+      if constexpr (!DEEP) {
+        load18<T, C, MODE>(tv_z(z + 1), ps, r_xy_off, nx, nxy, tvR);
+        wR = word(z + 1, r_xy_off);
+      }
+      if (has_r) shell_site(Drop{}, RN{}, z + 1, rsx, rsy, rmx, rpx, rmy, rpy, tvR, wR, dN, tmp);
     }
     __syncthreads();
     if constexpr (!DEEP) {
@@ -417,7 +451,9 @@ __global__ __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY <= 256) ? 2 : 1)
           const T t = tvC[K] * (sC[1] + sq[1][sy + oy][sx + ox]) + (sC[3] + sq[3][sy + oy][sx + ox]);
           g = knC[K] * dC[K] + T(0.25) * t;
         }
-        g = slot_ok<K>(mx, px, my, py, mz, pz) ? g : T(0);
+        bool ok = slot_ok<K>(mx, px, my, py, mz, pz);
+        if constexpr (BONDS) ok = ok && ((wC >> K) & 1u);
+        g = ok ? g : T(0);
         // (K u)_i += F n_k, n_k = -off_k / |off_k|; for diagonal slots F n_k = -G off_k
         if constexpr (ox != 0) fx += (ox > 0 ? -g : g);
         if constexpr (oy != 0) fy += (oy > 0 ? -g : g);

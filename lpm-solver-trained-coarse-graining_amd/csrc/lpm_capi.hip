@@ -14,12 +14,28 @@
 #include "ku3d.hpp"
 #include "post.hpp"
 #include "serve.hpp"
+#include "capi_common.hpp"
 
 using namespace lpm;
+using namespace lpmc;
 
 namespace {
 
 thread_local std::string g_err;
+
+}  // namespace
+
+int lpmc::fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+namespace {
 
 // Optional per-thread timing of the K u launches inside the (P)CG loop (HIP events on the solver's stream).
 struct KuProfile {
@@ -43,34 +59,16 @@ int prof_flush() {
   return 0;
 }
 
-int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
-int fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
-
-#define HIPCHK(expr)                                                                         \
-  do {                                                                                       \
-    hipError_t e_ = (expr);                                                                  \
-    if (e_ != hipSuccess) return fail(LPM_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
-#define LAUNCHCHK()                                                                           \
-  do {                                                                                        \
-    hipError_t e_ = hipGetLastError();                                                        \
-    if (e_ != hipSuccess) return fail(LPM_E_HIP, "kernel launch: %s", hipGetErrorString(e_)); \
-  } while (0)
-
 inline Bcast bc_of(lpm_bcast b) { return Bcast{b.sb, b.sl}; }
 inline int64_t nsites(const lpm_grid* g) { return (int64_t)g->nx * g->ny * (g->dim == 3 ? g->nz : 1); }
 inline int64_t guard_of(const lpm_grid* g) { return packed_guard(g->dim, g->nx, g->ny); }
 inline int64_t pstride_of(const lpm_grid* g) { return nsites(g) + 2 * guard_of(g); }
-inline Box box_of(const lpm_grid* g) { return Box{g->nx, g->ny, g->dim == 3 ? g->nz : 1, pstride_of(g)}; }
+inline Box box_of(const lpm_grid* g) {
+  Box b{g->nx, g->ny, g->dim == 3 ? g->nz : 1, pstride_of(g)};
+  b.bonds = g->bonds;
+  b.bb = Bcast{g->bb.sb, g->bb.sl};
+  return b;
+}
 // kernels address packed planes from their first particle (past the leading guard)
 inline const void* cbase(const lpm_grid* g, const void* p, int layout, int dtype) {
   if (layout != LPM_COEF_PACKED || !p) return p;
@@ -98,24 +96,19 @@ hipStream_t capture_stream() {
 // blocks per system of the grid-stride vector kernels: at most 1024, so the per-system partial sums stay short
 inline int vec_blocks(int64_t N) { return (int)std::min<int64_t>(1024, std::max<int64_t>(1, (N + 255) / 256)); }
 
-// ------------------------------------------------------------------------------------------------ K u launch
-struct KuGeom {
-  dim3 grid, block;
-  int ntx, nty, nzc, zc, tx, ty;
-};
-
 // tiling overrides (0 = automatic): x tile width of the 3D kernels and planes per block
 int g_tile_wide = 0, g_zc = 0;
 
 // tile shapes: 3D float 64x4 or 32x8, double 32x4 or 16x8; 2D float 32x8, double 32x4
 template <int DIM, typename T>
-KuGeom ku_geom(const lpm_grid* g, int nb, int nl) {
+KuGeom ku_geom(const lpm_grid* g, int nb, int nl, bool bonds) {
   KuGeom k{};
   if constexpr (DIM == 3) {
     const bool wide = g_tile_wide ? g_tile_wide > 0 : g->nx >= 48;
     if (sizeof(T) == 4) {
       k.tx = wide ? 64 : 32;
-      k.ty = g_tile_wide == 2 ? 8 : (wide ? 4 : 8);
+      // 64 x 8 exists without bond words only
+      k.ty = (g_tile_wide == 2 && !bonds) ? 8 : (wide ? 4 : 8);
     } else {
       k.tx = wide ? 32 : 16;
       k.ty = wide ? 4 : 8;
@@ -141,49 +134,14 @@ KuGeom ku_geom(const lpm_grid* g, int nb, int nl) {
   k.block = dim3(k.tx * k.ty);
   return k;
 }
-
-template <typename T, int MODE, int EPI>
-int launch_ku3(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const Ku3Args<T, T>& a0, hipStream_t st) {
-  Ku3Args<T, T> a = a0;
-  a.nx = g->nx;
-  a.ny = g->ny;
-  a.nz = g->nz;
-  a.nl = nl;
-  a.zc = kg.zc;
-  a.ntx = kg.ntx;
-  a.nty = kg.nty;
-  a.nzc = kg.nzc;
-  if constexpr (sizeof(T) == 4) {
-    if (kg.tx == 64 && kg.ty == 8)
-      hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 64, 8, EPI>), kg.grid, kg.block, 0, st, a);
-    else if (kg.tx == 64)
-      hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 64, 4, EPI>), kg.grid, kg.block, 0, st, a);
-    else
-      hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 32, 8, EPI>), kg.grid, kg.block, 0, st, a);
-  } else {
-    if (kg.tx == 32)
-      hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 32, 4, EPI>), kg.grid, kg.block, 0, st, a);
-    else
-      hipLaunchKernelGGL((ku3d_kernel<T, T, MODE, 16, 8, EPI>), kg.grid, kg.block, 0, st, a);
-  }
-  LAUNCHCHK();
-  return LPM_OK;
+template <int DIM, typename T>
+KuGeom ku_geom(const lpm_grid* g, int nb, int nl) {
+  return ku_geom<DIM, T>(g, nb, nl, g->bonds != nullptr);
 }
-
-template <typename T, int MODE, int EPI>
-int launch_ku2(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const Ku2Args<T, T>& a0, hipStream_t st) {
-  Ku2Args<T, T> a = a0;
-  a.nx = g->nx;
-  a.ny = g->ny;
-  a.nl = nl;
-  a.ntx = kg.ntx;
-  a.nty = kg.nty;
-  if constexpr (sizeof(T) == 4)
-    hipLaunchKernelGGL((ku2d_kernel<T, T, MODE, 32, 8, EPI>), kg.grid, kg.block, 0, st, a);
-  else
-    hipLaunchKernelGGL((ku2d_kernel<T, T, MODE, 32, 4, EPI>), kg.grid, kg.block, 0, st, a);
-  LAUNCHCHK();
-  return LPM_OK;
+// blocks per system of the K u launch with or without bond words, whichever is larger (workspace sizing)
+template <int DIM, typename T>
+int ku_blocks_max(const lpm_grid* g, int nsys) {
+  return (int)std::max(ku_geom<DIM, T>(g, 1, nsys, false).grid.x, ku_geom<DIM, T>(g, 1, nsys, true).grid.x);
 }
 
 // one K u launch with epilogue EPI for either dimension / layout
@@ -228,8 +186,13 @@ int ku_call(const lpm_grid* g, int nb, int nl, int layout, const KuGeom& kg, con
     a.partial = (T*)c.partial;
     a.active = c.active;
     a.act_stride = NINT;
-    return layout == LPM_COEF_PACKED ? launch_ku3<T, COEF_PACKED, EPI>(g, nb, nl, kg, a, st)
-                                     : launch_ku3<T, COEF_DIRECTED, EPI>(g, nb, nl, kg, a, st);
+    a.bonds = g->bonds;
+    a.bb = Bcast{g->bb.sb, g->bb.sl};
+    if (g->bonds)
+      return layout == LPM_COEF_PACKED ? launch_ku3<T, COEF_PACKED, EPI, true>(g, nb, nl, kg, a, st)
+                                       : launch_ku3<T, COEF_DIRECTED, EPI, true>(g, nb, nl, kg, a, st);
+    return layout == LPM_COEF_PACKED ? launch_ku3<T, COEF_PACKED, EPI, false>(g, nb, nl, kg, a, st)
+                                     : launch_ku3<T, COEF_DIRECTED, EPI, false>(g, nb, nl, kg, a, st);
   } else {
     Ku2Args<T, T> a{};
     a.u = (const T*)c.u;
@@ -247,8 +210,13 @@ int ku_call(const lpm_grid* g, int nb, int nl, int layout, const KuGeom& kg, con
     a.partial = (T*)c.partial;
     a.active = c.active;
     a.act_stride = NINT;
-    return layout == LPM_COEF_PACKED ? launch_ku2<T, COEF_PACKED, EPI>(g, nb, nl, kg, a, st)
-                                     : launch_ku2<T, COEF_DIRECTED, EPI>(g, nb, nl, kg, a, st);
+    a.bonds = g->bonds;
+    a.bb = Bcast{g->bb.sb, g->bb.sl};
+    if (g->bonds)
+      return layout == LPM_COEF_PACKED ? launch_ku2<T, COEF_PACKED, EPI, true>(g, nb, nl, kg, a, st)
+                                       : launch_ku2<T, COEF_DIRECTED, EPI, true>(g, nb, nl, kg, a, st);
+    return layout == LPM_COEF_PACKED ? launch_ku2<T, COEF_PACKED, EPI, false>(g, nb, nl, kg, a, st)
+                                     : launch_ku2<T, COEF_DIRECTED, EPI, false>(g, nb, nl, kg, a, st);
   }
 }
 
@@ -279,10 +247,9 @@ template <int DIM, typename T>
 PcgLayout pcg_layout(const lpm_grid* g, int nsys) {
   constexpr int D = DIM;
   const int64_t N = nsites(g);
-  const KuGeom kg = ku_geom<DIM, T>(g, 1, nsys);
   const int vblk = vec_blocks(N);
   PcgLayout L{};
-  L.maxblk = std::max<int>((int)kg.grid.x, vblk);
+  L.maxblk = std::max<int>(ku_blocks_max<DIM, T>(g, nsys), vblk);
   size_t o = 0;
   const size_t vec = align256(sizeof(T) * (size_t)nsys * N * D);
   L.r = o;
@@ -481,6 +448,22 @@ int lpm_apply(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t 
   return fail(LPM_E_BADARG, "lpm_apply: bad dtype %d", dtype);
 }
 
+int lpm_bond_words(const lpm_grid* g, int32_t nbatch, const uint8_t* solid, uint32_t* words, void* stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!solid || !words || nbatch < 1 || nbatch > 65535)
+    return fail(LPM_E_BADARG, "lpm_bond_words: null pointer or bad batch");
+  hipStream_t st = (hipStream_t)stream;
+  Box box = box_of(g);
+  box.bonds = nullptr;
+  const dim3 grid((unsigned)((nsites(g) + 255) / 256), nbatch);
+  if (g->dim == 3)
+    hipLaunchKernelGGL((bond_words_kernel<3>), grid, dim3(256), 0, st, box, solid, words);
+  else
+    hipLaunchKernelGGL((bond_words_kernel<2>), grid, dim3(256), 0, st, box, solid, words);
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
 int lpm_pack_coef(const lpm_grid* g, int32_t nbatch, int32_t dtype_in, int32_t dtype_out, const void* kn,
                   const void* tv, void* kn_fwd, void* tv_fwd, double rtol, uint32_t* asym, void* stream) {
   if (int rc = check_grid(g)) return rc;
@@ -663,12 +646,12 @@ int lpm_post_vjp(const lpm_grid* g, int32_t ncb, int32_t spc, int32_t dtype, int
     T* Ww = cw + (size_t)nsys * N * NN;                                                                             \
     hipLaunchKernelGGL((shell_sums_kernel<DIM, T, T, MODE>), grid, dim3(256), 0, st, box, spc, (const T*)u, bu,    \
                        (const T*)tv, bc, ws);                                                                       \
-    hipLaunchKernelGGL((vjp_weights_kernel<DIM, T>), sgrid, dim3(256), 0, st, box, g->h, (const T*)gku,            \
+    hipLaunchKernelGGL((vjp_weights_kernel<DIM, T>), sgrid, dim3(256), 0, st, box, spc, g->h, (const T*)gku,       \
                        (const T*)gstress, cw);                                                                      \
     hipLaunchKernelGGL((vjp_params_kernel<DIM, T, MODE>), ggrid, dim3(256), 0, st, box, spc, (const T*)u,          \
                        (const T*)kn, (const T*)tv, cstride, (const T*)ws, (const T*)cw, (const T*)genergy, Ww,     \
                        (T*)gkn, (T*)gtv);                                                                           \
-    hipLaunchKernelGGL((vjp_u_kernel<DIM, T>), sgrid, dim3(256), 0, st, box, (const T*)Ww, (T*)gu);                \
+    hipLaunchKernelGGL((vjp_u_kernel<DIM, T>), sgrid, dim3(256), 0, st, box, spc, (const T*)Ww, (T*)gu);           \
   } while (0)
   const int key = (g->dim == 3 ? 100 : 0) + dtype * 10 + (layout == LPM_COEF_PACKED);
   switch (key) {
@@ -760,8 +743,7 @@ RefLayout ref_layout(const lpm_grid* g, int nsys) {
   o += align256(sizeof(float) * (size_t)nsys * N * DIM);
   L.d32 = o;
   o += align256(sizeof(float) * (size_t)nsys * N * DIM);
-  const KuGeom kg = ku_geom<DIM, double>(g, 1, nsys);
-  const int64_t nblk = std::max<int64_t>(kg.grid.x, vec_blocks(N));
+  const int64_t nblk = std::max<int64_t>(ku_blocks_max<DIM, double>(g, nsys), vec_blocks(N));
   L.part = o;
   o += align256(sizeof(double) * (size_t)nsys * nblk);
   L.bn2 = o;

@@ -28,6 +28,13 @@ struct Lat<2> {
 struct Box {
   int nx, ny, nz;   // nz = 1 in 2D
   int64_t ps;       // plane stride of packed coefficients (N + 2 guards); unused for directed ones
+  // optional bond words (one uint32 per particle and system, bit k = slot k is a bond; removed particles): null =
+  // every slot that stays in the box is a bond
+  const uint32_t* bonds = nullptr;
+  Bcast bb{0, 0};
+  __device__ __forceinline__ uint32_t word(int b, int l, int64_t p) const {
+    return bonds ? bonds[bb.at(b, l) + p] : 0xffffffffu;
+  }
   __device__ __forceinline__ void coords(int64_t p, int& x, int& y, int& z) const {
     x = (int)(p % nx);
     const int64_t r = p / nx;
@@ -40,6 +47,11 @@ struct Box {
     const int qx = x + Lat<DIM>::off(k, 0), qy = y + Lat<DIM>::off(k, 1), qz = z + Lat<DIM>::off(k, 2);
     if (qx < 0 || qx >= nx || qy < 0 || qy >= ny || qz < 0 || qz >= nz) return -1;
     return ((int64_t)qz * ny + qy) * nx + qx;
+  }
+  // bond partner of (x, y, z) in slot k given the particle's bond word, or -1
+  template <int DIM>
+  __device__ __forceinline__ int64_t nbrw(int x, int y, int z, int k, uint32_t w) const {
+    return ((w >> k) & 1u) ? nbr<DIM>(x, y, z, k) : -1;
   }
 };
 
@@ -57,10 +69,11 @@ __global__ __launch_bounds__(256) void shell_sums_kernel(Box box, int nl, const 
   const C* tt = tv + bc.at(b, l);
   int x, y, z;
   box.coords(p, x, y, z);
+  const uint32_t bw = box.word(b, l, p);
   T s[2] = {T(0), T(0)}, ts[2] = {T(0), T(0)};
 #pragma unroll
   for (int k = 0; k < L::NN; ++k) {
-    const int64_t q = box.nbr<DIM>(x, y, z, k);
+    const int64_t q = box.nbrw<DIM>(x, y, z, k, bw);
     if (q < 0) continue;
     T dl = T(0);
 #pragma unroll
@@ -107,9 +120,10 @@ __global__ __launch_bounds__(256) void site_post_kernel(Box box, int nl, double 
   const T* w = ws + sys * N * 4;
   int x, y, z;
   box.coords(p, x, y, z);
+  const uint32_t bw = box.word(b, l, p);
   int nvalid = 0;
 #pragma unroll
-  for (int k = 0; k < NN; ++k) nvalid += box.nbr<DIM>(x, y, z, k) >= 0;
+  for (int k = 0; k < NN; ++k) nvalid += box.nbrw<DIM>(x, y, z, k, bw) >= 0;
   const bool full = nvalid == NN;
   const T si[2] = {w[4 * p + 0], w[4 * p + 1]}, ti[2] = {w[4 * p + 2], w[4 * p + 3]};
   const T vol = (T)(DIM == 3 ? h * h * h : h * h);
@@ -117,7 +131,7 @@ __global__ __launch_bounds__(256) void site_post_kernel(Box box, int nl, double 
   T en = T(0);
 #pragma unroll
   for (int k = 0; k < NN; ++k) {
-    const int64_t q = box.nbr<DIM>(x, y, z, k);
+    const int64_t q = box.nbrw<DIM>(x, y, z, k, bw);
     T dl = T(0), f = T(0);
     if (q >= 0) {
 #pragma unroll
@@ -130,7 +144,7 @@ __global__ __launch_bounds__(256) void site_post_kernel(Box box, int nl, double 
       en += T(0.5) * kv * dl * dl;
 #pragma unroll
       for (int a = 0; a < D; ++a) ku[a] += f * L::template nd<T>(k, a);
-      const bool opp_ok = box.nbr<DIM>(x, y, z, k ^ 1) >= 0;
+      const bool opp_ok = box.nbrw<DIM>(x, y, z, k ^ 1, bw) >= 0;
       const T len = (T)(k < (DIM == 3 ? 6 : 4) ? h : h * 1.4142135623730950488);
       const T wgt = ((full || opp_ok) ? T(0.5) : T(1)) / vol * len * f;
       const T n0 = L::template nd<T>(k, 0), n1 = L::template nd<T>(k, 1), n2 = L::template nd<T>(k, 2);
@@ -187,6 +201,7 @@ __global__ __launch_bounds__(256) void bond_grad_kernel(Box box, int spc, const 
     const T* ll = lam + s * N * D;
     const T* wu = wsu + s * N * 4;
     const T* wl = wsl + s * N * 4;
+    const uint32_t bw = box.word(cb, j, p);
     T lv[3] = {T(0), T(0), T(0)}, uv[3] = {T(0), T(0), T(0)};
     for (int a = 0; a < D; ++a) {
       lv[a] = ll[D * p + a];
@@ -194,7 +209,7 @@ __global__ __launch_bounds__(256) void bond_grad_kernel(Box box, int spc, const 
     }
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
-      const int64_t q = box.nbr<DIM>(x, y, z, k);
+      const int64_t q = box.nbrw<DIM>(x, y, z, k, bw);
       if (q < 0) continue;
       T dl = T(0), mu = T(0);
 #pragma unroll
@@ -239,11 +254,12 @@ __global__ __launch_bounds__(256) void block_jacobi_kernel(Box box, int nl, cons
   const C* tt = tv + bc.at(b, l);
   int x, y, z;
   box.coords(p, x, y, z);
+  const uint32_t bw = box.word(b, l, p);
   double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   double Tc[2][3] = {{0, 0, 0}, {0, 0, 0}}, sc[2][3] = {{0, 0, 0}, {0, 0, 0}};
 #pragma unroll
   for (int k = 0; k < NN; ++k) {
-    const int64_t q = box.nbr<DIM>(x, y, z, k);
+    const int64_t q = box.nbrw<DIM>(x, y, z, k, bw);
     if (q < 0) continue;
     const int c = L::shell(k);
     // coefficient of the reverse slot at the neighbour
@@ -379,22 +395,23 @@ __global__ __launch_bounds__(256) void pack_coef_kernel(Box box, const CI* __res
 // For gS = gU = 0 this is the adjoint of bond_grad_kernel with c = lam.n.  Three kernels: slot weights c, then
 // per coefficient batch the parameter gradients and W (summing the systems of the batch in registers), then du.
 
-// computeStress weight of slot k at site (x, y, z): of / V * l0
+// computeStress weight of slot k at site (x, y, z) with bond word bw: of / V * l0
 template <int DIM, typename T>
-__device__ __forceinline__ T stress_weight(const Box& box, int x, int y, int z, int k, double h) {
+__device__ __forceinline__ T stress_weight(const Box& box, int x, int y, int z, int k, double h, uint32_t bw) {
   using L = Lat<DIM>;
   int nvalid = 0;
 #pragma unroll
-  for (int j = 0; j < L::NN; ++j) nvalid += box.nbr<DIM>(x, y, z, j) >= 0;
-  const bool opp_ok = box.nbr<DIM>(x, y, z, k ^ 1) >= 0;
+  for (int j = 0; j < L::NN; ++j) nvalid += box.nbrw<DIM>(x, y, z, j, bw) >= 0;
+  const bool opp_ok = box.nbrw<DIM>(x, y, z, k ^ 1, bw) >= 0;
   const double vol = DIM == 3 ? h * h * h : h * h;
   const double len = k < (DIM == 3 ? 6 : 4) ? h : h * 1.4142135623730950488;
   return (T)(((nvalid == L::NN || opp_ok) ? 0.5 : 1.0) / vol * len);
 }
 
-// c (systems, N, NN) from gP (systems, N, D) and gS (systems, N, 3|6); either may be null
+// c (systems, N, NN) from gP (systems, N, D) and gS (systems, N, 3|6); either may be null.  System s is load
+// s % spc of coefficient batch s / spc.
 template <int DIM, typename T>
-__global__ __launch_bounds__(256) void vjp_weights_kernel(Box box, double h, const T* __restrict__ gP,
+__global__ __launch_bounds__(256) void vjp_weights_kernel(Box box, int spc, double h, const T* __restrict__ gP,
                                                           const T* __restrict__ gS, T* __restrict__ c) {
   using L = Lat<DIM>;
   constexpr int NN = L::NN, D = L::D, NS = DIM == 3 ? 6 : 3;
@@ -404,6 +421,7 @@ __global__ __launch_bounds__(256) void vjp_weights_kernel(Box box, double h, con
   const int64_t s = blockIdx.y;
   int x, y, z;
   box.coords(p, x, y, z);
+  const uint32_t bw = box.word((int)(s / spc), (int)(s % spc), p);
   T gp[3] = {T(0), T(0), T(0)}, gs[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
   if (gP)
     for (int a = 0; a < D; ++a) gp[a] = gP[(s * N + p) * D + a];
@@ -412,7 +430,7 @@ __global__ __launch_bounds__(256) void vjp_weights_kernel(Box box, double h, con
 #pragma unroll
   for (int k = 0; k < NN; ++k) {
     T v = T(0);
-    if (box.nbr<DIM>(x, y, z, k) >= 0) {
+    if (box.nbrw<DIM>(x, y, z, k, bw) >= 0) {
       const T n0 = L::template nd<T>(k, 0), n1 = L::template nd<T>(k, 1), n2 = L::template nd<T>(k, 2);
       v = gp[0] * n0 + gp[1] * n1 + gp[2] * n2;
       T nn;
@@ -420,7 +438,7 @@ __global__ __launch_bounds__(256) void vjp_weights_kernel(Box box, double h, con
         nn = gs[0] * n0 * n0 + gs[1] * n1 * n1 + gs[2] * n2 * n2 + gs[3] * n1 * n2 + gs[4] * n0 * n2 + gs[5] * n0 * n1;
       else
         nn = gs[0] * n0 * n0 + gs[1] * n1 * n1 + gs[2] * n0 * n1;
-      v += stress_weight<DIM, T>(box, x, y, z, k, h) * nn;
+      v += stress_weight<DIM, T>(box, x, y, z, k, h, bw) * nn;
     }
     c[(s * N + p) * NN + k] = v;
   }
@@ -467,12 +485,13 @@ __global__ __launch_bounds__(256) void vjp_params_kernel(Box box, int spc, const
     const T* w = ws + s * N * 4;
     const T* cs = c + s * N * NN;
     const T gu = gU ? gU[s * N + p] : T(0);
+    const uint32_t bw = box.word(cb, j, p);
     T dl[NN], cc[NN], Cc[2] = {T(0), T(0)}, Dc[2] = {T(0), T(0)};
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
       dl[k] = T(0);
       cc[k] = cs[p * NN + k];
-      if (q[k] < 0) continue;
+      if (q[k] < 0 || !((bw >> k) & 1u)) continue;
       for (int a = 0; a < D; ++a)
         if (L::off(k, a)) dl[k] += L::template nd<T>(k, a) * (uu[D * p + a] - uu[D * q[k] + a]);
       const T cq = cs[q[k] * NN + (k ^ 1)];
@@ -484,7 +503,7 @@ __global__ __launch_bounds__(256) void vjp_params_kernel(Box box, int spc, const
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
       T Wk = T(0);
-      if (q[k] >= 0) {
+      if (q[k] >= 0 && ((bw >> k) & 1u)) {
         const int sh = L::shell(k);
         const T sq = w[4 * q[k] + sh];
         gk[k] += T(2) * cc[k] * dl[k] + T(0.5) * gu * dl[k] * dl[k];
@@ -504,9 +523,9 @@ __global__ __launch_bounds__(256) void vjp_params_kernel(Box box, int spc, const
   }
 }
 
-// du_i = sum_k (W_ik + W_{q,k^1}) n_k; grid (ceil(N/256), systems)
+// du_i = sum_k (W_ik + W_{q,k^1}) n_k; grid (ceil(N/256), systems), system s = load s % spc of batch s / spc
 template <int DIM, typename T>
-__global__ __launch_bounds__(256) void vjp_u_kernel(Box box, const T* __restrict__ W, T* __restrict__ gu) {
+__global__ __launch_bounds__(256) void vjp_u_kernel(Box box, int spc, const T* __restrict__ W, T* __restrict__ gu) {
   using L = Lat<DIM>;
   constexpr int NN = L::NN, D = L::D;
   const int64_t N = (int64_t)box.nx * box.ny * box.nz;
@@ -516,16 +535,41 @@ __global__ __launch_bounds__(256) void vjp_u_kernel(Box box, const T* __restrict
   int x, y, z;
   box.coords(p, x, y, z);
   const T* ws = W + s * N * NN;
+  const uint32_t bw = box.word((int)(s / spc), (int)(s % spc), p);
   T g[3] = {T(0), T(0), T(0)};
 #pragma unroll
   for (int k = 0; k < NN; ++k) {
-    const int64_t q = box.nbr<DIM>(x, y, z, k);
+    const int64_t q = box.nbrw<DIM>(x, y, z, k, bw);
     if (q < 0) continue;
     const T t = ws[p * NN + k] + ws[q * NN + (k ^ 1)];
 #pragma unroll
     for (int a = 0; a < D; ++a) g[a] += t * L::template nd<T>(k, a);
   }
   for (int a = 0; a < D; ++a) gu[(s * N + p) * D + a] = g[a];
+}
+
+// ----------------------------------------------------------------------------------------- bond words
+// Bond words of removed particles (cement3d/torch_v1.py:36-53): bit k of particle p is set iff slot k stays in the
+// box and both p and its slot-k neighbour are solid.  solid (nbatch, N) bytes, nonzero = solid; words (nbatch, N).
+template <int DIM>
+__global__ __launch_bounds__(256) void bond_words_kernel(Box box, const uint8_t* __restrict__ solid,
+                                                         uint32_t* __restrict__ words) {
+  using L = Lat<DIM>;
+  const int64_t N = (int64_t)box.nx * box.ny * box.nz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const uint8_t* sd = solid + (int64_t)blockIdx.y * N;
+  uint32_t w = 0;
+  if (sd[p]) {
+    int x, y, z;
+    box.coords(p, x, y, z);
+#pragma unroll
+    for (int k = 0; k < L::NN; ++k) {
+      const int64_t q = box.nbr<DIM>(x, y, z, k);
+      if (q >= 0 && sd[q]) w |= 1u << k;
+    }
+  }
+  words[(int64_t)blockIdx.y * N + p] = w;
 }
 
 }  // namespace lpm

@@ -2,7 +2,9 @@
 
 Inputs and outputs are host NumPy arrays as in the reference (u (N, 3), Kn / Tv (N, 18) FP64); every call copies its
 inputs to the GPU, runs the kernels and copies the result back, so the PCIe transfer is part of each call.  Batched
-or resident work goes through lpm_ops / torch_lpm3d instead.  Lattices are full boxes.
+or resident work goes through lpm_ops / torch_lpm3d instead.  A lattice with an `active` mask (only active sites
+numbered, coarse3d/geom3d.py:38-65) runs on its full box with the inactive sites removed by bond words: compact
+arrays are scattered into the box on the GPU and the results gathered back.
 
 The assembled K (stiffness) is probed from the batched matrix-free operator into the reference's block pattern
 (coarse3d/lin3d.py:306-447); solve_direct keeps the reference's SciPy SuperLU solve (coarse3d/lin3d.py:535-546),
@@ -29,37 +31,65 @@ class Lattice3D(LatticeGeom):
 
     def __init__(self, nx, ny, nz, h=1.0, origin=(0.0, 0.0, 0.0), active=None, device=None):
         super().__init__(nx, ny, nz, h, origin, active)
-        if not self.full:
-            raise NotImplementedError("the MI355X kernels work on full-box lattices (no inactive sites)")
         self.device = _dev(device)
+        self.Nbox = self.nx * self.ny * self.nz
+        if self.full:
+            self.words, self._sites = None, None
+        else:
+            self._sites = torch.as_tensor(self.sites, device=self.device)
+            solid = torch.zeros(self.Nbox, dtype=torch.bool, device=self.device)
+            solid[self._sites] = True
+            self.words = ops.bond_words(self.dims, solid)
 
     def _t(self, a):
         return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=self.device)
 
+    def _box(self, a, width):
+        """Compact (N, width) host array -> (Nbox, width) device tensor, zero on inactive sites."""
+        t = self._t(np.reshape(a, (self.N, width)))
+        if self.full:
+            return t
+        out = torch.zeros((self.Nbox, width), device=self.device, dtype=t.dtype)
+        out[self._sites] = t
+        return out
+
+    def _host(self, t):
+        """(Nbox, ...) device tensor -> compact host array."""
+        return (t if self.full else t[self._sites]).cpu().numpy()
+
+    def _post(self, u, Kn, Tv, **what):
+        return ops.post(self.dims, self.h, self._box(u, 3), self._box(Kn, 18), self._box(Tv, 18), bonds=self.words,
+                        **what)
+
     def apply(self, u, Kn, Tv):
-        return ops.apply(self.dims, self.h, self._t(np.reshape(u, (self.N, 3))), self._t(Kn), self._t(Tv)).cpu().numpy()
+        return self._host(ops.apply(self.dims, self.h, self._box(u, 3), self._box(Kn, 18), self._box(Tv, 18),
+                                    bonds=self.words))
 
     def bond_forces(self, u, Kn, Tv):
-        o = ops.post(self.dims, self.h, self._t(np.reshape(u, (self.N, 3))), self._t(Kn), self._t(Tv), dl=True,
-                     force=True)
-        return o["dl"].cpu().numpy(), o["force"].cpu().numpy()
+        o = self._post(u, Kn, Tv, dl=True, force=True)
+        return self._host(o["dl"]), self._host(o["force"])
 
     def stress(self, u, Kn, Tv):
         """computeStress(): (N, 6) in the order xx, yy, zz, yz, xz, xy."""
-        o = ops.post(self.dims, self.h, self._t(np.reshape(u, (self.N, 3))), self._t(Kn), self._t(Tv), stress=True)
-        return o["stress"].cpu().numpy()
+        return self._host(self._post(u, Kn, Tv, stress=True)["stress"])
 
     def energy_per_particle(self, u, Kn, Tv):
-        o = ops.post(self.dims, self.h, self._t(np.reshape(u, (self.N, 3))), self._t(Kn), self._t(Tv), energy=True)
-        return o["energy"].cpu().numpy()
+        return self._host(self._post(u, Kn, Tv, energy=True)["energy"])
 
     def block_diag(self, Kn, Tv):
-        return ops.block_jacobi(self.dims, self.h, self._t(Kn), self._t(Tv), raw=True).cpu().numpy()
+        return self._host(ops.block_jacobi(self.dims, self.h, self._box(Kn, 18), self._box(Tv, 18), raw=True,
+                                           bonds=self.words))
 
     def stiffness(self, Kn, Tv):
-        """Assembled K (3N x 3N CSR)."""
-        indptr, indices, data = ops.stiffness_csr(self.dims, self.h, self._t(Kn), self._t(Tv))
-        return sp.csr_matrix((data, indices, indptr), shape=(3 * self.N, 3 * self.N))
+        """Assembled K (3N x 3N CSR).  With inactive sites: the rows and columns of the active sites of the box's
+        K (entries between two active sites that only a removed site links are stored as zeros)."""
+        indptr, indices, data = ops.stiffness_csr(self.dims, self.h, self._box(Kn, 18), self._box(Tv, 18),
+                                                  bonds=self.words)
+        K = sp.csr_matrix((data, indices, indptr), shape=(3 * self.Nbox, 3 * self.Nbox))
+        if self.full:
+            return K
+        dof = (3 * self.sites[:, None] + np.arange(3)[None, :]).ravel()
+        return K[dof][:, dof].tocsr()
 
 
 def k_block_jacobi(D, free, device=None):
@@ -75,16 +105,17 @@ def solve_pcg(lat, Kn, Tv, fixed, u_presc, tol=1e-10, maxit=20000, M=None, x0=No
     free = ~np.asarray(fixed, bool)
     up = np.zeros((lat.N, 3))
     up[~free] = np.asarray(u_presc).reshape(lat.N, 3)[~free]
-    t = lat._t
-    kn, tv, upt, fr = t(Kn), t(Tv), t(up), t(free.astype(float))
-    b = -ops.apply(lat.dims, lat.h, upt, kn, tv) * fr[:, None]
+    kn, tv, upt = lat._box(Kn, 18), lat._box(Tv, 18), lat._box(up, 3)
+    fr = lat._box(free.astype(float), 1)[:, 0]                              # inactive sites: fixed
+    w = lat.words
+    b = -ops.apply(lat.dims, lat.h, upt, kn, tv, bonds=w) * fr[:, None]
     if M is None:
-        minv = ops.block_jacobi(lat.dims, lat.h, kn, tv, free=fr)
+        minv = ops.block_jacobi(lat.dims, lat.h, kn, tv, free=fr, bonds=w)
     else:
-        minv = ops.sym_from_full(t(M))
-    xi = None if x0 is None else t(np.where(free[:, None], np.asarray(x0).reshape(lat.N, 3), 0.0))
-    x, info = ops.pcg(lat.dims, lat.h, b, kn, tv, minv=minv, free=fr, tol=tol, maxit=maxit, x0=xi, check=1)
-    return up + x.cpu().numpy(), dict(iters=int(info.iters.max()), relres=float(info.relres.max()))
+        minv = ops.sym_from_full(lat._box(np.reshape(M, (lat.N, 9)), 9).reshape(-1, 3, 3))
+    xi = None if x0 is None else lat._box(np.where(free[:, None], np.asarray(x0).reshape(lat.N, 3), 0.0), 3)
+    x, info = ops.pcg(lat.dims, lat.h, b, kn, tv, minv=minv, free=fr, tol=tol, maxit=maxit, x0=xi, check=1, bonds=w)
+    return up + lat._host(x), dict(iters=int(info.iters.max()), relres=float(info.relres.max()))
 
 
 def solve_direct(K, fixed, u_presc, lu=None):

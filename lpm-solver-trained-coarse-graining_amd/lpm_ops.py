@@ -31,6 +31,46 @@ def packed_len(dims):
     return int(np.prod([int(d) for d in dims])) + 2 * packed_guard(dims)
 
 
+def _bshape(bonds):
+    return [] if bonds is None else [bonds.shape[:-1]]
+
+
+def _bonds_view(bonds, batch, N):
+    """Bond words (..., N) int32 -> (tensor, Bcast) for the (b, l) systems of `batch`; (None, zero) if absent."""
+    if bonds is None:
+        return None, L.Bcast(0, 0)
+    L.require_gpu(bonds)
+    if bonds.dtype != torch.int32:
+        raise TypeError("bond words are int32 (uint32 bit patterns), got %s" % bonds.dtype)
+    return L.bview(bonds, batch, (N,))
+
+
+def _bonds_sys(bonds, batch, N, spc):
+    """Bond words for the gradient kernels, whose system s = cb * spc + j of the full batch reads word row s."""
+    if bonds is None:
+        return None, L.Bcast(0, 0)
+    L.require_gpu(bonds)
+    if bonds.dtype != torch.int32:
+        raise TypeError("bond words are int32 (uint32 bit patterns), got %s" % bonds.dtype)
+    nb, nl = L.batch2(batch)
+    w = bonds.expand(tuple(batch) + (N,)).reshape(nb * nl, N).contiguous()
+    return w, L.Bcast(spc * N, N)
+
+
+def bond_words(dims, solid):
+    """Bond words of removed particles: solid (..., N) bool/uint8 -> int32 (..., N), bit k of particle p set iff
+    slot k stays in the box and p and its slot-k neighbour are both solid (cement3d/torch_v1.py:36-53)."""
+    dims, N, d, NN = _dims(dims)
+    L.require_gpu(solid)
+    lead = solid.shape[:-1]
+    nbat = int(np.prod(lead)) if len(lead) else 1
+    s = (solid.reshape(nbat, N) != 0).to(torch.uint8).contiguous()
+    w = torch.empty((nbat, N), device=solid.device, dtype=torch.int32)
+    g = L.grid_of(dims, 1.0)
+    L.check(L.lib().lpm_bond_words(ctypes.byref(g), nbat, _ptr(s), _ptr(w), L.stream_of(solid)))
+    return w.reshape(tuple(lead) + (N,))
+
+
 def _coef_inner(dims, layout):
     """Per-batch shape of a coefficient array: directed (N, NN) or packed (NN/2, P)."""
     NN = L.nn_of(len(dims))
@@ -39,15 +79,16 @@ def _coef_inner(dims, layout):
     return (NN // 2, packed_len(dims))
 
 
-def apply(dims, h, u, Kn, Tv, layout=L.DIRECTED):
-    """K u for broadcast batches: u (..., N, d), Kn/Tv (..., N, NN) directed or (..., NN/2, N) packed."""
+def apply(dims, h, u, Kn, Tv, layout=L.DIRECTED, bonds=None):
+    """K u for broadcast batches: u (..., N, d), Kn/Tv (..., N, NN) directed or (..., NN/2, N) packed; optional
+    bond words (..., N) of removed particles (bond_words)."""
     dims, N, d, NN = _dims(dims)
     L.require_gpu(u, Kn, Tv)
     dt = u.dtype
     if Kn.dtype != dt or Tv.dtype != dt:
         raise TypeError("u, Kn and Tv must share one dtype")
     ci = _coef_inner(dims, layout)
-    batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2])
+    batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2], *_bshape(bonds))
     nb, nl = L.batch2(batch)
     ue, bu = L.bview(u, batch, (N, d))
     ke, bc = L.bview(Kn, batch, ci)
@@ -55,19 +96,21 @@ def apply(dims, h, u, Kn, Tv, layout=L.DIRECTED):
     if (bc.sb, bc.sl) != (bt.sb, bt.sl):
         raise ValueError("Kn and Tv must broadcast the same way")
     out = torch.empty(tuple(batch) + (N, d), device=u.device, dtype=dt)
-    g = L.grid_of(dims, h)
+    we, bb = _bonds_view(bonds, batch, N)
+    g = L.grid_of(dims, h, we, bb)
     L.check(L.lib().lpm_apply(ctypes.byref(g), nb, nl, L.dtype_code(dt), layout, _ptr(ue), bu, _ptr(ke), _ptr(te),
                               bc, _ptr(out), L.stream_of(u)))
     return out
 
 
-def post(dims, h, u, Kn, Tv, ku=False, stress=False, energy=False, dl=False, force=False, layout=L.DIRECTED):
+def post(dims, h, u, Kn, Tv, ku=False, stress=False, energy=False, dl=False, force=False, layout=L.DIRECTED,
+         bonds=None):
     """Per-particle outputs; returns a dict with the requested entries (batch = broadcast of u and Kn)."""
     dims, N, d, NN = _dims(dims)
     L.require_gpu(u, Kn, Tv)
     dt = u.dtype
     ci = _coef_inner(dims, layout)
-    batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2])
+    batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2], *_bshape(bonds))
     nb, nl = L.batch2(batch)
     ue, bu = L.bview(u, batch, (N, d))
     ke, bc = L.bview(Kn, batch, ci)
@@ -81,7 +124,8 @@ def post(dims, h, u, Kn, Tv, ku=False, stress=False, energy=False, dl=False, for
                energy=torch.empty(B + (N,), device=dev, dtype=dt) if energy else None,
                dl=torch.empty(B + (N, NN), device=dev, dtype=dt) if dl else None,
                force=torch.empty(B + (N, NN), device=dev, dtype=dt) if force else None)
-    g = L.grid_of(dims, h)
+    we, bb = _bonds_view(bonds, batch, N)
+    g = L.grid_of(dims, h, we, bb)
     L.check(L.lib().lpm_post(ctypes.byref(g), nb, nl, L.dtype_code(dt), layout, _ptr(ue), bu, _ptr(ke), _ptr(te), bc,
                              _ptr(work), _ptr(out["ku"]), _ptr(out["stress"]), _ptr(out["energy"]), _ptr(out["dl"]),
                              _ptr(out["force"]), L.stream_of(u)))
@@ -89,7 +133,7 @@ def post(dims, h, u, Kn, Tv, ku=False, stress=False, energy=False, dl=False, for
     return out
 
 
-def block_jacobi(dims, h, Kn, Tv, free=None, out_dtype=None, raw=False, batch=None, layout=L.DIRECTED):
+def block_jacobi(dims, h, Kn, Tv, free=None, out_dtype=None, raw=False, batch=None, layout=L.DIRECTED, bonds=None):
     """Block-Jacobi inverse blocks (batch, N, 6|3) symmetric, or the diagonal blocks (batch, N, d, d) if raw."""
     dims, N, d, NN = _dims(dims)
     L.require_gpu(Kn, Tv)
@@ -100,6 +144,7 @@ def block_jacobi(dims, h, Kn, Tv, free=None, out_dtype=None, raw=False, batch=No
         shapes.append(f.shape[:-2] if (f.dim() >= 2 and tuple(f.shape[-2:]) == (N, d)) else f.shape[:-1])
     if batch is not None:
         shapes.append(tuple(batch))
+    shapes += _bshape(bonds)
     bshape = torch.broadcast_shapes(*shapes)
     nb, nl = L.batch2(bshape)
     ke, bc = L.bview(Kn, bshape, ci)
@@ -108,7 +153,8 @@ def block_jacobi(dims, h, Kn, Tv, free=None, out_dtype=None, raw=False, batch=No
     odt = out_dtype or Kn.dtype
     per = (d, d) if raw else ((6,) if d == 3 else (3,))
     out = torch.empty(tuple(bshape) + (N,) + per, device=Kn.device, dtype=odt)
-    g = L.grid_of(dims, h)
+    we, bb = _bonds_view(bonds, bshape, N)
+    g = L.grid_of(dims, h, we, bb)
     L.check(L.lib().lpm_block_jacobi(ctypes.byref(g), nb, nl, L.dtype_code(Kn.dtype), L.dtype_code(odt), layout,
                                      _ptr(ke), _ptr(te), bc, _ptr(me), bm, _ptr(out), 1 if raw else 0,
                                      L.stream_of(Kn)))
@@ -177,13 +223,13 @@ def _sum_coef_grad(g, batch, Kn, N, NN, general):
     return g.reshape(kb + (N, NN))
 
 
-def bond_grad(dims, h, Kn, Tv, u, lam, scale=1.0, layout=L.DIRECTED):
+def bond_grad(dims, h, Kn, Tv, u, lam, scale=1.0, layout=L.DIRECTED, bonds=None):
     """scale * d/d(Kn, Tv) sum_s lam_s . K u_s per directed slot, reduced to the broadcast shape of Kn.
     u, lam (..., N, d) with the full batch; Kn (..., N, NN)."""
     dims, N, d, NN = _dims(dims)
     L.require_gpu(Kn, Tv, u, lam)
     dt = u.dtype
-    batch = torch.broadcast_shapes(u.shape[:-2], lam.shape[:-2], Kn.shape[:-2])
+    batch = torch.broadcast_shapes(u.shape[:-2], lam.shape[:-2], Kn.shape[:-2], *_bshape(bonds))
     ci = _coef_inner(dims, layout)
     nb, nl = L.batch2(batch)
     ue = u.expand(tuple(batch) + (N, d)).contiguous()
@@ -192,19 +238,20 @@ def bond_grad(dims, h, Kn, Tv, u, lam, scale=1.0, layout=L.DIRECTED):
     work = torch.empty((2, nb * nl, N, 4), device=u.device, dtype=dt)
     gk = torch.empty((ncb, N, NN), device=u.device, dtype=dt)
     gt = torch.empty_like(gk)
-    g = L.grid_of(dims, h)
+    we, bb = _bonds_sys(bonds, batch, N, spc)
+    g = L.grid_of(dims, h, we, bb)
     L.check(L.lib().lpm_bond_grad(ctypes.byref(g), ncb, spc, L.dtype_code(dt), layout, _ptr(kk), _ptr(tt), _ptr(ue),
                                   _ptr(le), float(scale), _ptr(work), _ptr(gk), _ptr(gt), L.stream_of(u)))
     return _sum_coef_grad(gk, batch, Kn, N, NN, general), _sum_coef_grad(gt, batch, Kn, N, NN, general)
 
 
-def post_vjp(dims, h, u, Kn, Tv, g_ku=None, g_stress=None, g_energy=None, layout=L.DIRECTED):
+def post_vjp(dims, h, u, Kn, Tv, g_ku=None, g_stress=None, g_energy=None, layout=L.DIRECTED, bonds=None):
     """Vector-Jacobian product of post()'s (ku, stress, energy) for the given cotangents (None = zero).
     Returns (gu with u's batch broadcast to the full batch, gKn, gTv directed and reduced to Kn's shape)."""
     dims, N, d, NN = _dims(dims)
     L.require_gpu(u, Kn, Tv)
     dt = u.dtype
-    batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2])
+    batch = torch.broadcast_shapes(u.shape[:-2], Kn.shape[:-2], Tv.shape[:-2], *_bshape(bonds))
     ci = _coef_inner(dims, layout)
     nb, nl = L.batch2(batch)
     B = tuple(batch)
@@ -218,7 +265,8 @@ def post_vjp(dims, h, u, Kn, Tv, g_ku=None, g_stress=None, g_energy=None, layout
     gu = torch.empty(B + (N, d), device=u.device, dtype=dt)
     gk = torch.empty((ncb, N, NN), device=u.device, dtype=dt)
     gt = torch.empty_like(gk)
-    g = L.grid_of(dims, h)
+    we, bb = _bonds_sys(bonds, batch, N, spc)
+    g = L.grid_of(dims, h, we, bb)
     L.check(L.lib().lpm_post_vjp(ctypes.byref(g), ncb, spc, L.dtype_code(dt), layout, _ptr(kk), _ptr(tt), _ptr(ue),
                                  _ptr(cot[0]), _ptr(cot[1]), _ptr(cot[2]), _ptr(work), _ptr(gu), _ptr(gk), _ptr(gt),
                                  L.stream_of(u)))
@@ -251,7 +299,7 @@ def _opts(precond, tol, maxit, check, graph, inner_tol=1e-5, max_outer=20):
 
 
 def pcg(dims, h, b, Kn, Tv, minv=None, free=None, tol=1e-10, maxit=20000, x0=None, check=10, graph=False,
-        layout=L.DIRECTED):
+        layout=L.DIRECTED, bonds=None):
     """Batched (P)CG on K_ff x = b_f.  b (B, L, N, d) or broadcastable; minv symmetric blocks (…, N, 6|3) or None
     (plain CG).  Returns x (batch, N, d) and SolveInfo (per-system iterations, relative residual, convergence)."""
     dims, N, d, NN = _dims(dims)
@@ -264,6 +312,7 @@ def pcg(dims, h, b, Kn, Tv, minv=None, free=None, tol=1e-10, maxit=20000, x0=Non
     if free is not None:
         f = torch.as_tensor(free)
         shapes.append(f.shape[:-2] if (f.dim() >= 2 and tuple(f.shape[-2:]) == (N, d)) else f.shape[:-1])
+    shapes += _bshape(bonds)
     batch = torch.broadcast_shapes(*shapes)
     nb, nl = L.batch2(batch)
     be = b.expand(tuple(batch) + (N, d)).contiguous()
@@ -278,7 +327,8 @@ def pcg(dims, h, b, Kn, Tv, minv=None, free=None, tol=1e-10, maxit=20000, x0=Non
         x = x0.to(dt).expand(tuple(batch) + (N, d)).contiguous().clone()
     else:
         x = torch.empty(tuple(batch) + (N, d), device=b.device, dtype=dt)
-    g = L.grid_of(dims, h)
+    we, bb = _bonds_view(bonds, batch, N)
+    g = L.grid_of(dims, h, we, bb)
     nsys = nb * nl
     wbytes = L.lib().lpm_pcg_workspace(ctypes.byref(g), nsys, L.dtype_code(dt))
     work = torch.empty(int(wbytes), device=b.device, dtype=torch.uint8)
@@ -295,7 +345,7 @@ def pcg(dims, h, b, Kn, Tv, minv=None, free=None, tol=1e-10, maxit=20000, x0=Non
 
 
 def pcg_refined(dims, h, b, Kn64, Tv64, Kn32, Tv32, minv32=None, free=None, tol=1e-8, maxit=20000, inner_tol=1e-5,
-                max_outer=30, x0=None, check=10, graph=False, layout64=L.DIRECTED, layout32=L.DIRECTED):
+                max_outer=30, x0=None, check=10, graph=False, layout64=L.DIRECTED, layout32=L.DIRECTED, bonds=None):
     """FP32 (P)CG inner solves inside an FP64 iterative refinement; b FP64 (…, N, d).  Returns x FP64 and SolveInfo
     (relres = true FP64 relative residual, iters = total inner iterations, outer = refinement steps)."""
     dims, N, d, NN = _dims(dims)
@@ -310,6 +360,7 @@ def pcg_refined(dims, h, b, Kn64, Tv64, Kn32, Tv32, minv32=None, free=None, tol=
     if free is not None:
         f = torch.as_tensor(free)
         shapes.append(f.shape[:-2] if (f.dim() >= 2 and tuple(f.shape[-2:]) == (N, d)) else f.shape[:-1])
+    shapes += _bshape(bonds)
     batch = torch.broadcast_shapes(*shapes)
     nb, nl = L.batch2(batch)
     be = b.expand(tuple(batch) + (N, d)).contiguous()
@@ -326,7 +377,8 @@ def pcg_refined(dims, h, b, Kn64, Tv64, Kn32, Tv32, minv32=None, free=None, tol=
         x = x0.to(torch.float64).expand(tuple(batch) + (N, d)).contiguous().clone()
     else:
         x = torch.empty(tuple(batch) + (N, d), device=b.device, dtype=torch.float64)
-    g = L.grid_of(dims, h)
+    we, bb = _bonds_view(bonds, batch, N)
+    g = L.grid_of(dims, h, we, bb)
     nsys = nb * nl
     work = torch.empty(int(L.lib().lpm_refined_workspace(ctypes.byref(g), nsys)), device=b.device, dtype=torch.uint8)
     iters = np.zeros(nsys, np.int32)
@@ -388,7 +440,7 @@ def csr_pattern(dims):
     return _PATTERNS[dims]
 
 
-def stiffness_csr(dims, h, Kn, Tv, colours_per_pass=None):
+def stiffness_csr(dims, h, Kn, Tv, colours_per_pass=None, bonds=None):
     """Assembled K (d N x d N) in CSR form for one symmetric coefficient set Kn, Tv (N, NN) on the GPU.
 
     K is probed with the batched matrix-free operator: two particles closer than 5 sites along every axis never
@@ -396,7 +448,8 @@ def stiffness_csr(dims, h, Kn, Tv, colours_per_pass=None):
     indicator of (colour c, component b) holds in row (p, a) exactly the entry K[(p, a), (q, b)] of the single
     column particle q of colour c in p's pattern.  5^d x d probe vectors are applied in batches; entries are
     gathered into the pattern of csr_pattern (columns ascending).  Returns (indptr int64, indices int32, data)
-    as host NumPy arrays; data in Kn's dtype."""
+    as host NumPy arrays; data in Kn's dtype.  With bond words (N,) of removed particles the values are those of
+    the reduced lattice (entries through removed bonds are zero in the full-box pattern)."""
     dims, N, d, NN = _dims(dims)
     L.require_gpu(Kn, Tv)
     if Kn.shape != (N, NN) or Tv.shape != (N, NN):
@@ -423,7 +476,7 @@ def stiffness_csr(dims, h, Kn, Tv, colours_per_pass=None):
         hit = (colour_t[None, :] == cs[:, None]).to(dt)                     # (nc, N)
         for b in range(d):
             probe[:, b, :, b] = hit
-        Y = apply(dims, h, probe, Kn, Tv)                                   # [colour, b, p, a]
+        Y = apply(dims, h, probe, Kn, Tv, bonds=bonds)                     # [colour, b, p, a]
         sel = np.nonzero((c_idx >= c0) & (c_idx < c0 + len(cs)))[0]
         if len(sel) == 0:
             continue

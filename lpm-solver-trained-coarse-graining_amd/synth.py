@@ -107,6 +107,57 @@ def cement_phases(n, seed, fractions=(0.15, 0.55, 0.15, 0.15), rmean=6.0):
     return lab.ravel()
 
 
+# ------------------------------------------------------------------------------------------------ removed particles
+_SC18 = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, -1, 0), (1, -1, 0),
+         (-1, 1, 0), (1, 0, 1), (-1, 0, -1), (1, 0, -1), (-1, 0, 1), (0, 1, 1), (0, -1, -1), (0, 1, -1), (0, -1, 1)]
+
+
+def _shifted(a, o):
+    """b[z, y, x] = a[(z, y, x) + o] with o = (dx, dy, dz), False outside the box."""
+    b = np.zeros_like(a)
+    n = a.shape
+    src = [slice(max(0, d), n[ax] + min(0, d)) for ax, d in zip((2, 1, 0), o)]
+    dst = [slice(max(0, -d), n[ax] - max(0, d)) for ax, d in zip((2, 1, 0), o)]
+    b[dst[2], dst[1], dst[0]] = a[src[2], src[1], src[0]]
+    return b
+
+
+def clean_solid(solid, kmin=4, max_it=50):
+    """Connectivity cleaning of a solid set (nz, ny, nx) bool, as cement3d/materials.py:51-84: keep the largest
+    18-connected component, then remove, until nothing changes, sites with fewer than kmin bonds to solid
+    neighbours or whose bond directions do not span R^3 (a singular diagonal block of K).  At most 8 of the 18
+    directions are coplanar, so only sites with <= 8 bonds are tested for spanning.  Returns (mask, #removed)."""
+    from scipy import ndimage as ndi
+    s = np.asarray(solid, bool).copy()
+    n0 = int(s.sum())
+    st = ndi.generate_binary_structure(3, 2)
+    nd = np.array(_SC18, float)
+    nd /= np.linalg.norm(nd, axis=1)[:, None]
+    for _ in range(max_it):
+        lab, nl = ndi.label(s, structure=st)
+        if nl > 1:
+            sz = np.bincount(lab.ravel())
+            sz[0] = 0
+            s = lab == sz.argmax()
+        shifted = [_shifted(s, o) for o in _SC18]
+        nb = np.zeros(s.shape, np.int16)
+        for sh in shifted:
+            nb += sh
+        bad = s & (nb < kmin)
+        cand = s & ~bad & (nb <= 8)
+        if cand.any():
+            idx = np.nonzero(cand)
+            M = np.zeros((len(idx[0]), 3, 3))
+            for k, sh in enumerate(shifted):
+                M += sh[idx][:, None, None] * np.outer(nd[k], nd[k])[None]
+            deg = np.linalg.eigvalsh(M)[:, 0] < 1e-8
+            bad[tuple(i[deg] for i in idx)] = True
+        if not bad.any() and nl <= 1:
+            break
+        s &= ~bad
+    return s, n0 - int(s.sum())
+
+
 # ------------------------------------------------------------------------------------------------ coefficients
 def guard_len(dims):
     """Zero guard values on both sides of a packed plane (one z-plane + one row + one site; 2D: one row + one)."""

@@ -10,6 +10,11 @@ bond-parameter gradient kernel (no autograd graph through the operator).
 With mixed=True an SCT in float64 solves with FP32 inner PCG iterations inside an FP64 iterative refinement
 (true residual and solution update in FP64).  Symmetric coefficient sets are packed once per solve into
 forward-slot planes, which halves the coefficient traffic of every K u in the iteration.
+
+SCTV / CoarseTV / FineRefTV (cement3d/torch_v1.py:18-235) add per-specimen removed particles (voids): a bond exists
+only if both ends are solid.  The validity travels to the kernels as one 32-bit bond word per particle (bit k = slot
+k is a bond) instead of a (B, 1, N, 18) float tensor; every kernel (K u, shell sums, stress opp-flag rule, block
+diagonal, gradients) tests the bit.
 """
 import numpy as np
 import torch
@@ -53,18 +58,19 @@ class SCT:
         self.opp = torch.as_tensor(np.arange(NN) ^ 1, device=self.dev)[None, :]
         self.shell = torch.as_tensor(np.asarray(lat.shell), device=self.dev)
         self.last_info = None
+        self.bonds = None      # bond words (B|1, 1, N) int32 of removed particles, or None (full box)
 
     # ------------------------------------------------------------------------------------------ kernels
     def apply(self, u, Kn, Tv):
-        return ops.apply(self.dims, self.h, u, Kn, Tv)
+        return ops.apply(self.dims, self.h, u, Kn, Tv, bonds=self.bonds)
 
     def forces(self, u, Kn, Tv):
-        o = ops.post(self.dims, self.h, u, Kn, Tv, dl=True, force=True)
+        o = ops.post(self.dims, self.h, u, Kn, Tv, dl=True, force=True, bonds=self.bonds)
         return o["dl"], o["force"]
 
     def state(self, u, Kn, Tv):
         """dL (..., N, 18), per-slot own shell sums Si, TSi and neighbour shell sums Sq, TSq."""
-        o = ops.post(self.dims, self.h, u, Kn, Tv, dl=True)
+        o = ops.post(self.dims, self.h, u, Kn, Tv, dl=True, bonds=self.bonds)
         w = o["shell"]                                                   # (..., N, 4) S0 S1 TS0 TS1
         S, TS = w[..., :2], w[..., 2:]
         Si, TSi = S[..., self.shell], TS[..., self.shell]
@@ -74,17 +80,17 @@ class SCT:
         return o["dl"], Si, TSi, Sq, TSq
 
     def stress(self, u, Kn, Tv):
-        return ops.post(self.dims, self.h, u, Kn, Tv, stress=True)["stress"]
+        return ops.post(self.dims, self.h, u, Kn, Tv, stress=True, bonds=self.bonds)["stress"]
 
     def energy(self, u, Kn, Tv):
-        return ops.post(self.dims, self.h, u, Kn, Tv, energy=True)["energy"]
+        return ops.post(self.dims, self.h, u, Kn, Tv, energy=True, bonds=self.bonds)["energy"]
 
     def block_diag(self, Kn, Tv):
-        return ops.block_jacobi(self.dims, self.h, Kn, Tv, raw=True)
+        return ops.block_jacobi(self.dims, self.h, Kn, Tv, raw=True, bonds=self.bonds)
 
     def precond(self, Kn, Tv, free):
         """Block-Jacobi inverse (..., N, d, d), zero on fixed particles (free: (..., N) float)."""
-        return ops.full_from_sym(ops.block_jacobi(self.dims, self.h, Kn, Tv, free=free))
+        return ops.full_from_sym(ops.block_jacobi(self.dims, self.h, Kn, Tv, free=free, bonds=self.bonds))
 
     # ------------------------------------------------------------------------------------------ PCG
     def _coef(self, Kn, Tv, dtype):
@@ -106,11 +112,12 @@ class SCT:
             k32, t32, lay32 = self._coef(Kn, Tv, torch.float32)
             x, info = ops.pcg_refined(self.dims, self.h, b, k64, t64, k32, t32,
                                       minv32=None if Msym is None else Msym.float(), free=free, tol=tol, maxit=maxit,
-                                      x0=x0, check=check, graph=self.graph, layout64=lay64, layout32=lay32)
+                                      x0=x0, check=check, graph=self.graph, layout64=lay64, layout32=lay32,
+                                      bonds=self.bonds)
         else:
             kk, tt, lay = self._coef(Kn, Tv, b.dtype)
             x, info = ops.pcg(self.dims, self.h, b, kk, tt, minv=Msym, free=free, tol=tol, maxit=maxit, x0=x0,
-                              check=check, graph=self.graph, layout=lay)
+                              check=check, graph=self.graph, layout=lay, bonds=self.bonds)
         self.last_info = info
         return x, info.it, info.relmax
 
@@ -123,8 +130,22 @@ class SCT:
         return up + x, it, rel
 
 
+class _Bonds:
+    """Runs a block with op.bonds set to the bond words a forward pass saw (the op may have been re-masked since)."""
+
+    def __init__(self, op, bonds):
+        self.op, self.bonds = op, bonds
+
+    def __enter__(self):
+        self.saved = self.op.bonds
+        self.op.bonds = self.bonds
+
+    def __exit__(self, *exc):
+        self.op.bonds = self.saved
+
+
 class LinSolve(torch.autograd.Function):
-    """u = solve(Kn, Tv) with the exact adjoint gradient w.r.t. Kn, Tv (op: SCT; up, free fixed)."""
+    """u = solve(Kn, Tv) with the exact adjoint gradient w.r.t. Kn, Tv (op: SCT / SCTV; up, free fixed)."""
 
     @staticmethod
     def forward(ctx, Kn, Tv, op, up, free, tol):
@@ -132,7 +153,7 @@ class LinSolve(torch.autograd.Function):
             M = op.precond(Kn, Tv, free)
             u, it, rel = op.solve(Kn, Tv, up, free, tol, M=M)
         ctx.save_for_backward(Kn, Tv, u, M)
-        ctx.op, ctx.free, ctx.tol = op, free, tol
+        ctx.op, ctx.free, ctx.tol, ctx.bonds = op, free, tol, op.bonds
         ctx.info = (it, rel)
         return u
 
@@ -140,9 +161,9 @@ class LinSolve(torch.autograd.Function):
     def backward(ctx, g):
         Kn, Tv, u, M = ctx.saved_tensors
         op, free = ctx.op, ctx.free
-        with torch.no_grad():
+        with torch.no_grad(), _Bonds(op, ctx.bonds):
             lam, _, _ = op.pcg((g * free[..., None]).contiguous(), Kn, Tv, M, free, ctx.tol)
-            gK, gT = ops.bond_grad(op.dims, op.h, Kn, Tv, u, lam, scale=-1.0)
+            gK, gT = ops.bond_grad(op.dims, op.h, Kn, Tv, u, lam, scale=-1.0, bonds=ctx.bonds)
         return gK, gT, None, None, None, None
 
 
@@ -232,9 +253,9 @@ class _PostFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, Kn, Tv, u, op):
         ctx.set_materialize_grads(False)
-        o = ops.post(op.dims, op.h, u, Kn, Tv, ku=True, stress=True, energy=True)
+        o = ops.post(op.dims, op.h, u, Kn, Tv, ku=True, stress=True, energy=True, bonds=op.bonds)
         ctx.save_for_backward(Kn, Tv, u)
-        ctx.op = op
+        ctx.op, ctx.bonds = op, op.bonds
         return o["stress"], o["energy"], o["ku"]
 
     @staticmethod
@@ -244,7 +265,7 @@ class _PostFn(torch.autograd.Function):
         if gs is None and ge is None and gp is None:
             return None, None, None, None
         with torch.no_grad():
-            gu, gK, gT = ops.post_vjp(op.dims, op.h, u, Kn, Tv, g_ku=gp, g_stress=gs, g_energy=ge)
+            gu, gK, gT = ops.post_vjp(op.dims, op.h, u, Kn, Tv, g_ku=gp, g_stress=gs, g_energy=ge, bonds=ctx.bonds)
         gu = gu.sum_to_size(u.shape) if ctx.needs_input_grad[2] else None
         return (gK if ctx.needs_input_grad[0] else None, gT if ctx.needs_input_grad[1] else None, gu, None)
 
@@ -278,3 +299,124 @@ class FineRefT:
         F = _reactions(o["ku"], self.bc.moving, self.loads, Kn.shape[0], Kn)
         return dict(u=self.bmean(u), sig=self.bmean(o["stress"]), w=self.bmean(o["energy"][..., None])[..., 0] / self.h ** 3,
                     F=F, iters=it, relres=rel)
+
+
+# =====================================================================================================================
+class SCTV(SCT):
+    """SC operator with per-specimen removed particles (cement3d/torch_v1.py:18-128): set_mask(solid) with solid
+    (B, N) bool (or None) makes a bond exist only if both ends are solid; shell sums, forces, the stress opp-flag
+    rule (removed neighbours count as missing bonds), the block diagonal and the gradients all follow.  Removed
+    particles have no bonds and must be fixed (free = 0) in solves.  With solid = None it is SCT."""
+
+    def __init__(self, lat, device="cuda", dtype=torch.float64, mixed=False, pack=True, graph=False):
+        super().__init__(lat, device, dtype, mixed=mixed, pack=pack, graph=graph)
+        self.gvalid = torch.as_tensor(lat.valid, device=self.dev)
+        self.l0 = torch.as_tensor(lat.l0, device=self.dev, dtype=dtype)
+        self.Vp = lat.V
+        self.set_mask(None)
+
+    def set_mask(self, solid):
+        """solid: None or (B, N) bool tensor.  Sets the bond words (B, 1, N) and, as the reference exposes them, the
+        validity valid (B|1, 1, N, 18) in the operator dtype."""
+        if solid is None:
+            self.bonds = None
+            self.vb = self.gvalid[None, None]
+        else:
+            solid = torch.as_tensor(solid, device=self.dev)
+            if solid.dim() != 2 or solid.shape[1] != self.N:
+                raise ValueError("solid must be (B, N) = (B, %d), got %s" % (self.N, tuple(solid.shape)))
+            self.bonds = ops.bond_words(self.dims, solid)[:, None]
+            bits = torch.arange(self.lat.NN, device=self.dev, dtype=torch.int32)
+            self.vb = ((self.bonds[..., None] >> bits) & 1).bool()
+        self.valid = self.vb.to(self.dt)
+        return self.bonds
+
+
+class CoarseTV:
+    """Coarse SC model (nc^3 grid, spacing s h) with per-specimen cell masks and the 9 load cases
+    (cement3d/torch_v1.py:159-199)."""
+
+    def __init__(self, nc, s, h=1.0, device="cuda", dtype=torch.float64, loads=LOADS, mixed=False):
+        self.nc, self.s, self.h = nc, s, h
+        self.lat = LatticeGeom(nc, nc, nc, s * h)
+        self.op = SCTV(self.lat, device, dtype, mixed=mixed)
+        self.N = self.lat.N
+        self.loads = loads
+        self.bc = LoadSet(self.lat, self.lat.ijk, (nc, nc, nc), s, h, device, dtype, loads)
+        self.freecell = self.bc.free.bool()
+        surf = [free_surface_cells(ld, (nc, nc, nc)) if ld in PLATES else np.zeros(self.N, bool) for ld in loads]
+        self.surf = torch.as_tensor(np.array(surf), device=device)
+        self.V = (s * h) ** 3
+        self.dt = dtype
+        self.info = None
+
+    def masks(self, cell):
+        """cell (B, N) bool or None -> free (B|1, L, N), up (B|1, L, N, 3); removed cells are fixed at u = 0."""
+        if cell is None:
+            return self.bc.free[None], self.bc.up[None]
+        cm = torch.as_tensor(cell, device=self.bc.free.device).to(self.dt)[:, None]
+        return self.bc.free[None] * cm, self.bc.up[None] * cm[..., None]
+
+    def run(self, Kn, Tv, cell=None, tol=1e-10, differentiable=True, maxit=20000):
+        """Kn, Tv (B, N, 18), cell (B, N) bool or None -> dict u (B,L,N,3), sig (B,L,N,6), w (B,L,N), F (B,L).
+        Kn, Tv are zeroed on missing bonds (differentiably) as in the reference."""
+        self.op.set_mask(cell)
+        Kn = Kn * self.op.valid[:, 0]
+        Tv = Tv * self.op.valid[:, 0]
+        Kn4, Tv4 = Kn[:, None], Tv[:, None]
+        free, up = self.masks(cell)
+        B = Kn.shape[0]
+        if differentiable:
+            u = LinSolve.apply(Kn4, Tv4, self.op, up.expand(B, -1, -1, -1), free.expand(B, -1, -1), tol)
+        else:
+            with torch.no_grad():
+                u, it, rel = self.op.solve(Kn4, Tv4, up, free, tol, maxit=maxit)
+                self.info = (it, rel)
+        sig, U, P = _PostFn.apply(Kn4, Tv4, u, self.op)
+        F = _reactions(P, self.bc.moving, self.loads, B, Kn)
+        return dict(u=u, sig=sig, w=U / self.V, F=F)
+
+
+class FineRefTV:
+    """Fine reference with voids (cement3d/torch_v1.py:202-235): block sums of stress and energy divided by s^3
+    (voids count as zero), displacement block means over the solid particles, grip reactions.  No grad."""
+
+    def __init__(self, n, s, h=1.0, device="cuda", dtype=torch.float64, loads=LOADS, mixed=False):
+        self.n, self.s, self.h = n, s, h
+        self.nc = n // s
+        self.lat = LatticeGeom(n, n, n, h)
+        self.op = SCTV(self.lat, device, dtype, mixed=mixed)
+        IJK = self.lat.ijk // s
+        self.bc = LoadSet(self.lat, IJK, (self.nc,) * 3, s, h, device, dtype, loads)
+        self.loads = loads
+        cid = (IJK[:, 2] * self.nc + IJK[:, 1]) * self.nc + IJK[:, 0]
+        self.cid = torch.as_tensor(cid, device=device)
+        self.Nc = self.nc ** 3
+        self.dt = dtype
+
+    def bsum(self, f):
+        out = torch.zeros(f.shape[:-2] + (self.Nc, f.shape[-1]), device=f.device, dtype=f.dtype)
+        out.index_add_(f.dim() - 2, self.cid, f)
+        return out
+
+    @torch.no_grad()
+    def run(self, Kn, Tv, solid=None, tol=1e-10, maxit=40000):
+        self.op.set_mask(solid)
+        Kn = Kn * self.op.valid[:, 0]
+        Tv = Tv * self.op.valid[:, 0]
+        Kn4, Tv4 = Kn[:, None], Tv[:, None]
+        if solid is None:
+            free, up = self.bc.free[None], self.bc.up[None]
+            sm = torch.ones(1, 1, self.lat.N, device=Kn.device, dtype=self.dt)
+        else:
+            sm = torch.as_tensor(solid, device=Kn.device).to(self.dt)[:, None]
+            free, up = self.bc.free[None] * sm, self.bc.up[None] * sm[..., None]
+        u, it, rel = self.op.solve(Kn4, Tv4, up, free, tol, maxit=maxit)
+        u = u * sm[..., None]
+        o = ops.post(self.op.dims, self.op.h, u, Kn4, Tv4, ku=True, stress=True, energy=True, bonds=self.op.bonds)
+        F = _reactions(o["ku"], self.bc.moving, self.loads, Kn.shape[0], Kn)
+        cnt = self.bsum(sm[..., None])
+        um = self.bsum(u) / cnt.clamp_min(1.0)
+        return dict(u=um, sig=self.bsum(o["stress"]) / self.s ** 3,
+                    w=self.bsum(o["energy"][..., None])[..., 0] / self.s ** 3 / self.h ** 3, F=F, iters=it,
+                    relres=rel)

@@ -6,6 +6,9 @@ kubc3d      48^3 simple cubic, spheres r = 4 +- 1 at fraction 0.25, contrast 10,
             (6 unit strains, eps = 1e-3, plus 3 combined) as nine right-hand sides of one batched refined PCG.
 train3d     16 specimens of 48^3 fine voxels with ellipsoidal inclusions (aspect 1-3, random orientation, fraction
             0.3, contrast 5 or 10, nu = 0.2) coarsened at s = 2 to 24^3 coarse lattices with per-bond Kn, Tv; nine loads.
+porous3d    n^3 four-phase cement-like labels (synth.cement_phases) with the pore voxels removed (the void treatment
+            of cement3d/materials.py), the solid set cleaned (synth.clean_solid); hydrate 1 / portlandite 2 /
+            clinker 5, Tv = 0.3 Kn, two-end mean; tension z on the solid particles of the bottom / top layers.
 """
 import numpy as np
 import torch
@@ -107,3 +110,31 @@ def train3d(B=16, nf=48, s=2, seed=3, device="cuda", phi=0.3):
         Ts.append(Tv)
     f32 = lambda a: torch.as_tensor(np.array(a), device=device, dtype=torch.float32)
     return dict(nc=nc, s=s, lat=lat, Kn=f32(Ks), Tv=f32(Ts))
+
+
+# ------------------------------------------------------------------------------------------------ voids 3D
+PHASE_KN = np.array([0.1, 1.0, 2.0, 5.0])
+
+
+def porous3d(n=256, seed=2024, device="cuda", lab=None):
+    """Removed pore particles on the cement-like labels; returns packed FP64 / FP32 coefficients, the bond words,
+    tension-z boundary data restricted to the solid particles and the solid mask (host and device)."""
+    import lpm_ops as ops
+    if lab is None:
+        lab = synth.cement_phases(n, seed)
+    solid, nrm = synth.clean_solid((lab != 0).reshape(n, n, n))
+    solid = solid.ravel()
+    kn = PHASE_KN[lab]
+    ppar = np.stack([kn, kn, 0.3 * kn], 1)
+    dims = (n, n, n)
+    kp64, tp64 = synth.packed_from_particles(dims, ppar, device, torch.float64)
+    sd = torch.as_tensor(solid, device=device)
+    words = ops.bond_words(dims, sd)
+    z = torch.arange(n, device=device).repeat_interleave(n * n)
+    fixed = ((z == 0) | (z == n - 1)) & sd
+    up = torch.zeros((n ** 3, 3), device=device, dtype=torch.float64)
+    up[(z == n - 1) & sd, 2] = 1e-3 * (n - 1)
+    free = (sd & ~fixed).to(torch.float64)
+    return dict(dims=dims, lab=lab, solid=solid, removed_by_cleaning=int(nrm), kp64=kp64, tp64=tp64,
+                kp32=kp64.float(), tp32=tp64.float(), words=words, up=up, free=free, fixed=fixed, ppar=ppar,
+                solid_fraction=float(solid.mean()))

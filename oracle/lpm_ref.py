@@ -73,6 +73,28 @@ class Box:
         self.opp = np.arange(self.NN) ^ 1
 
 
+class ActiveBox(Box):
+    """Lattice of the active (solid) sites of a box, numbered compactly in box order: the reference's lattice with
+    removed particles (coarse3d/geom3d.py:38-65, `active` mask; only active sites are numbered and a slot is a bond
+    iff its neighbour is an active site).  Every operator function of this module accepts it."""
+
+    def __init__(self, dims, active, h=1.0):
+        super().__init__(dims, h)
+        act = np.asarray(active, bool).ravel()
+        full_nbr, full_ok = self.nbr, self.valid
+        self.sites = np.flatnonzero(act)
+        site_of = np.full(len(act), -1, np.int64)
+        site_of[self.sites] = np.arange(len(self.sites))
+        self.site_of = site_of
+        self.N = len(self.sites)
+        self.ijk = self.ijk[self.sites]
+        self.x = self.x[self.sites]
+        q = np.where(full_ok, full_nbr, 0)[self.sites]
+        ok = full_ok[self.sites] & act[q]
+        self.nbr = np.where(ok, site_of[q], -1)
+        self.valid = ok
+
+
 # ------------------------------------------------------------------------------------------------ operator
 def bond_dl(lat, u):
     """dL (N, NN) = n_k.(u_i - u_q), zero on missing bonds."""
