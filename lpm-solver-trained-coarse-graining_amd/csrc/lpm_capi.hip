@@ -99,6 +99,19 @@ inline int vec_blocks(int64_t N) { return (int)std::min<int64_t>(1024, std::max<
 // tiling overrides (0 = automatic): x tile width of the 3D kernels and planes per block
 int g_tile_wide = 0, g_zc = 0;
 
+// compute units of the current device (cached per device; 256 when the runtime cannot tell)
+int cu_count() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cache[dev] <= 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cache[dev] = n;
+  }
+  return cache[dev];
+}
+
 // tile shapes: 3D float 64x4 or 32x8, double 32x4 or 16x8; 2D float 32x8, double 32x4
 template <int DIM, typename T>
 KuGeom ku_geom(const lpm_grid* g, int nb, int nl, bool bonds) {
@@ -116,10 +129,20 @@ KuGeom ku_geom(const lpm_grid* g, int nb, int nl, bool bonds) {
     k.ntx = (g->nx + k.tx - 1) / k.tx;
     k.nty = (g->ny + k.ty - 1) / k.ty;
     const int64_t xy = (int64_t)k.ntx * k.nty * nb * nl;
-    // about 512 blocks, i.e. one round of two resident blocks per CU: no tail round, and the longest z marches
-    // amortise the 5-plane prologue
-    const int64_t want = std::max<int64_t>(1, (512 + xy - 1) / xy);
-    k.zc = g_zc > 0 ? g_zc : std::max<int>(8, (int)((g->nz + want - 1) / want));
+    // Every 3D tile kernel keeps two blocks resident per CU (`slots` blocks on the chip).  The z range of a column
+    // tile is split into chunks of at least 8 planes so that the launch has about `slots` blocks.  Rounding the
+    // chunk count up can leave a short tail round (slots < blocks < 1.5 slots) that costs nearly a whole round;
+    // then the count is rounded down to one round.  Longer overshoots (1.5 slots and more) are kept: the block
+    // scheduler refills freed slots, and they measured faster than one partly filled round.
+    if (g_zc > 0) {
+      k.zc = g_zc;
+    } else {
+      const int64_t slots = 2 * (int64_t)cu_count();
+      auto zc_for = [&](int64_t chunks) { return std::max<int>(8, (int)((g->nz + chunks - 1) / chunks)); };
+      k.zc = zc_for(std::max<int64_t>(1, (slots + xy - 1) / xy));
+      const int64_t blocks = xy * ((g->nz + k.zc - 1) / k.zc);
+      if (blocks > slots && 2 * blocks < 3 * slots && xy <= slots) k.zc = zc_for(std::max<int64_t>(1, slots / xy));
+    }
     k.nzc = (g->nz + k.zc - 1) / k.zc;
     k.grid = dim3(k.ntx * k.nty * k.nzc, nl, nb);
   } else {
