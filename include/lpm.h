@@ -86,6 +86,14 @@ int lpm_bond_words(const lpm_grid* g, int32_t nbatch, const uint8_t* solid, uint
 int lpm_apply(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* u,
               lpm_bcast bu, const void* kn, const void* tv, lpm_bcast bc, void* out, void* stream);
 
+/* out (nb, nl, N, dim) = mask * (rhs - K u): the true residual of the free DOFs (mask per DOF, 1 = free; null =
+ * all free).  work: lpm_residual_workspace bytes.  Replaces the fused residual _kx_free(sub=True)
+ * (timing/t3d/amg3d.py:199-229) and the FP64 residual of the refinement loop. */
+size_t lpm_residual_workspace(const lpm_grid* g, int32_t nsys, int32_t dtype);
+int lpm_residual(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* u,
+                 lpm_bcast bu, const void* kn, const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm,
+                 const void* rhs, lpm_bcast brhs, void* out, void* work, void* stream);
+
 /* Plane length P of the packed layout (N plus the two guard bands). */
 int64_t lpm_packed_len(const lpm_grid* g);
 
@@ -173,6 +181,71 @@ int lpm_serve_poll(void* server, int32_t* state, int32_t* iters, int32_t* outer,
 int lpm_serve_result(void* server, int32_t slot, double* u, void* stream);
 int lpm_serve_release(void* server, int32_t slot);
 size_t lpm_serve_arena_free(void* server);
+
+/* ---- Aggregation AMG preconditioner of the 3D SC lattice (timing/t3d/amg3d.py, AMG_CFG of timing/t3d/problems.py).
+ * Level 0: the free particles (a box: first particle lo, extent L) in chunks of agg particles per axis, remainder
+ * merged into the last chunk, g0 aggregates per axis (amg3d.py:42-48, 438-462).  The rigid-body-mode tentative
+ * prolongator block of a particle is read from the shape templates tmpl + toff[s] ((sx sy sz, 3, 6) row-major, s =
+ * bit a set for the last chunk along axis a), the Gram-Schmidt basis of gmg._tentative (timing/t3d/gmg.py:70-143),
+ * which depends on the aggregate's shape only.  Stencil level k (k = 0 is the Galerkin operator of level 0) has
+ * the node grid g[k], the 27-point 6x6 stencil A[k] stored structure-of-arrays A[(d 36 + r 6 + c) n + I] with
+ * d = (dz+1) 9 + (dy+1) 3 + (dx+1), and for k < nlev - 1 the per-node blocks Q[k] (n, 6, 6) towards level k + 1,
+ * the inverses Minv[k] (SoA (36, n)) of its diagonal blocks and the damping w[k] = omega / rho_k.  dinv is the
+ * dense inverse of the coarsest stencil.  All pointers are device memory of the call's dtype. */
+#define LPM_AMG_MAXLEV 12
+#define LPM_AMG_SCRATCH_BYTES 4096 /* device scratch of lpm_amg_inv_diag / lpm_amg_coarsest / lpm_amg_dot */
+typedef struct {
+  int32_t lo[3], L[3], g0[3], agg;
+  const void* tmpl;
+  int64_t toff[8];
+  const void* minv0; /* particle block-Jacobi inverse (N, 6) (lpm_block_jacobi layout), zero on fixed DOFs */
+  double w0;         /* omega / rho0 */
+  int32_t nlev, nu, nu_c, gamma, w_from, pad_;
+  int32_t g[LPM_AMG_MAXLEV][3];
+  const void* A[LPM_AMG_MAXLEV];
+  const void* Q[LPM_AMG_MAXLEV];
+  const void* Minv[LPM_AMG_MAXLEV];
+  double w[LPM_AMG_MAXLEV];
+  const void* dinv;
+} lpm_amg;
+
+enum { LPM_AMG_PROLONG_ADD = 0, LPM_AMG_RESTRICT = 1, LPM_AMG_PROBE = 2, LPM_AMG_GATHER = 3 };
+
+/* Level-0 transfer: PROLONG_ADD out (N,3) += P in (nagg,6) (amg3d.py:318-331); RESTRICT out (nagg,6) = P^T in (N,3)
+ * (amg3d.py:303-315); PROBE out (6,N,3) = columns of P on the aggregates of colour col (0..26, (I mod 3) per axis);
+ * GATHER: column j of out[I, d(I, col)] (stencil A[0] layout) = P_I^T in_j for in (6,N,3) = K PROBE: with
+ * lpm_apply this replaces the bond-level Galerkin product _galerkin0 (amg3d.py:151-196). */
+int lpm_amg_level0(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t op, int32_t col, const void* in,
+                   void* out, void* stream);
+/* The same four operations between stencil levels k and k + 1 with the blocks Q[k] (amg3d.py:256-331). */
+int lpm_amg_levelq(const lpm_amg* a, int32_t k, int32_t dtype, int32_t op, int32_t col, const void* in, void* out,
+                   void* stream);
+/* y = A x (b null) or y = b - A x on a stencil of grid (gx, gy, gz); nsys systems of x and y (n, 6) share A
+ * (amg3d.py:230-253). */
+int lpm_amg_st_mv(int32_t gx, int32_t gy, int32_t gz, int32_t nsys, int32_t dtype, const void* A, const void* x,
+                  const void* b, void* y, void* stream);
+/* Minv (SoA (36, n)) = inverses of the diagonal blocks of the stencil A (np.linalg.inv(A[:, 13]), amg3d.py:536);
+ * LPM_E_NOT_SPD on a singular block. */
+int lpm_amg_inv_diag(int32_t n, int32_t dtype, const void* A, void* Minv, void* scratch, void* stream);
+/* out = w M b (add = 0) or out += w M b: M (n, 6) symmetric particle blocks (k = 3) or SoA 6x6 node blocks
+ * (k = 6) (amg3d.py:334-356). */
+int lpm_amg_bj(int64_t n, int32_t k, int32_t dtype, const void* M, const void* b, double w, int32_t add, void* out,
+               void* stream);
+/* dinv ((6n)^2) = inverse of the symmetrised dense matrix of the coarsest stencil (amg3d.py:544-546);
+ * LPM_E_NOT_SPD when it is not positive definite. */
+int lpm_amg_coarsest(int32_t gx, int32_t gy, int32_t gz, int32_t dtype, const void* A, void* dinv, void* scratch,
+                     void* stream);
+/* *out = x . y over n values (fixed-order reduction, host result; synchronises the stream). */
+int lpm_amg_dot(int64_t n, int32_t dtype, const void* x, const void* y, double* out, void* scratch, void* stream);
+/* One preconditioner application z = B b (b (N, 3) zero on fixed DOFs; amg3d.py:620-646 with the W-cycle of
+ * amg3d.py:578-618) and the AMG-PCG K_ff x = b from x = 0 with ||r|| <= tol ||b|| tested every iteration
+ * (amg3d.py:649-672).  kn / tv (layout) define K; mask per DOF (1 = free).  work: lpm_amg_workspace bytes. */
+size_t lpm_amg_workspace(const lpm_grid* g, const lpm_amg* a, int32_t dtype);
+int lpm_amg_vcycle(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t layout, const void* kn, const void* tv,
+                   const uint8_t* mask, const void* b, void* z, void* work, void* stream);
+int lpm_amg_pcg(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t layout, const void* kn, const void* tv,
+                const uint8_t* mask, const void* b, void* x, double tol, int32_t maxit, void* work, int32_t* iters,
+                double* relres, void* stream);
 
 #ifdef __cplusplus
 }

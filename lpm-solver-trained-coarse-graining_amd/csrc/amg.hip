@@ -1,0 +1,520 @@
+// C ABI of the aggregation AMG preconditioner (declarations and reference citations in include/lpm.h): level
+// transfers, probing assembly of the Galerkin stencils, smoothers, the coarsest dense inverse, and the V/W-cycle and
+// AMG-PCG drivers (timing/t3d/amg3d.py:508-672).  The fine-level residuals go through lpm_residual (the fused K u
+// kernel with the residual epilogue).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "amg.hpp"
+#include "capi_common.hpp"
+
+using namespace lpm;
+using lpmc::fail;
+
+namespace {
+
+constexpr int VEC_BLOCKS = 512;  // partial sums of the dot products (LPM_AMG_SCRATCH_BYTES = 8 VEC_BLOCKS)
+static_assert(LPM_AMG_SCRATCH_BYTES >= 8 * VEC_BLOCKS, "scratch holds the partial sums");
+
+inline unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+inline int64_t nodes(const int g[3]) { return (int64_t)g[0] * g[1] * g[2]; }
+
+// level-0 geometry and grids (enough for the transfer primitives used while the hierarchy is being built)
+int check_geo(const lpm_amg* a) {
+  if (!a) return fail(LPM_E_BADARG, "amg: null hierarchy");
+  if (a->nlev < 1 || a->nlev > LPM_AMG_MAXLEV) return fail(LPM_E_BADARG, "amg: %d stencil levels", a->nlev);
+  if (a->agg < 1) return fail(LPM_E_BADARG, "amg: chunk size %d", a->agg);
+  for (int k = 0; k < 3; ++k)
+    if (a->L[k] < 1 || a->g0[k] < 1 || a->g0[k] != std::max(1, a->L[k] / a->agg))
+      return fail(LPM_E_BADARG, "amg: free box and aggregate grid disagree");
+  for (int k = 0; k < a->nlev; ++k)
+    for (int d = 0; d < 3; ++d) {
+      const int nodes_below = k == 0 ? a->L[d] : a->g[k - 1][d];
+      const int want = k == 0 ? a->g0[d] : std::max(1, nodes_below / a->agg);
+      if (a->g[k][d] != want) return fail(LPM_E_BADARG, "amg: grid of stencil level %d inconsistent", k);
+    }
+  return LPM_OK;
+}
+
+// the complete hierarchy (cycle, PCG, workspace)
+int check_amg(const lpm_amg* a) {
+  if (int rc = check_geo(a)) return rc;
+  for (int k = 0; k < a->nlev; ++k) {
+    if (nodes(a->g[k]) < 1 || !a->A[k]) return fail(LPM_E_BADARG, "amg: stencil level %d incomplete", k);
+    if (k + 1 < a->nlev && (!a->Q[k] || !a->Minv[k])) return fail(LPM_E_BADARG, "amg: level %d incomplete", k);
+  }
+  if (!a->dinv) return fail(LPM_E_BADARG, "amg: coarsest inverse missing");
+  return LPM_OK;
+}
+
+Amg0Geo geo0(const lpm_grid* g, const lpm_amg* a) {
+  Amg0Geo r{};
+  r.nx = g->nx;
+  r.ny = g->ny;
+  r.nz = g->nz;
+  for (int k = 0; k < 3; ++k) {
+    r.lo[k] = a->lo[k];
+    r.L[k] = a->L[k];
+    r.g[k] = a->g0[k];
+  }
+  r.agg = a->agg;
+  for (int s = 0; s < 8; ++s) r.toff[s] = a->toff[s];
+  return r;
+}
+
+AmgLGeo geoq(const lpm_amg* a, int k) {
+  AmgLGeo r{};
+  for (int i = 0; i < 3; ++i) {
+    r.g[i] = a->g[k][i];
+    r.gc[i] = a->g[k + 1][i];
+  }
+  r.agg = a->agg;
+  return r;
+}
+
+template <typename T>
+int level0(const lpm_grid* g, const lpm_amg* a, int op, int col, const void* in, void* out, hipStream_t st) {
+  const Amg0Geo geo = geo0(g, a);
+  const T* tm = (const T*)a->tmpl;
+  const int64_t N = (int64_t)g->nx * g->ny * g->nz;
+  const int64_t nagg = nodes(a->g0);
+  switch (op) {
+    case LPM_AMG_PROLONG_ADD:
+      hipLaunchKernelGGL((amg0_prolong_kernel<T>), dim3(blocks_for(N)), dim3(256), 0, st, geo, tm, (const T*)in, (T*)out);
+      break;
+    case LPM_AMG_RESTRICT:
+      hipLaunchKernelGGL((amg0_restrict_kernel<T>), dim3(blocks_for(nagg)), dim3(256), 0, st, geo, tm, (const T*)in,
+                         (T*)out);
+      break;
+    case LPM_AMG_PROBE:
+      hipLaunchKernelGGL((amg0_probe_kernel<T>), dim3(blocks_for(N)), dim3(256), 0, st, geo, tm, col, (T*)out);
+      break;
+    case LPM_AMG_GATHER:
+      hipLaunchKernelGGL((amg0_gather_kernel<T>), dim3(blocks_for(nagg)), dim3(256), 0, st, geo, tm, col,
+                         (const T*)in, (T*)out);
+      break;
+    default:
+      return fail(LPM_E_BADARG, "lpm_amg_level0: bad op %d", op);
+  }
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+template <typename T>
+int levelq(const lpm_amg* a, int k, int op, int col, const void* in, void* out, hipStream_t st) {
+  const AmgLGeo geo = geoq(a, k);
+  const T* Q = (const T*)a->Q[k];
+  const int64_t n = nodes(a->g[k]), nc = nodes(a->g[k + 1]);
+  switch (op) {
+    case LPM_AMG_PROLONG_ADD:
+      hipLaunchKernelGGL((amgq_prolong_kernel<T>), dim3(blocks_for(n)), dim3(256), 0, st, geo, Q, (const T*)in, (T*)out);
+      break;
+    case LPM_AMG_RESTRICT:
+      hipLaunchKernelGGL((amgq_restrict_kernel<T>), dim3(blocks_for(nc)), dim3(256), 0, st, geo, Q, (const T*)in,
+                         (T*)out);
+      break;
+    case LPM_AMG_PROBE:
+      hipLaunchKernelGGL((amgq_probe_kernel<T>), dim3(blocks_for(n)), dim3(256), 0, st, geo, Q, col, (T*)out);
+      break;
+    case LPM_AMG_GATHER:
+      hipLaunchKernelGGL((amgq_gather_kernel<T>), dim3(blocks_for(nc)), dim3(256), 0, st, geo, Q, col, (const T*)in,
+                         (T*)out);
+      break;
+    default:
+      return fail(LPM_E_BADARG, "lpm_amg_levelq: bad op %d", op);
+  }
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+template <typename T>
+int st_mv(const int g[3], int nsys, const void* A, const void* x, const void* b, void* y, hipStream_t st) {
+  const int64_t n = nodes(g);
+  hipLaunchKernelGGL((amg_st_mv_kernel<T>), dim3(blocks_for(n * nsys)), dim3(256), 0, st, g[0], g[1], g[2], nsys,
+                     (const T*)A, (const T*)x, (const T*)b, (T*)y);
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+template <typename T>
+int bj(int64_t n, int k, const void* M, const void* b, double w, int add, void* out, hipStream_t st) {
+  if (k == 3)
+    hipLaunchKernelGGL((amg_bj3_kernel<T>), dim3(blocks_for(n)), dim3(256), 0, st, n, (const T*)M, (const T*)b, (T)w,
+                       add, (T*)out);
+  else
+    hipLaunchKernelGGL((amg_bj6_kernel<T>), dim3(blocks_for(n)), dim3(256), 0, st, (int)n, (const T*)M, (const T*)b,
+                       (T)w, add, (T*)out);
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------- cycle
+template <typename T>
+struct Cycle {
+  const lpm_grid* g;
+  const lpm_amg* a;
+  int layout;
+  const void* kn;
+  const void* tv;
+  const uint8_t* mask;
+  hipStream_t st;
+  // workspace
+  T* r0;      // (N, 3) fine residual
+  void* rwork;  // lpm_residual partial sums
+  T* b[LPM_AMG_MAXLEV];
+  T* e[LPM_AMG_MAXLEV];
+  T* t[LPM_AMG_MAXLEV];
+  T* x[LPM_AMG_MAXLEV];
+  T* r[LPM_AMG_MAXLEV];
+
+  int64_t N() const { return (int64_t)g->nx * g->ny * g->nz; }
+
+  int resid0(const T* rhs, const T* u, T* out) {
+    const int64_t n3 = N() * 3;
+    return lpm_residual(g, 1, 1, sizeof(T) == 4 ? LPM_F32 : LPM_F64, layout, u, lpm_bcast{n3, n3}, kn, tv,
+                        lpm_bcast{0, 0}, mask, lpm_bcast{n3, n3}, rhs, lpm_bcast{n3, n3}, out, rwork, st);
+  }
+
+  // out = approximate solution of A_k out = rhs (one cycle from zero); amg3d.py:590-618
+  int vc(int k, const T* rhs, T* out) {
+    const int64_t n = nodes(a->g[k]);
+    if (k == a->nlev - 1) {
+      const int m = (int)(6 * n);
+      hipLaunchKernelGGL((amg_dense_mv_kernel<T>), dim3(blocks_for(m)), dim3(256), 0, st, m, (const T*)a->dinv, rhs, out);
+      LAUNCHCHK();
+      return LPM_OK;
+    }
+    const double w = a->w[k];
+    int rc;
+    if ((rc = bj<T>(n, 6, a->Minv[k], rhs, w, 0, out, st))) return rc;
+    for (int it = 0; it < a->nu_c - 1; ++it) {
+      if ((rc = st_mv<T>(a->g[k], 1, a->A[k], out, rhs, r[k], st))) return rc;
+      if ((rc = bj<T>(n, 6, a->Minv[k], r[k], w, 1, out, st))) return rc;
+    }
+    if ((rc = st_mv<T>(a->g[k], 1, a->A[k], out, rhs, r[k], st))) return rc;
+    if ((rc = levelq<T>(a, k, LPM_AMG_RESTRICT, 0, r[k], b[k + 1], st))) return rc;
+    if ((rc = coarse(k + 1, b[k + 1], e[k + 1]))) return rc;
+    if ((rc = levelq<T>(a, k, LPM_AMG_PROLONG_ADD, 0, e[k + 1], out, st))) return rc;
+    for (int it = 0; it < a->nu_c; ++it) {
+      if ((rc = st_mv<T>(a->g[k], 1, a->A[k], out, rhs, r[k], st))) return rc;
+      if ((rc = bj<T>(n, 6, a->Minv[k], r[k], w, 1, out, st))) return rc;
+    }
+    return LPM_OK;
+  }
+
+  // e = gamma cycles on level k from zero (one on the coarsest level or below w_from); amg3d.py:578-588
+  int coarse(int k, const T* rhs, T* ek) {
+    int rc;
+    if ((rc = vc(k, rhs, ek))) return rc;
+    if (k == a->nlev - 1) return LPM_OK;
+    const int lvl = k + 1;  // the reference numbers the first stencil level 1
+    const int extra = lvl >= a->w_from ? a->gamma - 1 : 0;
+    const int64_t n6 = nodes(a->g[k]) * 6;
+    for (int c = 0; c < extra; ++c) {
+      if ((rc = st_mv<T>(a->g[k], 1, a->A[k], ek, rhs, t[k], st))) return rc;
+      if ((rc = vc(k, t[k], x[k]))) return rc;
+      hipLaunchKernelGGL((amg_add_kernel<T>), dim3(std::min<unsigned>(blocks_for(n6), 4096)), dim3(256), 0, st, n6,
+                         (const T*)x[k], ek);
+      LAUNCHCHK();
+    }
+    return LPM_OK;
+  }
+
+  // z = B b on the particles; amg3d.py:620-646
+  int fine(const T* bb, T* z) {
+    const int64_t N_ = N();
+    const double w0 = a->w0;
+    int rc;
+    if ((rc = bj<T>(N_, 3, a->minv0, bb, w0, 0, z, st))) return rc;
+    for (int it = 0; it < a->nu - 1; ++it) {
+      if ((rc = resid0(bb, z, r0))) return rc;
+      if ((rc = bj<T>(N_, 3, a->minv0, r0, w0, 1, z, st))) return rc;
+    }
+    if ((rc = resid0(bb, z, r0))) return rc;
+    if ((rc = level0<T>(g, a, LPM_AMG_RESTRICT, 0, r0, b[0], st))) return rc;
+    if ((rc = coarse(0, b[0], e[0]))) return rc;
+    if ((rc = level0<T>(g, a, LPM_AMG_PROLONG_ADD, 0, e[0], z, st))) return rc;
+    for (int it = 0; it < a->nu; ++it) {
+      if ((rc = resid0(bb, z, r0))) return rc;
+      if ((rc = bj<T>(N_, 3, a->minv0, r0, w0, 1, z, st))) return rc;
+    }
+    return LPM_OK;
+  }
+};
+
+struct WorkLayout {
+  size_t r0, rwork, lev[LPM_AMG_MAXLEV][5], pcg[4], part, total;
+};
+
+template <typename T>
+WorkLayout work_layout(const lpm_grid* g, const lpm_amg* a) {
+  WorkLayout L{};
+  const int64_t N = (int64_t)g->nx * g->ny * g->nz;
+  size_t o = 0;
+  const size_t vec = align256(sizeof(T) * N * 3);
+  L.r0 = o;
+  o += vec;
+  L.rwork = o;
+  o += lpm_residual_workspace(g, 1, sizeof(T) == 4 ? LPM_F32 : LPM_F64);
+  for (int k = 0; k < a->nlev; ++k) {
+    const size_t v = align256(sizeof(T) * nodes(a->g[k]) * 6);
+    for (int j = 0; j < 5; ++j) {
+      L.lev[k][j] = o;
+      o += v;
+    }
+  }
+  for (int j = 0; j < 4; ++j) {  // PCG: r, z, p, Ap
+    L.pcg[j] = o;
+    o += vec;
+  }
+  L.part = o;
+  o += align256(sizeof(double) * VEC_BLOCKS);
+  L.total = o;
+  return L;
+}
+
+template <typename T>
+Cycle<T> make_cycle(const lpm_grid* g, const lpm_amg* a, int layout, const void* kn, const void* tv,
+                    const uint8_t* mask, void* work, hipStream_t st) {
+  const WorkLayout L = work_layout<T>(g, a);
+  char* w = (char*)work;
+  Cycle<T> c{};
+  c.g = g;
+  c.a = a;
+  c.layout = layout;
+  c.kn = kn;
+  c.tv = tv;
+  c.mask = mask;
+  c.st = st;
+  c.r0 = (T*)(w + L.r0);
+  c.rwork = w + L.rwork;
+  for (int k = 0; k < a->nlev; ++k) {
+    c.b[k] = (T*)(w + L.lev[k][0]);
+    c.e[k] = (T*)(w + L.lev[k][1]);
+    c.t[k] = (T*)(w + L.lev[k][2]);
+    c.x[k] = (T*)(w + L.lev[k][3]);
+    c.r[k] = (T*)(w + L.lev[k][4]);
+  }
+  return c;
+}
+
+template <typename T>
+int dot(int64_t n, const T* x, const T* y, double* part, double* out, hipStream_t st) {
+  hipLaunchKernelGGL((amg_dot_kernel<T>), dim3(VEC_BLOCKS), dim3(256), 0, st, n, x, y, part);
+  LAUNCHCHK();
+  std::vector<double> h(VEC_BLOCKS);
+  HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(double) * VEC_BLOCKS, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double s = 0.0;
+  for (int i = 0; i < VEC_BLOCKS; ++i) s += h[i];
+  *out = s;
+  return LPM_OK;
+}
+
+template <typename T>
+int pcg(const lpm_grid* g, const lpm_amg* a, int layout, const void* kn, const void* tv, const uint8_t* mask,
+        const T* b, T* x, double tol, int maxit, void* work, int32_t* iters, double* relres, hipStream_t st) {
+  const WorkLayout L = work_layout<T>(g, a);
+  Cycle<T> cy = make_cycle<T>(g, a, layout, kn, tv, mask, work, st);
+  char* w = (char*)work;
+  T* r = (T*)(w + L.pcg[0]);
+  T* z = (T*)(w + L.pcg[1]);
+  T* p = (T*)(w + L.pcg[2]);
+  T* ap = (T*)(w + L.pcg[3]);
+  double* part = (double*)(w + L.part);
+  const int64_t N = cy.N(), n3 = N * 3;
+  const unsigned eg = std::min<unsigned>(blocks_for(n3), 4096);
+  int rc;
+  hipLaunchKernelGGL((amg_fill_kernel<T>), dim3(eg), dim3(256), 0, st, n3, T(0), x);
+  HIPCHK(hipMemcpyAsync(r, b, sizeof(T) * n3, hipMemcpyDeviceToDevice, st));
+  double bn2;
+  if ((rc = dot<T>(n3, b, b, part, &bn2, st))) return rc;
+  const double bn = std::sqrt(bn2);
+  *iters = 0;
+  *relres = 0.0;
+  if (bn == 0.0) return LPM_OK;
+  if ((rc = cy.fine(r, z))) return rc;
+  HIPCHK(hipMemcpyAsync(p, z, sizeof(T) * n3, hipMemcpyDeviceToDevice, st));
+  double rz;
+  if ((rc = dot<T>(n3, r, z, part, &rz, st))) return rc;
+  int it = 0;
+  double rn = bn;
+  const lpm_bcast full{n3, n3}, none{0, 0};
+  for (it = 1; it <= maxit; ++it) {
+    if ((rc = lpm_apply(g, 1, 1, sizeof(T) == 4 ? LPM_F32 : LPM_F64, layout, p, full, kn, tv, none, ap, st)))
+      return rc;
+    double pap;
+    if ((rc = dot<T>(n3, p, ap, part, &pap, st))) return rc;
+    if (!(pap > 0.0)) return fail(LPM_E_NOT_SPD, "AMG-PCG: p.Ap = %g", pap);
+    const double alpha = rz / pap;
+    hipLaunchKernelGGL((amg_cg_step_kernel<T>), dim3(VEC_BLOCKS), dim3(256), 0, st, n3, (T)alpha, (const T*)p,
+                       (const T*)ap, mask, x, r, part);
+    LAUNCHCHK();
+    {
+      std::vector<double> h(VEC_BLOCKS);
+      HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(double) * VEC_BLOCKS, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      double s = 0.0;
+      for (int i = 0; i < VEC_BLOCKS; ++i) s += h[i];
+      rn = std::sqrt(s);
+    }
+    if (!std::isfinite(rn)) return fail(LPM_E_NONFINITE, "AMG-PCG: non-finite residual");
+    if (rn <= tol * bn) break;
+    if ((rc = cy.fine(r, z))) return rc;
+    double rz2;
+    if ((rc = dot<T>(n3, r, z, part, &rz2, st))) return rc;
+    hipLaunchKernelGGL((amg_xpby_kernel<T>), dim3(eg), dim3(256), 0, st, n3, (const T*)z, (T)(rz2 / rz), p);
+    LAUNCHCHK();
+    rz = rz2;
+  }
+  *iters = std::min(it, maxit);
+  *relres = rn / bn;
+  return LPM_OK;
+}
+
+bool fine_ok(const lpm_grid* g, const lpm_amg* a) {
+  if (!g || g->dim != 3) return false;
+  const int n[3] = {g->nx, g->ny, g->nz};
+  for (int k = 0; k < 3; ++k)
+    if (a->lo[k] < 0 || a->lo[k] + a->L[k] > n[k]) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpm_amg_level0(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t op, int32_t col, const void* in,
+                   void* out, void* stream) {
+  if (int rc = check_geo(a)) return rc;
+  if (!fine_ok(g, a)) return fail(LPM_E_BADARG, "lpm_amg_level0: free box outside the 3D lattice");
+  if (!a->tmpl || !out || (op != LPM_AMG_PROBE && !in) || col < 0 || col > 26)
+    return fail(LPM_E_BADARG, "lpm_amg_level0: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == LPM_F64) return level0<double>(g, a, op, col, in, out, st);
+  if (dtype == LPM_F32) return level0<float>(g, a, op, col, in, out, st);
+  return fail(LPM_E_BADARG, "lpm_amg_level0: bad dtype %d", dtype);
+}
+
+int lpm_amg_levelq(const lpm_amg* a, int32_t k, int32_t dtype, int32_t op, int32_t col, const void* in, void* out,
+                   void* stream) {
+  if (int rc = check_geo(a)) return rc;
+  if (k < 0 || k + 1 >= a->nlev || !a->Q[k] || !out || (op != LPM_AMG_PROBE && !in) || col < 0 || col > 26)
+    return fail(LPM_E_BADARG, "lpm_amg_levelq: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == LPM_F64) return levelq<double>(a, k, op, col, in, out, st);
+  if (dtype == LPM_F32) return levelq<float>(a, k, op, col, in, out, st);
+  return fail(LPM_E_BADARG, "lpm_amg_levelq: bad dtype %d", dtype);
+}
+
+int lpm_amg_st_mv(int32_t gx, int32_t gy, int32_t gz, int32_t nsys, int32_t dtype, const void* A, const void* x,
+                  const void* b, void* y, void* stream) {
+  if (gx < 1 || gy < 1 || gz < 1 || nsys < 1 || !A || !x || !y) return fail(LPM_E_BADARG, "lpm_amg_st_mv: bad arguments");
+  const int g[3] = {gx, gy, gz};
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == LPM_F64) return st_mv<double>(g, nsys, A, x, b, y, st);
+  if (dtype == LPM_F32) return st_mv<float>(g, nsys, A, x, b, y, st);
+  return fail(LPM_E_BADARG, "lpm_amg_st_mv: bad dtype %d", dtype);
+}
+
+int lpm_amg_inv_diag(int32_t n, int32_t dtype, const void* A, void* Minv, void* scratch, void* stream) {
+  if (n < 1 || !A || !Minv || !scratch) return fail(LPM_E_BADARG, "lpm_amg_inv_diag: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  int* bad = (int*)scratch;
+  hipLaunchKernelGGL((amg_fill_kernel<int>), dim3(1), dim3(64), 0, st, (int64_t)1, 0, bad);
+  if (dtype == LPM_F64)
+    hipLaunchKernelGGL((amg_inv_diag_kernel<double>), dim3(blocks_for(n)), dim3(256), 0, st, n, (const double*)A,
+                       (double*)Minv, bad);
+  else if (dtype == LPM_F32)
+    hipLaunchKernelGGL((amg_inv_diag_kernel<float>), dim3(blocks_for(n)), dim3(256), 0, st, n, (const float*)A,
+                       (float*)Minv, bad);
+  else
+    return fail(LPM_E_BADARG, "lpm_amg_inv_diag: bad dtype %d", dtype);
+  LAUNCHCHK();
+  int hb = 0;
+  HIPCHK(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (hb) return fail(LPM_E_NOT_SPD, "lpm_amg_inv_diag: singular diagonal block");
+  return LPM_OK;
+}
+
+int lpm_amg_bj(int64_t n, int32_t k, int32_t dtype, const void* M, const void* b, double w, int32_t add, void* out,
+               void* stream) {
+  if (n < 1 || (k != 3 && k != 6) || !M || !b || !out || (k == 6 && n >= ((int64_t)1 << 31)))
+    return fail(LPM_E_BADARG, "lpm_amg_bj: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == LPM_F64) return bj<double>(n, k, M, b, w, add, out, st);
+  if (dtype == LPM_F32) return bj<float>(n, k, M, b, w, add, out, st);
+  return fail(LPM_E_BADARG, "lpm_amg_bj: bad dtype %d", dtype);
+}
+
+int lpm_amg_coarsest(int32_t gx, int32_t gy, int32_t gz, int32_t dtype, const void* A, void* dinv, void* scratch,
+                     void* stream) {
+  const int64_t n = (int64_t)gx * gy * gz, m = 6 * n;
+  if (n < 1 || !A || !dinv || !scratch || m > 8192)
+    return fail(LPM_E_BADARG, "lpm_amg_coarsest: bad arguments (6n = %lld)", (long long)m);
+  hipStream_t st = (hipStream_t)stream;
+  int* bad = (int*)scratch;
+  hipLaunchKernelGGL((amg_fill_kernel<int>), dim3(1), dim3(64), 0, st, (int64_t)1, 0, bad);
+  if (dtype == LPM_F64) {
+    hipLaunchKernelGGL((amg_dense_kernel<double>), dim3(blocks_for(m * m)), dim3(256), 0, st, gx, gy, gz,
+                       (const double*)A, (double*)dinv);
+    hipLaunchKernelGGL((amg_spd_inverse_kernel<double>), dim3(1), dim3(1024), 0, st, (int)m, (double*)dinv, bad);
+  } else if (dtype == LPM_F32) {
+    hipLaunchKernelGGL((amg_dense_kernel<float>), dim3(blocks_for(m * m)), dim3(256), 0, st, gx, gy, gz,
+                       (const float*)A, (float*)dinv);
+    hipLaunchKernelGGL((amg_spd_inverse_kernel<float>), dim3(1), dim3(1024), 0, st, (int)m, (float*)dinv, bad);
+  } else {
+    return fail(LPM_E_BADARG, "lpm_amg_coarsest: bad dtype %d", dtype);
+  }
+  LAUNCHCHK();
+  int hb = 0;
+  HIPCHK(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (hb) return fail(LPM_E_NOT_SPD, "lpm_amg_coarsest: coarsest operator not positive definite");
+  return LPM_OK;
+}
+
+int lpm_amg_dot(int64_t n, int32_t dtype, const void* x, const void* y, double* out, void* scratch, void* stream) {
+  if (n < 1 || !x || !y || !out || !scratch) return fail(LPM_E_BADARG, "lpm_amg_dot: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  double* part = (double*)scratch;
+  if (dtype == LPM_F64) return dot<double>(n, (const double*)x, (const double*)y, part, out, st);
+  if (dtype == LPM_F32) return dot<float>(n, (const float*)x, (const float*)y, part, out, st);
+  return fail(LPM_E_BADARG, "lpm_amg_dot: bad dtype %d", dtype);
+}
+
+size_t lpm_amg_workspace(const lpm_grid* g, const lpm_amg* a, int32_t dtype) {
+  if (!g || check_amg(a) || !fine_ok(g, a)) return 0;
+  return dtype == LPM_F32 ? work_layout<float>(g, a).total : work_layout<double>(g, a).total;
+}
+
+int lpm_amg_vcycle(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t layout, const void* kn, const void* tv,
+                   const uint8_t* mask, const void* b, void* z, void* work, void* stream) {
+  if (int rc = check_amg(a)) return rc;
+  if (!fine_ok(g, a) || !kn || !tv || !b || !z || !work || !a->minv0 || !a->tmpl)
+    return fail(LPM_E_BADARG, "lpm_amg_vcycle: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == LPM_F64) return make_cycle<double>(g, a, layout, kn, tv, mask, work, st).fine((const double*)b, (double*)z);
+  if (dtype == LPM_F32) return make_cycle<float>(g, a, layout, kn, tv, mask, work, st).fine((const float*)b, (float*)z);
+  return fail(LPM_E_BADARG, "lpm_amg_vcycle: bad dtype %d", dtype);
+}
+
+int lpm_amg_pcg(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t layout, const void* kn, const void* tv,
+                const uint8_t* mask, const void* b, void* x, double tol, int32_t maxit, void* work, int32_t* iters,
+                double* relres, void* stream) {
+  if (int rc = check_amg(a)) return rc;
+  if (!fine_ok(g, a) || !kn || !tv || !mask || !b || !x || !work || !iters || !relres || maxit < 1 || !a->minv0 ||
+      !a->tmpl)
+    return fail(LPM_E_BADARG, "lpm_amg_pcg: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == LPM_F64)
+    return pcg<double>(g, a, layout, kn, tv, mask, (const double*)b, (double*)x, tol, maxit, work, iters, relres, st);
+  if (dtype == LPM_F32)
+    return pcg<float>(g, a, layout, kn, tv, mask, (const float*)b, (float*)x, tol, maxit, work, iters, relres, st);
+  return fail(LPM_E_BADARG, "lpm_amg_pcg: bad dtype %d", dtype);
+}
+
+}  // extern "C"

@@ -1,0 +1,561 @@
+// Kernels of the aggregation AMG preconditioner of the 3D SC lattice (timing/t3d/amg3d.py).
+//
+// Level 0 is the particle lattice: the free particles form a box (first particle lo, extent L per axis), aggregated
+// into chunks of `agg` particles per axis with the remainder merged into the last chunk (amg3d.py:42-48), aggregate
+// I = (cz gy + cy) gx + cx.  The tentative prolongator block P_p (3 x 6) of a free particle is the orthonormalised
+// rigid-body basis of its aggregate (gmg.py:70-162); Gram-Schmidt removes the translation part of the rotations
+// about the lattice centroid, so the block depends only on the aggregate's shape (3 or the merged length per axis,
+// at most 8 shapes) and the particle's place in it: the kernels read it from shape templates.
+//
+// Levels >= 1 are node grids with 6 unknowns per node and a 27-point stencil of 6x6 blocks, offset
+// d = (dz+1) 9 + (dy+1) 3 + (dx+1) (amg3d.py:39).  Stencils are stored structure-of-arrays,
+// A[(d 36 + r 6 + c) n + I], so consecutive nodes read consecutive addresses.  Nodes are aggregated like the
+// particles, with per-node blocks Q (n, 6, 6) row-major.
+//
+// Galerkin operators are assembled by probing: with aggregates coloured by (I mod 3) per axis, two aggregates of
+// one colour are at least 3 chunks apart, while the operator couples only neighbouring aggregates (the particle
+// operator reaches 2 sites, a chunk has at least 3; a stencil reaches 1 node), so P^T A (sum over one colour of
+// P e_j) gives, for every aggregate I, column j of its block towards the one aggregate of that colour next to it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lpm {
+
+struct Amg0Geo {
+  int nx, ny, nz;
+  int lo[3], L[3], g[3];
+  int agg;
+  int64_t toff[8];  // template of shape s (bit a = last chunk along axis a): (sx sy sz, 3, 6) at tmpl + toff[s]
+};
+
+struct AmgLGeo {
+  int g[3];   // node grid of this level
+  int gc[3];  // aggregate grid (next level)
+  int agg;
+};
+
+__device__ __forceinline__ int chunk_of(int rel, int agg, int nch) { return min(rel / agg, nch - 1); }
+__device__ __forceinline__ int chunk_len(int c, int agg, int nch, int L) { return c == nch - 1 ? L - agg * (nch - 1) : agg; }
+
+// Level-0 coordinates of particle (x, y, z): aggregate, template row pointer; false outside the free box
+template <typename T>
+__device__ __forceinline__ bool p0_locate(const Amg0Geo& a, const T* __restrict__ tmpl, int x, int y, int z, int& I,
+                                          int cc[3], const T*& prow) {
+  const int p[3] = {x - a.lo[0], y - a.lo[1], z - a.lo[2]};
+  int loc[3], len[3], s = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (p[k] < 0 || p[k] >= a.L[k]) return false;
+    cc[k] = chunk_of(p[k], a.agg, a.g[k]);
+    loc[k] = p[k] - a.agg * cc[k];
+    len[k] = chunk_len(cc[k], a.agg, a.g[k], a.L[k]);
+    if (cc[k] == a.g[k] - 1) s |= 1 << k;
+  }
+  I = (cc[2] * a.g[1] + cc[1]) * a.g[0] + cc[0];
+  prow = tmpl + a.toff[s] + (int64_t)((loc[2] * len[1] + loc[1]) * len[0] + loc[0]) * 18;
+  return true;
+}
+
+// the aggregate of colour col next to aggregate coordinates ic (one axis): -1 if none in [0, g)
+__device__ __forceinline__ int colour_partner(int ic, int col, int g) {
+#pragma unroll
+  for (int o = -1; o <= 1; ++o) {
+    const int j = ic + o;
+    if (j >= 0 && j < g && j % 3 == col) return j;
+  }
+  return -1;
+}
+
+// x_p += P_p e_{agg(p)} on the free particles (amg3d.py:318-331, level 0)
+template <typename T>
+__global__ __launch_bounds__(256) void amg0_prolong_kernel(Amg0Geo a, const T* __restrict__ tmpl,
+                                                           const T* __restrict__ e, T* __restrict__ xo) {
+  const int64_t N = (int64_t)a.nx * a.ny * a.nz;
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int x = (int)(p % a.nx), y = (int)((p / a.nx) % a.ny), z = (int)(p / ((int64_t)a.nx * a.ny));
+  int I, cc[3];
+  const T* pr;
+  if (!p0_locate(a, tmpl, x, y, z, I, cc, pr)) return;
+  T ev[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) ev[c] = e[(int64_t)I * 6 + c];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    T s = T(0);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s += pr[r * 6 + c] * ev[c];
+    xo[p * 3 + r] += s;
+  }
+}
+
+// rc_I = sum_{p in I} P_p^T r_p (amg3d.py:303-315, level 0); one thread per aggregate, members in a fixed order
+template <typename T>
+__global__ __launch_bounds__(256) void amg0_restrict_kernel(Amg0Geo a, const T* __restrict__ tmpl,
+                                                            const T* __restrict__ r, T* __restrict__ rc) {
+  const int nagg = a.g[0] * a.g[1] * a.g[2];
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nagg) return;
+  const int cc[3] = {I % a.g[0], (I / a.g[0]) % a.g[1], I / (a.g[0] * a.g[1])};
+  int s0[3], len[3], sh = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s0[k] = a.lo[k] + a.agg * cc[k];
+    len[k] = chunk_len(cc[k], a.agg, a.g[k], a.L[k]);
+    if (cc[k] == a.g[k] - 1) sh |= 1 << k;
+  }
+  const T* t = tmpl + a.toff[sh];
+  T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  int row = 0;
+  for (int lz = 0; lz < len[2]; ++lz)
+    for (int ly = 0; ly < len[1]; ++ly)
+      for (int lx = 0; lx < len[0]; ++lx, ++row) {
+        const int64_t p = ((int64_t)(s0[2] + lz) * a.ny + (s0[1] + ly)) * a.nx + (s0[0] + lx);
+        const T r0 = r[p * 3], r1 = r[p * 3 + 1], r2 = r[p * 3 + 2];
+        const T* pr = t + (int64_t)row * 18;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] += pr[c] * r0 + pr[6 + c] * r1 + pr[12 + c] * r2;
+      }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) rc[(int64_t)I * 6 + c] = acc[c];
+}
+
+// probe fields of colour col: v_j(p) = column j of P_p on aggregates of that colour, 0 elsewhere; v (6, N, 3)
+template <typename T>
+__global__ __launch_bounds__(256) void amg0_probe_kernel(Amg0Geo a, const T* __restrict__ tmpl, int col,
+                                                         T* __restrict__ v) {
+  const int64_t N = (int64_t)a.nx * a.ny * a.nz;
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const int x = (int)(p % a.nx), y = (int)((p / a.nx) % a.ny), z = (int)(p / ((int64_t)a.nx * a.ny));
+  int I, cc[3];
+  const T* pr;
+  bool on = p0_locate(a, tmpl, x, y, z, I, cc, pr);
+  on = on && cc[0] % 3 == col % 3 && cc[1] % 3 == (col / 3) % 3 && cc[2] % 3 == col / 9;
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) v[(int64_t)j * N * 3 + p * 3 + r] = on ? pr[r * 6 + j] : T(0);
+}
+
+// column j of the blocks A1[I, d(I, col)] = P_I^T (K v_j) from the probe products kv (6, N, 3) (replaces the
+// bond-level Galerkin product of amg3d.py:151-196)
+template <typename T>
+__global__ __launch_bounds__(256) void amg0_gather_kernel(Amg0Geo a, const T* __restrict__ tmpl, int col,
+                                                          const T* __restrict__ kv, T* __restrict__ A) {
+  const int nagg = a.g[0] * a.g[1] * a.g[2];
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nagg) return;
+  const int64_t N = (int64_t)a.nx * a.ny * a.nz;
+  const int cc[3] = {I % a.g[0], (I / a.g[0]) % a.g[1], I / (a.g[0] * a.g[1])};
+  const int cols[3] = {col % 3, (col / 3) % 3, col / 9};
+  int jc[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    jc[k] = colour_partner(cc[k], cols[k], a.g[k]);
+    if (jc[k] < 0) return;
+  }
+  const int d = (jc[2] - cc[2] + 1) * 9 + (jc[1] - cc[1] + 1) * 3 + (jc[0] - cc[0] + 1);
+  int s0[3], len[3], sh = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s0[k] = a.lo[k] + a.agg * cc[k];
+    len[k] = chunk_len(cc[k], a.agg, a.g[k], a.L[k]);
+    if (cc[k] == a.g[k] - 1) sh |= 1 << k;
+  }
+  const T* t = tmpl + a.toff[sh];
+  for (int j = 0; j < 6; ++j) {
+    const T* w = kv + (int64_t)j * N * 3;
+    T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    int row = 0;
+    for (int lz = 0; lz < len[2]; ++lz)
+      for (int ly = 0; ly < len[1]; ++ly)
+        for (int lx = 0; lx < len[0]; ++lx, ++row) {
+          const int64_t p = ((int64_t)(s0[2] + lz) * a.ny + (s0[1] + ly)) * a.nx + (s0[0] + lx);
+          const T w0 = w[p * 3], w1 = w[p * 3 + 1], w2 = w[p * 3 + 2];
+          const T* pr = t + (int64_t)row * 18;
+#pragma unroll
+          for (int r = 0; r < 6; ++r) acc[r] += pr[r] * w0 + pr[6 + r] * w1 + pr[12 + r] * w2;
+        }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) A[((int64_t)d * 36 + r * 6 + j) * nagg + I] = acc[r];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------- levels >= 1
+__device__ __forceinline__ void node_xyz(int n, const int g[3], int c[3]) {
+  c[0] = n % g[0];
+  c[1] = (n / g[0]) % g[1];
+  c[2] = n / (g[0] * g[1]);
+}
+
+// y = A x (sub = 0) or y = b - A x (sub = 1) on a 27-point 6x6 stencil (amg3d.py:230-253); nsys systems of x / y
+// sharing A (probing), b used by system 0 only.  One thread per (node, system).
+template <typename T>
+__global__ __launch_bounds__(256) void amg_st_mv_kernel(int gx, int gy, int gz, int nsys, const T* __restrict__ A,
+                                                        const T* __restrict__ x, const T* __restrict__ b,
+                                                        T* __restrict__ y) {
+  const int n = gx * gy * gz;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * nsys) return;
+  const int I = (int)(t % n), s = (int)(t / n);
+  const int g[3] = {gx, gy, gz};
+  int c[3];
+  node_xyz(I, g, c);
+  const T* xs = x + (int64_t)s * n * 6;
+  T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int d = 0; d < 27; ++d) {
+    const int j0 = c[0] + d % 3 - 1, j1 = c[1] + (d / 3) % 3 - 1, j2 = c[2] + d / 9 - 1;
+    if (j0 < 0 || j0 >= gx || j1 < 0 || j1 >= gy || j2 < 0 || j2 >= gz) continue;
+    const int J = (j2 * gy + j1) * gx + j0;
+    T xv[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) xv[k] = xs[(int64_t)J * 6 + k];
+    const T* Ad = A + (int64_t)d * 36 * n + I;
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[r] += Ad[(int64_t)(r * 6 + k) * n] * xv[k];
+  }
+  T* ys = y + (int64_t)s * n * 6 + (int64_t)I * 6;
+  if (b) {
+#pragma unroll
+    for (int r = 0; r < 6; ++r) ys[r] = b[(int64_t)I * 6 + r] - acc[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < 6; ++r) ys[r] = acc[r];
+  }
+}
+
+// members of aggregate K of a node level: first node and lengths per axis
+__device__ __forceinline__ void agg_box(const AmgLGeo& l, int K, int s0[3], int len[3], int kc[3]) {
+  node_xyz(K, l.gc, kc);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s0[k] = l.agg * kc[k];
+    len[k] = chunk_len(kc[k], l.agg, l.gc[k], l.g[k]);
+  }
+}
+
+// rc_K = sum_{n in K} Q_n^T r_n (amg3d.py:303-315 on a node level)
+template <typename T>
+__global__ __launch_bounds__(256) void amgq_restrict_kernel(AmgLGeo l, const T* __restrict__ Q, const T* __restrict__ r,
+                                                            T* __restrict__ rc) {
+  const int nc = l.gc[0] * l.gc[1] * l.gc[2];
+  const int K = blockIdx.x * blockDim.x + threadIdx.x;
+  if (K >= nc) return;
+  int s0[3], len[3], kc[3];
+  agg_box(l, K, s0, len, kc);
+  T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int z = 0; z < len[2]; ++z)
+    for (int y = 0; y < len[1]; ++y)
+      for (int x = 0; x < len[0]; ++x) {
+        const int64_t n = ((int64_t)(s0[2] + z) * l.g[1] + (s0[1] + y)) * l.g[0] + (s0[0] + x);
+        const T* q = Q + n * 36;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+          const T rv = r[n * 6 + a];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) acc[c] += q[a * 6 + c] * rv;
+        }
+      }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) rc[(int64_t)K * 6 + c] = acc[c];
+}
+
+// x_n += Q_n e_{agg(n)} (amg3d.py:318-331 on a node level)
+template <typename T>
+__global__ __launch_bounds__(256) void amgq_prolong_kernel(AmgLGeo l, const T* __restrict__ Q, const T* __restrict__ e,
+                                                           T* __restrict__ xo) {
+  const int n = l.g[0] * l.g[1] * l.g[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c[3];
+  node_xyz(i, l.g, c);
+  const int K = (chunk_of(c[2], l.agg, l.gc[2]) * l.gc[1] + chunk_of(c[1], l.agg, l.gc[1])) * l.gc[0] +
+                chunk_of(c[0], l.agg, l.gc[0]);
+  const T* q = Q + (int64_t)i * 36;
+  T ev[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) ev[k] = e[(int64_t)K * 6 + k];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    T s = T(0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s += q[a * 6 + k] * ev[k];
+    xo[(int64_t)i * 6 + a] += s;
+  }
+}
+
+// probe fields of colour col on a node level: v_j(n) = column j of Q_n on aggregates of that colour; v (6, n, 6)
+template <typename T>
+__global__ __launch_bounds__(256) void amgq_probe_kernel(AmgLGeo l, const T* __restrict__ Q, int col,
+                                                         T* __restrict__ v) {
+  const int n = l.g[0] * l.g[1] * l.g[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c[3];
+  node_xyz(i, l.g, c);
+  const int k0 = chunk_of(c[0], l.agg, l.gc[0]), k1 = chunk_of(c[1], l.agg, l.gc[1]), k2 = chunk_of(c[2], l.agg, l.gc[2]);
+  const bool on = k0 % 3 == col % 3 && k1 % 3 == (col / 3) % 3 && k2 % 3 == col / 9;
+  const T* q = Q + (int64_t)i * 36;
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+#pragma unroll
+    for (int a = 0; a < 6; ++a) v[((int64_t)j * n + i) * 6 + a] = on ? q[a * 6 + j] : T(0);
+}
+
+// column j of Ac[K, d(K, col)] = Q_K^T (A v_j) from w (6, n, 6) (replaces amg3d.py:256-300)
+template <typename T>
+__global__ __launch_bounds__(256) void amgq_gather_kernel(AmgLGeo l, const T* __restrict__ Q, int col,
+                                                          const T* __restrict__ w, T* __restrict__ Ac) {
+  const int nc = l.gc[0] * l.gc[1] * l.gc[2];
+  const int n = l.g[0] * l.g[1] * l.g[2];
+  const int K = blockIdx.x * blockDim.x + threadIdx.x;
+  if (K >= nc) return;
+  int s0[3], len[3], kc[3];
+  agg_box(l, K, s0, len, kc);
+  const int cols[3] = {col % 3, (col / 3) % 3, col / 9};
+  int jc[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    jc[k] = colour_partner(kc[k], cols[k], l.gc[k]);
+    if (jc[k] < 0) return;
+  }
+  const int d = (jc[2] - kc[2] + 1) * 9 + (jc[1] - kc[1] + 1) * 3 + (jc[0] - kc[0] + 1);
+  for (int j = 0; j < 6; ++j) {
+    const T* wj = w + (int64_t)j * n * 6;
+    T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    for (int z = 0; z < len[2]; ++z)
+      for (int y = 0; y < len[1]; ++y)
+        for (int x = 0; x < len[0]; ++x) {
+          const int64_t m = ((int64_t)(s0[2] + z) * l.g[1] + (s0[1] + y)) * l.g[0] + (s0[0] + x);
+          const T* q = Q + m * 36;
+#pragma unroll
+          for (int a = 0; a < 6; ++a) {
+            const T wv = wj[m * 6 + a];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) acc[r] += q[a * 6 + r] * wv;
+          }
+        }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) Ac[((int64_t)d * 36 + r * 6 + j) * nc + K] = acc[r];
+  }
+}
+
+// Minv_I = inverse of the diagonal block A[I, 13] (Gauss-Jordan with partial pivoting), SoA (36, n); a singular
+// block sets *bad
+template <typename T>
+__global__ __launch_bounds__(256) void amg_inv_diag_kernel(int n, const T* __restrict__ A, T* __restrict__ Minv,
+                                                           int* bad) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  double m[6][12];
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      m[r][c] = (double)A[((int64_t)13 * 36 + r * 6 + c) * n + I];
+      m[r][6 + c] = r == c ? 1.0 : 0.0;
+    }
+  for (int k = 0; k < 6; ++k) {
+    int pv = k;
+    for (int r = k + 1; r < 6; ++r)
+      if (fabs(m[r][k]) > fabs(m[pv][k])) pv = r;
+    if (!(fabs(m[pv][k]) > 0.0)) {
+      atomicExch(bad, 1);
+      return;
+    }
+    if (pv != k)
+      for (int c = 0; c < 12; ++c) {
+        const double tmp = m[k][c];
+        m[k][c] = m[pv][c];
+        m[pv][c] = tmp;
+      }
+    const double ip = 1.0 / m[k][k];
+    for (int c = 0; c < 12; ++c) m[k][c] *= ip;
+    for (int r = 0; r < 6; ++r) {
+      if (r == k) continue;
+      const double f = m[r][k];
+      for (int c = 0; c < 12; ++c) m[r][c] -= f * m[k][c];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) Minv[(int64_t)(r * 6 + c) * n + I] = (T)m[r][6 + c];
+}
+
+// x = w Minv b (add = 0) or x += w Minv b (add = 1), 6x6 SoA blocks (amg3d.py:334-356 on a node level)
+template <typename T>
+__global__ __launch_bounds__(256) void amg_bj6_kernel(int n, const T* __restrict__ Minv, const T* __restrict__ b, T w,
+                                                      int add, T* __restrict__ xo) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  T bv[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) bv[k] = b[(int64_t)I * 6 + k];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    T s = T(0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s += Minv[(int64_t)(r * 6 + k) * n + I] * bv[k];
+    xo[(int64_t)I * 6 + r] = add ? xo[(int64_t)I * 6 + r] + w * s : w * s;
+  }
+}
+
+// particle block Jacobi: x = w M b or x += w M b with the symmetric inverse blocks (N, 6) of lpm_block_jacobi
+// (xx yy zz yz xz xy, zero on fixed DOFs) (amg3d.py:334-356 on the particles)
+template <typename T>
+__global__ __launch_bounds__(256) void amg_bj3_kernel(int64_t N, const T* __restrict__ M, const T* __restrict__ b, T w,
+                                                      int add, T* __restrict__ xo) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const T* m = M + p * 6;
+  const T b0 = b[p * 3], b1 = b[p * 3 + 1], b2 = b[p * 3 + 2];
+  const T z0 = m[0] * b0 + m[5] * b1 + m[4] * b2;
+  const T z1 = m[5] * b0 + m[1] * b1 + m[3] * b2;
+  const T z2 = m[4] * b0 + m[3] * b1 + m[2] * b2;
+  if (add) {
+    xo[p * 3] += w * z0;
+    xo[p * 3 + 1] += w * z1;
+    xo[p * 3 + 2] += w * z2;
+  } else {
+    xo[p * 3] = w * z0;
+    xo[p * 3 + 1] = w * z1;
+    xo[p * 3 + 2] = w * z2;
+  }
+}
+
+// dense (6n)^2 matrix of the coarsest stencil, symmetrised: D = (S + S^T) / 2 (amg3d.py:418-432, 544-546)
+template <typename T>
+__global__ __launch_bounds__(256) void amg_dense_kernel(int gx, int gy, int gz, const T* __restrict__ A,
+                                                        T* __restrict__ D) {
+  const int n = gx * gy * gz, m = 6 * n;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)m * m) return;
+  const int i = (int)(t / m), j = (int)(t % m);
+  const int I = i / 6, r = i % 6, J = j / 6, c = j % 6;
+  const int g[3] = {gx, gy, gz};
+  int ci[3], cj[3];
+  node_xyz(I, g, ci);
+  node_xyz(J, g, cj);
+  const int dx = cj[0] - ci[0], dy = cj[1] - ci[1], dz = cj[2] - ci[2];
+  T v = T(0);
+  if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
+    const int d = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1), dt = 26 - d;
+    v = T(0.5) * (A[((int64_t)d * 36 + r * 6 + c) * n + I] + A[((int64_t)dt * 36 + c * 6 + r) * n + J]);
+  }
+  D[t] = v;
+}
+
+// in-place inverse of an SPD matrix (m x m) by Gauss-Jordan elimination without pivoting, one workgroup; a
+// non-positive pivot sets *bad and stops (the coarsest level's Cholesky of amg3d.py:544-546 in its inverse form)
+template <typename T>
+__global__ __launch_bounds__(1024) void amg_spd_inverse_kernel(int m, T* __restrict__ D, int* bad) {
+  __shared__ double piv;
+  for (int k = 0; k < m; ++k) {
+    __syncthreads();
+    if (threadIdx.x == 0) piv = (double)D[(int64_t)k * m + k];
+    __syncthreads();
+    if (!(piv > 0.0)) {
+      if (threadIdx.x == 0) *bad = 1;
+      return;
+    }
+    const double ip = 1.0 / piv;
+    // row k scaled; D[k][k] becomes 1 / pivot
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+      const int64_t o = (int64_t)k * m + j;
+      D[o] = j == k ? (T)ip : (T)((double)D[o] * ip);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      if (i == k) continue;
+      const double f = (double)D[(int64_t)i * m + k];
+      for (int j = 0; j < m; ++j) {
+        const int64_t o = (int64_t)i * m + j;
+        if (j == k)
+          D[o] = (T)(-f * ip);
+        else
+          D[o] = (T)((double)D[o] - f * (double)D[(int64_t)k * m + j]);
+      }
+    }
+  }
+}
+
+// x = Dinv b for the symmetric dense inverse (m x m); thread i reads column i (= row i), coalesced
+template <typename T>
+__global__ __launch_bounds__(256) void amg_dense_mv_kernel(int m, const T* __restrict__ Dinv, const T* __restrict__ b,
+                                                           T* __restrict__ xo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  T s = T(0);
+  for (int j = 0; j < m; ++j) s += Dinv[(int64_t)j * m + i] * b[j];
+  xo[i] = s;
+}
+
+// ---------------------------------------------------------------------------------------------- vectors
+// per-block partial sums of x.y (fixed order: grid-stride then a block reduction), partial[blockIdx.x]
+template <typename T>
+__global__ __launch_bounds__(256) void amg_dot_kernel(int64_t n, const T* __restrict__ x, const T* __restrict__ y,
+                                                      double* __restrict__ partial) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += (double)x[i] * (double)y[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// x += alpha p; r -= alpha mask Ap; per-block partial sums of r.r (amg3d.py:399-408 with the free mask of apply0)
+template <typename T>
+__global__ __launch_bounds__(256) void amg_cg_step_kernel(int64_t n, T alpha, const T* __restrict__ p,
+                                                          const T* __restrict__ ap, const uint8_t* __restrict__ mask,
+                                                          T* __restrict__ x, T* __restrict__ r,
+                                                          double* __restrict__ partial) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const T ri = r[i] - (mask[i] ? alpha * ap[i] : T(0));
+    r[i] = ri;
+    s += (double)ri * (double)ri;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// p = z + beta p (amg3d.py:411-415)
+template <typename T>
+__global__ __launch_bounds__(256) void amg_xpby_kernel(int64_t n, const T* __restrict__ z, T beta, T* __restrict__ p) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+// out = a - b (elementwise)
+template <typename T>
+__global__ __launch_bounds__(256) void amg_sub_kernel(int64_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                                      T* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = a[i] - b[i];
+}
+
+// y += x (elementwise)
+template <typename T>
+__global__ __launch_bounds__(256) void amg_add_kernel(int64_t n, const T* __restrict__ x, T* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += x[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void amg_fill_kernel(int64_t n, T v, T* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = v;
+}
+
+}  // namespace lpm

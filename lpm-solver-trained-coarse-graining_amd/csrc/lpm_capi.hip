@@ -471,6 +471,36 @@ int lpm_apply(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t 
   return fail(LPM_E_BADARG, "lpm_apply: bad dtype %d", dtype);
 }
 
+size_t lpm_residual_workspace(const lpm_grid* g, int32_t nsys, int32_t dtype) {
+  if (check_grid(g) || nsys < 1) return 0;
+  const size_t w = dtype == LPM_F32 ? 4 : 8;
+  const int blk = g->dim == 3 ? (dtype == LPM_F32 ? ku_blocks_max<3, float>(g, nsys) : ku_blocks_max<3, double>(g, nsys))
+                              : (dtype == LPM_F32 ? ku_blocks_max<2, float>(g, nsys) : ku_blocks_max<2, double>(g, nsys));
+  return align256(w * (size_t)nsys * (size_t)blk);
+}
+
+int lpm_residual(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* u,
+                 lpm_bcast bu, const void* kn, const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm,
+                 const void* rhs, lpm_bcast brhs, void* out, void* work, void* stream) {
+  if (int rc = check_grid(g)) return rc;
+  if (!u || !kn || !tv || !rhs || !out || !work || nb < 1 || nl < 1)
+    return fail(LPM_E_BADARG, "lpm_residual: null pointer or empty batch");
+  if (nb > 65535 || nl > 65535) return fail(LPM_E_BADARG, "lpm_residual: batch too large");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t N = nsites(g);
+  const int D = g->dim;
+  KuCall c{u, bc_of(bu), kn, tv, bc_of(bc), mask, bc_of(bm), rhs, bc_of(brhs), out,
+           {(int64_t)nl * N * D, N * D}, work, nullptr};
+  if (g->dim == 3) {
+    if (dtype == LPM_F32) return ku_call<3, float, EPI_RESID>(g, nb, nl, layout, ku_geom<3, float>(g, nb, nl), c, st);
+    if (dtype == LPM_F64) return ku_call<3, double, EPI_RESID>(g, nb, nl, layout, ku_geom<3, double>(g, nb, nl), c, st);
+  } else {
+    if (dtype == LPM_F32) return ku_call<2, float, EPI_RESID>(g, nb, nl, layout, ku_geom<2, float>(g, nb, nl), c, st);
+    if (dtype == LPM_F64) return ku_call<2, double, EPI_RESID>(g, nb, nl, layout, ku_geom<2, double>(g, nb, nl), c, st);
+  }
+  return fail(LPM_E_BADARG, "lpm_residual: bad dtype %d", dtype);
+}
+
 int lpm_bond_words(const lpm_grid* g, int32_t nbatch, const uint8_t* solid, uint32_t* words, void* stream) {
   if (int rc = check_grid(g)) return rc;
   if (!solid || !words || nbatch < 1 || nbatch > 65535)

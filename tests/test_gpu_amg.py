@@ -1,0 +1,130 @@
+"""GPU aggregation AMG (amg3d.py of the package) against the AMG oracle (oracle/amg_ref.py): hierarchy geometry,
+Galerkin stencils, spectral radii, the W-cycle and AMG-PCG (timing/t3d/amg3d.py)."""
+import os
+import sys
+import types
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+ROOT = os.path.join(os.path.dirname(__file__), "..")
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "lpm-solver-trained-coarse-graining_amd"))
+import amg_ref as A  # noqa: E402
+import lpm_ref as R  # noqa: E402
+
+torch = pytest.importorskip("torch")
+import amg3d  # noqa: E402
+
+FP64_TOL = 1e-10
+
+
+def _lat(dims, device="cuda"):
+    return types.SimpleNamespace(nx=dims[0], ny=dims[1], nz=dims[2], h=1.0, full=True, device=torch.device(device))
+
+
+def _problem(dims, seed=0, load="k_xy"):
+    rng = np.random.default_rng(seed)
+    lat = R.Box(dims)
+    Kn, Tv = R.random_symmetric_params(lat, rng, kn=(0.5, 5.0), tv=(-0.05, 0.3))
+    fixed, up, _ = R.prescribed3(load, lat.x, lat.ijk, dims, 1)
+    b = -R.apply(lat, up * fixed[:, None], Kn, Tv) * (~fixed)[:, None]
+    return lat, Kn, Tv, ~fixed, b
+
+
+@pytest.mark.parametrize("dims,coarse_max", [((14, 13, 12), 30), ((17, 11, 16), 50), ((9, 8, 7), 3000)])
+def test_hierarchy_geometry_matches_oracle(dims, coarse_max):
+    """Host-side hierarchy (CPU): shape templates evaluated per particle, the R factors of level 0 and the node
+    blocks / R factors of the coarse levels equal the oracle's per-aggregate Gram-Schmidt."""
+    lat, Kn, Tv, free, _ = _problem(dims)
+    ref = A.AMGGeom(lat, free, agg=3, coarse_max=coarse_max)
+    geo = amg3d.AMGGeom(_lat(dims, "cpu"), free, agg=3, coarse_max=coarse_max)
+    assert tuple(geo.g0) == tuple(ref.l0["g"]) and len(geo.levels) == len(ref.levels)
+    # per-particle blocks from the templates
+    fi = np.nonzero(free)[0]
+    P = np.zeros((lat.N, 3, 6))
+    a = geo.agg
+    for p in fi:
+        rel = lat.ijk[p] - geo.lo
+        cc = np.minimum(rel // a, np.array(geo.g0) - 1)
+        loc = rel - a * cc
+        s = sum(1 << d for d in range(3) if cc[d] == geo.g0[d] - 1)
+        ln = [geo.L[d] - a * (geo.g0[d] - 1) if cc[d] == geo.g0[d] - 1 else a for d in range(3)]
+        row = (loc[2] * ln[1] + loc[1]) * ln[0] + loc[0]
+        P[p] = geo.tmpl64[geo.toff[s] + 18 * row: geo.toff[s] + 18 * row + 18].reshape(3, 6)
+    assert np.abs(P - ref.l0["P"]).max() < 1e-12
+    assert np.abs(geo.Bc0 - ref.l0["Bc"]).max() < 1e-10 * max(1.0, np.abs(ref.l0["Bc"]).max())
+    for Lg, Lr in zip(geo.levels, ref.levels):
+        assert tuple(Lg["gc"]) == tuple(Lr["gc"])
+        assert np.abs(Lg["Q64"] - Lr["Q"]).max() < 1e-10
+
+
+def _amg_pair(dims, coarse_max, gamma=2, seed=0, load="k_xy", **kw):
+    lat, Kn, Tv, free, b = _problem(dims, seed, load)
+    ref_geom = A.AMGGeom(lat, free, agg=3, coarse_max=coarse_max)
+    ref = A.AMG3D(ref_geom, Kn, Tv, nu=1, gamma=gamma, power_its=5)
+    geo = amg3d.AMGGeom(_lat(dims), free, agg=3, coarse_max=coarse_max)
+    gpu = amg3d.AMG3D(geo, Kn, Tv, nu=1, gamma=gamma, power_its=5, **kw)
+    return lat, Kn, Tv, free, b, ref, gpu
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims,coarse_max", [((14, 13, 12), 30), ((20, 19, 21), 100)])
+def test_stencils_and_rho_match_oracle(dims, coarse_max):
+    lat, Kn, Tv, free, b, ref, gpu = _amg_pair(dims, coarse_max)
+    assert len(gpu.A) == len(ref.A)
+    for k in range(len(ref.A)):
+        got = gpu.stencil(k).cpu().numpy()
+        scale = np.abs(ref.A[k]).max()
+        assert np.abs(got - ref.A[k]).max() <= FP64_TOL * scale, "stencil level %d" % (k + 1)
+    assert abs(gpu.rho0 - ref.rho0) <= FP64_TOL * ref.rho0
+    for a_, b_ in zip(gpu.rho, ref.rho):
+        assert abs(a_ - b_) <= FP64_TOL * b_
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gamma", [1, 2])
+def test_cycle_matches_oracle(gamma):
+    dims = (20, 19, 21)
+    lat, Kn, Tv, free, b, ref, gpu = _amg_pair(dims, 100, gamma=gamma)
+    rng = np.random.default_rng(7)
+    r = rng.standard_normal((lat.N, 3)) * free[:, None]
+    z = gpu.vcycle(torch.as_tensor(r, device="cuda")).cpu().numpy()
+    zr = ref.vcycle(r)
+    assert np.abs(z - zr).max() <= FP64_TOL * np.abs(zr).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("load", ["k_xy", "tb_tz"])
+def test_amg_pcg_matches_superlu_and_oracle(load):
+    dims = (21, 20, 19)
+    lat, Kn, Tv, free, b, ref, gpu = _amg_pair(dims, 100, load=load)
+    x, it, rr = gpu.pcg(torch.as_tensor(b, device="cuda"), 1e-10, 500)
+    x = x.cpu().numpy()
+    assert rr <= 1e-10
+    xr, itr, _ = ref.pcg(b, 1e-10, 500)
+    assert abs(it - itr) <= 1, (it, itr)
+    K = R.stiffness(lat, Kn, Tv)
+    fd = np.repeat(free, 3)
+    xs = np.zeros(3 * lat.N)
+    xs[fd] = spla.splu(sp.csc_matrix(K[fd][:, fd])).solve(b.reshape(-1)[fd])
+    assert np.abs(x.reshape(-1) - xs).max() <= 1e-8 * np.abs(xs).max()
+
+
+@pytest.mark.gpu
+def test_amg_pcg_zero_rhs_and_single_level():
+    dims = (9, 8, 7)
+    lat, Kn, Tv, free, b = _problem(dims)
+    geo = amg3d.AMGGeom(_lat(dims), free, agg=3, coarse_max=3000)
+    assert len(geo.levels) == 0
+    gpu = amg3d.AMG3D(geo, Kn, Tv, gamma=2, power_its=5)
+    x, it, rr = gpu.pcg(torch.zeros((lat.N, 3), device="cuda", dtype=torch.float64), 1e-10)
+    assert it == 0 and rr == 0.0 and float(x.abs().max()) == 0.0
+    x, it, rr = gpu.pcg(torch.as_tensor(b, device="cuda"), 1e-10)
+    K = R.stiffness(lat, Kn, Tv)
+    fd = np.repeat(free, 3)
+    xs = np.zeros(3 * lat.N)
+    xs[fd] = spla.splu(sp.csc_matrix(K[fd][:, fd])).solve(b.reshape(-1)[fd])
+    assert rr <= 1e-10 and np.abs(x.cpu().numpy().reshape(-1) - xs).max() <= 1e-8 * np.abs(xs).max()
