@@ -359,6 +359,88 @@ def bench_kubc3d(a):
                             "inner iterations over the 9 RHS" % (done, t_it)}})
 
 
+def bench_amg256(a):
+    """The 256^3 cement-like problem of the default line solved by AMG-PCG (timing/t3d/amg3d.py with AMG_CFG of
+    timing/t3d/problems.py:19: 3x3x3 rigid-body-mode aggregation, W(1,1) cycles, dense coarsest level) in FP64 to
+    rel-res tol.  The timed solve includes the numeric setup (block Jacobi, rho by power iteration, Galerkin
+    stencils by probing, coarsest inverse), as the reference's timed AMG solves do; the aggregation geometry is
+    built once per lattice and BC set, as in the reference."""
+    import types
+    import torch
+    import amg3d
+    dev = torch.device("cuda:0")
+    P = build_problem(a.n, a.seed, dev)
+    ops, L = P["ops"], P["L"]
+    dims = P["dims"]
+    n = a.n
+    N = n ** 3
+    b = -ops.apply(dims, 1.0, P["up"], P["kp64"], P["tp64"], layout=L.PACKED) * P["free"][:, None]
+    lat = types.SimpleNamespace(nx=n, ny=n, nz=n, h=1.0, full=True, device=dev)
+    t0 = time.perf_counter()
+    cfg = dict(amg3d.AMG_CFG)
+    geom = amg3d.AMGGeom(lat, (P["free"] > 0).cpu().numpy(), agg=cfg["agg"], coarse_max=cfg["coarse_max"])
+    t_geom = time.perf_counter() - t0
+    print("[bench] amg geometry %.1f s: %s" % (t_geom, geom.describe()), file=sys.stderr, flush=True)
+
+    def step(fp64=False):
+        ts = time.perf_counter()
+        amg = amg3d.AMG3D(geom, P["kp64"] if fp64 else P["kp32"], P["tp64"] if fp64 else P["tp32"], nu=cfg["nu"],
+                          gamma=cfg["gamma"], power_its=cfg["power_its"], starts="device",
+                          dtype=torch.float64 if fp64 else torch.float32)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - ts
+        if fp64:
+            x, it, rr = amg.pcg(b, a.tol, 5000)
+        else:
+            x, it, _, rr = amg.pcg_refined(b, P["kp64"], P["tp64"], tol=a.tol, inner_tol=a.inner_tol, maxit=1000)
+        torch.cuda.synchronize()
+        return x, it, rr, t_setup, time.perf_counter() - ts, amg
+
+    for w in range(a.warmup):
+        _, it, rr, ts_, tt, _ = step()
+        print("[bench] warmup %d: %d its, relres %.2e, setup %.3f s, total %.3f s" % (w, it, rr, ts_, tt),
+              file=sys.stderr, flush=True)
+    lat_s, its, setups = [], [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        x, it, rr, ts_, tt, amg = step()
+        lat_s.append(tt)
+        its.append(it)
+        setups.append(ts_)
+        print("[bench] step %d: %.3f s (setup %.3f s), %d its, relres %.2e" % (s, tt, ts_, it, rr), file=sys.stderr,
+              flush=True)
+    total = time.perf_counter() - t0
+    # the same solve with the FP64 hierarchy and FP64 PCG (the reference's arithmetic), its true residual, and the
+    # distance of both solutions to the block-Jacobi refined solve
+    x64, it64, _, setup64, t64, amg64 = step(fp64=True)
+    r = b + amg64._kff(x64)                    # _kff(x) = -K_ff x
+    rel_true = float(torch.linalg.vector_norm(r * P["free"][:, None]) / torch.linalg.vector_norm(b))
+    xb, info = solve(P, a.tol)
+    dx64 = float(torch.linalg.vector_norm(x64 - xb) / torch.linalg.vector_norm(xb))
+    dx = float(torch.linalg.vector_norm(x - xb) / torch.linalg.vector_norm(xb))
+    lat_ms = np.array(lat_s) * 1e3
+    out = {
+        "metric": METRIC, "value": a.steps / total, "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": total / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 inner + f64 refinement", "data": "synthetic",
+        "config": {"workload": "SC LPM %d^3 (%d DOF), 4-phase cement-like, tension z, AMG-PCG (3x3x3 rigid-body-mode "
+                               "aggregation, W(1,1)) FP32 inside FP64 refinement to rel-res %g, numeric setup inside "
+                               "the timed solve"
+                               % (n, 3 * N, a.tol), "n": n, "tol": a.tol, "seed": a.seed, "amg": cfg, "inner_tol": a.inner_tol,
+                   "levels": geom.describe()},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
+                       "max": float(lat_ms.max())},
+        "solve": {"pcg_iters": int(np.median(its)), "setup_s": float(np.median(setups)),
+                  "ms_per_iter": float((np.median(lat_s) - np.median(setups)) / max(1, np.median(its)) * 1e3),
+                  "relres_true_fp64": rr, "rel_diff_vs_block_jacobi_solve": dx, "block_jacobi_iters": info.it,
+                  "geometry_s": t_geom},
+        "fp64_amg_pcg": {"s_per_solve": t64, "setup_s": setup64, "pcg_iters": it64, "relres_true_fp64": rel_true,
+                         "rel_diff_vs_block_jacobi_solve": dx64},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_porous256(a):
     """256^3 cement-like microstructure with the pore particles removed (bond words), tension z, FP32 PCG + FP64
     refinement to 1e-8; the masked K p kernel's roofline and the C oracle's PCG on the same reduced lattice."""
@@ -617,13 +699,14 @@ def main():
     ap.add_argument("--probe", type=int, default=0, help="run the profiler probe on an n^3 lattice and exit")
     ap.add_argument("--probe-reps", type=int, default=10)
     ap.add_argument("--config", default="cube256",
-                    choices=["cube256", "plain2d", "kubc3d", "serve", "train3d", "porous256"])
+                    choices=["cube256", "plain2d", "kubc3d", "serve", "train3d", "porous256", "amg256"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--slots", type=int, default=512)
     ap.add_argument("--budget", type=int, default=400000)
     ap.add_argument("--load", type=float, default=0.8)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--threads", type=int, default=1024, help="serving: workgroup size per request slot")
+    ap.add_argument("--inner-tol", type=float, default=1e-6, help="amg256: relative tolerance of the FP32 inner solves")
     a = ap.parse_args()
     if a.probe:
         probe(a.probe, a.probe_reps)
@@ -634,7 +717,7 @@ def main():
         sys.exit(2)
     if a.config != "cube256":
         {"plain2d": bench_plain2d, "kubc3d": bench_kubc3d, "serve": bench_serve, "train3d": bench_train3d,
-         "porous256": bench_porous256}[a.config](a)
+         "porous256": bench_porous256, "amg256": bench_amg256}[a.config](a)
         return
     import torch
     dev = torch.device("cuda:0")

@@ -220,7 +220,7 @@ int lpm_amg_level0(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t o
 /* The same four operations between stencil levels k and k + 1 with the blocks Q[k] (amg3d.py:256-331). */
 int lpm_amg_levelq(const lpm_amg* a, int32_t k, int32_t dtype, int32_t op, int32_t col, const void* in, void* out,
                    void* stream);
-/* y = A x (b null) or y = b - A x on a stencil of grid (gx, gy, gz); nsys systems of x and y (n, 6) share A
+/* y = A x (b null) or y = b - A x on a stencil of grid (gx, gy, gz); nsys systems of x and y (n, 6) share A and b
  * (amg3d.py:230-253). */
 int lpm_amg_st_mv(int32_t gx, int32_t gy, int32_t gz, int32_t nsys, int32_t dtype, const void* A, const void* x,
                   const void* b, void* y, void* stream);
@@ -246,6 +246,17 @@ int lpm_amg_vcycle(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t l
 int lpm_amg_pcg(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t layout, const void* kn, const void* tv,
                 const uint8_t* mask, const void* b, void* x, double tol, int32_t maxit, void* work, int32_t* iters,
                 double* relres, void* stream);
+
+/* Mixed precision: FP32 AMG-PCG inner solves (hierarchy a32 in FP32, operator kn32 / tv32) inside an FP64
+ * iterative refinement in which the true residual r = mask (b - K x) (kn64 / tv64) and the update x += d are FP64;
+ * each inner solve stops at min(1/2, max(inner_tol, tol / (2 relres))) of its right-hand side.  iters = total inner
+ * iterations, outer = refinement steps, relres = true FP64 relative residual.  Same problem as lpm_amg_pcg; no
+ * counterpart in the reference, whose AMG-PCG is FP64 (amg3d.py:649-672). */
+size_t lpm_amg_refined_workspace(const lpm_grid* g, const lpm_amg* a32);
+int lpm_amg_pcg_refined(const lpm_grid* g, const lpm_amg* a32, int32_t layout64, const void* kn64, const void* tv64,
+                        int32_t layout32, const void* kn32, const void* tv32, const uint8_t* mask, const void* b64,
+                        void* x64, double tol, double inner_tol, int32_t max_outer, int32_t maxit, void* work,
+                        int32_t* iters, int32_t* outer, double* relres, void* stream);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,10 @@ _SIGS = {
     "lpm_amg_vcycle": (L.I32, [L.GP, ctypes.POINTER(AmgStruct), L.I32, L.I32, L.P, L.P, L.P, L.P, L.P, L.P, L.P]),
     "lpm_amg_pcg": (L.I32, [L.GP, ctypes.POINTER(AmgStruct), L.I32, L.I32, L.P, L.P, L.P, L.P, L.P, ctypes.c_double,
                             L.I32, L.P, ctypes.POINTER(L.I32), ctypes.POINTER(ctypes.c_double), L.P]),
+    "lpm_amg_refined_workspace": (ctypes.c_size_t, [L.GP, ctypes.POINTER(AmgStruct)]),
+    "lpm_amg_pcg_refined": (L.I32, [L.GP, ctypes.POINTER(AmgStruct), L.I32, L.P, L.P, L.I32, L.P, L.P, L.P, L.P, L.P,
+                                    ctypes.c_double, ctypes.c_double, L.I32, L.I32, L.P, ctypes.POINTER(L.I32),
+                                    ctypes.POINTER(L.I32), ctypes.POINTER(ctypes.c_double), L.P]),
 }
 L.SIGS.update(_SIGS)
 
@@ -264,8 +268,9 @@ def _dot(x, y):
 class AMG3D:
     """Numeric hierarchy for one specimen (Kn, Tv) on an AMGGeom; vcycle() is the preconditioner, pcg() the solver
     (amg3d.py:508-672).  Kn, Tv: directed (N, 18) FP64 arrays or tensors (symmetric pairs).  dtype: arithmetic of
-    the hierarchy, the cycle and the PCG (FP64 as the reference).  starts: optional list of power-iteration start
-    vectors per level (default: default_rng(0).standard_normal, as the reference)."""
+    the hierarchy, the cycle and the PCG (FP64 as the reference).  starts: power-iteration start vectors per level
+    as a dict {level: array} (default: default_rng(0).standard_normal, as the reference), or "device" for normal vectors drawn on the GPU
+    (seed 0), which avoids the host RNG on large lattices.  Kn, Tv may also be packed forward planes (9, P)."""
 
     def __init__(self, geom, Kn, Tv, nu=1, omega=1.32, power_its=10, Minv0=None, nu_c=None, gamma=1,
                  fine_additive=False, w_from=1, rho0=None, dtype=torch.float64, starts=None, rho=None):
@@ -283,9 +288,12 @@ class AMG3D:
         st = L.stream_of(torch.empty(0, device=dev))
         D = geom.dev(dtype)
         self.mask = D["mask"]
-        kn = torch.as_tensor(Kn, device=dev, dtype=dtype).reshape(N, 18)
-        tv = torch.as_tensor(Tv, device=dev, dtype=dtype).reshape(N, 18)
-        self.kn, self.tv, _ = ops.pack(dims, kn, tv)
+        kn = torch.as_tensor(Kn, device=dev, dtype=dtype)
+        tv = torch.as_tensor(Tv, device=dev, dtype=dtype)
+        if tuple(kn.shape) == (9, ops.packed_len(dims)):
+            self.kn, self.tv = kn.contiguous(), tv.contiguous()      # packed forward planes already
+        else:
+            self.kn, self.tv, _ = ops.pack(dims, kn.reshape(N, 18), tv.reshape(N, 18))
         self.grid = L.grid_of(dims, geom.h)
         dc = L.dtype_code(dtype)
         free = torch.as_tensor(geom.free, device=dev)
@@ -314,17 +322,24 @@ class AMG3D:
         self.s = s
         self.rwork = torch.empty(max(1, lib.lpm_residual_workspace(ctypes.byref(self.grid), 1, dc)) // 8 + 1,
                                  device=dev, dtype=torch.float64)
-        rng_starts = starts or {}
+        device_starts = starts == "device"
+        rng_starts = {} if (starts is None or device_starts) else starts
+        gen = torch.Generator(device=dev).manual_seed(0) if device_starts else None
+
+        def start_vec(k, shape):
+            if k in rng_starts:
+                return torch.as_tensor(rng_starts[k], device=dev, dtype=dtype).reshape(shape)
+            if device_starts:
+                return torch.randn(shape, generator=gen, device=dev, dtype=dtype)
+            return torch.as_tensor(np.random.default_rng(0).standard_normal(shape), device=dev, dtype=dtype)
+
         # rho of the finest level (amg3d.py:528-529)
         if rho0 is not None:
             self.rho0 = float(rho0)
         elif rho is not None:
             self.rho0 = float(rho[0])
         else:
-            st0 = rng_starts.get(0)
-            if st0 is None:
-                st0 = np.random.default_rng(0).standard_normal((N, 3))
-            x0 = torch.as_tensor(st0, device=dev, dtype=dtype).reshape(N, 3) * free[:, None].to(dtype)
+            x0 = start_vec(0, (N, 3))
             self.rho0 = _power_rho(self._kff, lambda v, w: self._bj3(v, w), x0, int(power_its))
         s.w0 = self.omega / self.rho0
         # Galerkin operator of level 0 by probing (replaces _galerkin0, amg3d.py:151-196)
@@ -349,10 +364,7 @@ class AMG3D:
             if rho is not None:
                 r_k = float(rho[k + 1])
             else:
-                stk = rng_starts.get(k + 1)
-                if stk is None:
-                    stk = np.random.default_rng(0).standard_normal((n, 6))
-                xk = torch.as_tensor(stk, device=dev, dtype=dtype).reshape(n, 6)
+                xk = start_vec(k + 1, (n, 6))
                 r_k = _power_rho(lambda x, A=A, g=g: self._st(A, g, x), lambda x, w, Mi=Mi: self._bj6(Mi, x, w),
                                  xk, int(power_its))
             self.rho.append(r_k)
@@ -438,3 +450,30 @@ class AMG3D:
                                    _p(self.kn), _p(self.tv), _p(self.mask), _p(b), _p(x), float(tol), int(maxit),
                                    _p(self.work), ctypes.byref(it), ctypes.byref(rr), L.stream_of(b)))
         return x, int(it.value), float(rr.value)
+
+    def pcg_refined(self, b, Kn64, Tv64, tol=1e-8, inner_tol=1e-4, max_outer=20, maxit=5000):
+        """FP32 AMG-PCG inner solves (this hierarchy, dtype float32) inside an FP64 iterative refinement: the true
+        residual r = mask (b - K x) with the FP64 operator (Kn64, Tv64: packed (9, P) or directed (N, 18)) and the
+        update x += d are FP64.  Returns x (N, 3) FP64, total inner iterations, refinement steps, true relres."""
+        if self.dtype != torch.float32:
+            raise ValueError("pcg_refined runs the FP32 hierarchy inside the FP64 refinement (build with "
+                             "dtype=torch.float32)")
+        dev, dims, N = self.geom.device, self.geom.dims, self.N
+        k64 = torch.as_tensor(Kn64, device=dev, dtype=torch.float64)
+        t64 = torch.as_tensor(Tv64, device=dev, dtype=torch.float64)
+        if tuple(k64.shape) != (9, ops.packed_len(dims)):
+            k64, t64, _ = ops.pack(dims, k64.reshape(N, 18), t64.reshape(N, 18))
+        b = torch.as_tensor(b, device=dev, dtype=torch.float64).reshape(N, 3).contiguous()
+        lib = _lib()
+        ws = lib.lpm_amg_refined_workspace(ctypes.byref(self.grid), ctypes.byref(self.s))
+        if ws == 0:
+            raise RuntimeError("lpm_amg_refined_workspace: %s" % lib.lpm_last_error().decode())
+        if getattr(self, "_rwork", None) is None or self._rwork.numel() * 8 < ws:
+            self._rwork = torch.empty(ws // 8 + 1, device=dev, dtype=torch.float64)
+        x = torch.empty_like(b)
+        it, nout, rr = L.I32(), L.I32(), ctypes.c_double()
+        L.check(lib.lpm_amg_pcg_refined(ctypes.byref(self.grid), ctypes.byref(self.s), L.PACKED, _p(k64), _p(t64),
+                                        L.PACKED, _p(self.kn), _p(self.tv), _p(self.mask), _p(b), _p(x), float(tol),
+                                        float(inner_tol), int(max_outer), int(maxit), _p(self._rwork), ctypes.byref(it),
+                                        ctypes.byref(nout), ctypes.byref(rr), L.stream_of(b)))
+        return x, int(it.value), int(nout.value), float(rr.value)

@@ -103,7 +103,7 @@ int level0(const lpm_grid* g, const lpm_amg* a, int op, int col, const void* in,
 }
 
 template <typename T>
-int levelq(const lpm_amg* a, int k, int op, int col, const void* in, void* out, hipStream_t st) {
+int levelq(const lpm_amg* a, int k, int op, int col, const void* in, void* out, hipStream_t st, T* tmp = nullptr) {
   const AmgLGeo geo = geoq(a, k);
   const T* Q = (const T*)a->Q[k];
   const int64_t n = nodes(a->g[k]), nc = nodes(a->g[k + 1]);
@@ -112,8 +112,13 @@ int levelq(const lpm_amg* a, int k, int op, int col, const void* in, void* out, 
       hipLaunchKernelGGL((amgq_prolong_kernel<T>), dim3(blocks_for(n)), dim3(256), 0, st, geo, Q, (const T*)in, (T*)out);
       break;
     case LPM_AMG_RESTRICT:
-      hipLaunchKernelGGL((amgq_restrict_kernel<T>), dim3(blocks_for(nc)), dim3(256), 0, st, geo, Q, (const T*)in,
-                         (T*)out);
+      if (tmp) {
+        hipLaunchKernelGGL((amgq_qtr_kernel<T>), dim3(blocks_for(n)), dim3(256), 0, st, (int)n, Q, (const T*)in, tmp);
+        hipLaunchKernelGGL((amgq_sum_kernel<T>), dim3(blocks_for(nc)), dim3(256), 0, st, geo, (const T*)tmp, (T*)out);
+      } else {
+        hipLaunchKernelGGL((amgq_restrict_kernel<T>), dim3(blocks_for(nc)), dim3(256), 0, st, geo, Q, (const T*)in,
+                           (T*)out);
+      }
       break;
     case LPM_AMG_PROBE:
       hipLaunchKernelGGL((amgq_probe_kernel<T>), dim3(blocks_for(n)), dim3(256), 0, st, geo, Q, col, (T*)out);
@@ -132,8 +137,14 @@ int levelq(const lpm_amg* a, int k, int op, int col, const void* in, void* out, 
 template <typename T>
 int st_mv(const int g[3], int nsys, const void* A, const void* x, const void* b, void* y, hipStream_t st) {
   const int64_t n = nodes(g);
-  hipLaunchKernelGGL((amg_st_mv_kernel<T>), dim3(blocks_for(n * nsys)), dim3(256), 0, st, g[0], g[1], g[2], nsys,
-                     (const T*)A, (const T*)x, (const T*)b, (T*)y);
+  if (nsys == 6 && !b) {
+    hipLaunchKernelGGL((amg_st_mv_kernel<T, 6>), dim3(blocks_for(n)), dim3(256), 0, st, g[0], g[1], g[2], (const T*)A,
+                       (const T*)x, (const T*)nullptr, (T*)y);
+  } else {
+    for (int s = 0; s < nsys; ++s)
+      hipLaunchKernelGGL((amg_st_mv_kernel<T, 1>), dim3(blocks_for(n)), dim3(256), 0, st, g[0], g[1], g[2],
+                         (const T*)A, (const T*)x + s * n * 6, (const T*)b, (T*)y + s * n * 6);
+  }
   LAUNCHCHK();
   return LPM_OK;
 }
@@ -168,6 +179,7 @@ struct Cycle {
   T* t[LPM_AMG_MAXLEV];
   T* x[LPM_AMG_MAXLEV];
   T* r[LPM_AMG_MAXLEV];
+  T* q[LPM_AMG_MAXLEV];  // restriction stage buffer (n_k, 6)
 
   int64_t N() const { return (int64_t)g->nx * g->ny * g->nz; }
 
@@ -194,7 +206,7 @@ struct Cycle {
       if ((rc = bj<T>(n, 6, a->Minv[k], r[k], w, 1, out, st))) return rc;
     }
     if ((rc = st_mv<T>(a->g[k], 1, a->A[k], out, rhs, r[k], st))) return rc;
-    if ((rc = levelq<T>(a, k, LPM_AMG_RESTRICT, 0, r[k], b[k + 1], st))) return rc;
+    if ((rc = levelq<T>(a, k, LPM_AMG_RESTRICT, 0, r[k], b[k + 1], st, q[k]))) return rc;
     if ((rc = coarse(k + 1, b[k + 1], e[k + 1]))) return rc;
     if ((rc = levelq<T>(a, k, LPM_AMG_PROLONG_ADD, 0, e[k + 1], out, st))) return rc;
     for (int it = 0; it < a->nu_c; ++it) {
@@ -245,7 +257,7 @@ struct Cycle {
 };
 
 struct WorkLayout {
-  size_t r0, rwork, lev[LPM_AMG_MAXLEV][5], pcg[4], part, total;
+  size_t r0, rwork, lev[LPM_AMG_MAXLEV][6], pcg[4], part, total;
 };
 
 template <typename T>
@@ -260,7 +272,7 @@ WorkLayout work_layout(const lpm_grid* g, const lpm_amg* a) {
   o += lpm_residual_workspace(g, 1, sizeof(T) == 4 ? LPM_F32 : LPM_F64);
   for (int k = 0; k < a->nlev; ++k) {
     const size_t v = align256(sizeof(T) * nodes(a->g[k]) * 6);
-    for (int j = 0; j < 5; ++j) {
+    for (int j = 0; j < 6; ++j) {
       L.lev[k][j] = o;
       o += v;
     }
@@ -296,6 +308,7 @@ Cycle<T> make_cycle(const lpm_grid* g, const lpm_amg* a, int layout, const void*
     c.t[k] = (T*)(w + L.lev[k][2]);
     c.x[k] = (T*)(w + L.lev[k][3]);
     c.r[k] = (T*)(w + L.lev[k][4]);
+    c.q[k] = (T*)(w + L.lev[k][5]);
   }
   return c;
 }
@@ -372,6 +385,31 @@ int pcg(const lpm_grid* g, const lpm_amg* a, int layout, const void* kn, const v
   *iters = std::min(it, maxit);
   *relres = rn / bn;
   return LPM_OK;
+}
+
+// workspace of the refined solve: the FP32 AMG-PCG workspace, then r64, r32, d32 and the FP64 residual partials
+struct RefLayout {
+  size_t inner, r64, r32, d32, rwork, part, total;
+};
+
+RefLayout ref_layout(const lpm_grid* g, const lpm_amg* a) {
+  RefLayout L{};
+  const int64_t n3 = (int64_t)g->nx * g->ny * g->nz * 3;
+  size_t o = 0;
+  L.inner = o;
+  o += work_layout<float>(g, a).total;
+  L.r64 = o;
+  o += align256(8 * n3);
+  L.r32 = o;
+  o += align256(4 * n3);
+  L.d32 = o;
+  o += align256(4 * n3);
+  L.rwork = o;
+  o += lpm_residual_workspace(g, 1, LPM_F64);
+  L.part = o;
+  o += align256(8 * VEC_BLOCKS);
+  L.total = o;
+  return L;
 }
 
 bool fine_ok(const lpm_grid* g, const lpm_amg* a) {
@@ -515,6 +553,66 @@ int lpm_amg_pcg(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t layo
   if (dtype == LPM_F32)
     return pcg<float>(g, a, layout, kn, tv, mask, (const float*)b, (float*)x, tol, maxit, work, iters, relres, st);
   return fail(LPM_E_BADARG, "lpm_amg_pcg: bad dtype %d", dtype);
+}
+
+size_t lpm_amg_refined_workspace(const lpm_grid* g, const lpm_amg* a32) {
+  if (!g || check_amg(a32) || !fine_ok(g, a32)) return 0;
+  return ref_layout(g, a32).total;
+}
+
+int lpm_amg_pcg_refined(const lpm_grid* g, const lpm_amg* a32, int32_t layout64, const void* kn64, const void* tv64,
+                        int32_t layout32, const void* kn32, const void* tv32, const uint8_t* mask, const void* b64,
+                        void* x64, double tol, double inner_tol, int32_t max_outer, int32_t maxit, void* work,
+                        int32_t* iters, int32_t* outer, double* relres, void* stream) {
+  if (int rc = check_amg(a32)) return rc;
+  if (!fine_ok(g, a32) || !kn64 || !tv64 || !kn32 || !tv32 || !mask || !b64 || !x64 || !work || !iters || !outer ||
+      !relres || maxit < 1 || max_outer < 1 || !a32->minv0 || !a32->tmpl)
+    return fail(LPM_E_BADARG, "lpm_amg_pcg_refined: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const RefLayout L = ref_layout(g, a32);
+  char* w = (char*)work;
+  double* r64 = (double*)(w + L.r64);
+  float* r32 = (float*)(w + L.r32);
+  float* d32 = (float*)(w + L.d32);
+  double* part = (double*)(w + L.part);
+  const int64_t n3 = (int64_t)g->nx * g->ny * g->nz * 3;
+  const unsigned eg = std::min<unsigned>(blocks_for(n3), 4096);
+  const lpm_bcast full{n3, n3}, none{0, 0};
+  hipLaunchKernelGGL((amg_fill_kernel<double>), dim3(eg), dim3(256), 0, st, n3, 0.0, (double*)x64);
+  LAUNCHCHK();
+  double bn2;
+  int rc;
+  if ((rc = dot<double>(n3, (const double*)b64, (const double*)b64, part, &bn2, st))) return rc;
+  const double bn = std::sqrt(bn2);
+  *iters = 0;
+  *outer = 0;
+  *relres = 0.0;
+  if (bn == 0.0) return LPM_OK;
+  for (int k = 0;; ++k) {
+    // true FP64 residual r = mask (b - K x)
+    if ((rc = lpm_residual(g, 1, 1, LPM_F64, layout64, x64, full, kn64, tv64, none, mask, full, b64, full, r64,
+                           w + L.rwork, st)))
+      return rc;
+    double rr;
+    if ((rc = dot<double>(n3, r64, r64, part, &rr, st))) return rc;
+    const double rel = std::sqrt(rr) / bn;
+    if (!std::isfinite(rel)) return fail(LPM_E_NONFINITE, "AMG refinement: non-finite residual");
+    *relres = rel;
+    *outer = k;
+    if (rel <= tol || k >= max_outer) break;
+    // FP32 AMG-PCG on the residual, to the tolerance that reaches the target in this step (never looser than 0.5)
+    const double t = std::min(0.5, std::max(inner_tol, 0.5 * tol / rel));
+    hipLaunchKernelGGL(amg_to32_kernel, dim3(eg), dim3(256), 0, st, n3, (const double*)r64, r32);
+    LAUNCHCHK();
+    int32_t it = 0;
+    double rel_in = 0.0;
+    if ((rc = pcg<float>(g, a32, layout32, kn32, tv32, mask, r32, d32, t, maxit, w + L.inner, &it, &rel_in, st)))
+      return rc;
+    *iters += it;
+    hipLaunchKernelGGL(amg_add64_kernel, dim3(eg), dim3(256), 0, st, n3, (const float*)d32, (double*)x64);
+    LAUNCHCHK();
+  }
+  return LPM_OK;
 }
 
 }  // extern "C"

@@ -190,41 +190,47 @@ __device__ __forceinline__ void node_xyz(int n, const int g[3], int c[3]) {
   c[2] = n / (g[0] * g[1]);
 }
 
-// y = A x (sub = 0) or y = b - A x (sub = 1) on a 27-point 6x6 stencil (amg3d.py:230-253); nsys systems of x / y
-// sharing A (probing), b used by system 0 only.  One thread per (node, system).
-template <typename T>
-__global__ __launch_bounds__(256) void amg_st_mv_kernel(int gx, int gy, int gz, int nsys, const T* __restrict__ A,
+// y_s = A x_s (b null) or y_s = b - A x_s on a 27-point 6x6 stencil (amg3d.py:230-253) for NS systems sharing A
+// (probing); b is used with NS = 1 only.  One thread per node: each block of A is read once for all systems.
+template <typename T, int NS>
+__global__ __launch_bounds__(256) void amg_st_mv_kernel(int gx, int gy, int gz, const T* __restrict__ A,
                                                         const T* __restrict__ x, const T* __restrict__ b,
                                                         T* __restrict__ y) {
   const int n = gx * gy * gz;
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= (int64_t)n * nsys) return;
-  const int I = (int)(t % n), s = (int)(t / n);
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
   const int g[3] = {gx, gy, gz};
   int c[3];
   node_xyz(I, g, c);
-  const T* xs = x + (int64_t)s * n * 6;
-  T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  T acc[NS][6];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int r = 0; r < 6; ++r) acc[s][r] = T(0);
   for (int d = 0; d < 27; ++d) {
     const int j0 = c[0] + d % 3 - 1, j1 = c[1] + (d / 3) % 3 - 1, j2 = c[2] + d / 9 - 1;
     if (j0 < 0 || j0 >= gx || j1 < 0 || j1 >= gy || j2 < 0 || j2 >= gz) continue;
     const int J = (j2 * gy + j1) * gx + j0;
-    T xv[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) xv[k] = xs[(int64_t)J * 6 + k];
+    T ab[36];
     const T* Ad = A + (int64_t)d * 36 * n + I;
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
+    for (int q = 0; q < 36; ++q) ab[q] = Ad[(int64_t)q * n];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) acc[r] += Ad[(int64_t)(r * 6 + k) * n] * xv[k];
+    for (int s = 0; s < NS; ++s) {
+      T xv[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) xv[k] = x[((int64_t)s * n + J) * 6 + k];
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[s][r] += ab[r * 6 + k] * xv[k];
+    }
   }
-  T* ys = y + (int64_t)s * n * 6 + (int64_t)I * 6;
-  if (b) {
 #pragma unroll
-    for (int r = 0; r < 6; ++r) ys[r] = b[(int64_t)I * 6 + r] - acc[r];
-  } else {
+  for (int s = 0; s < NS; ++s) {
+    T* ys = y + ((int64_t)s * n + I) * 6;
 #pragma unroll
-    for (int r = 0; r < 6; ++r) ys[r] = acc[r];
+    for (int r = 0; r < 6; ++r) ys[r] = (NS == 1 && b) ? b[(int64_t)I * 6 + r] - acc[s][r] : acc[s][r];
   }
 }
 
@@ -259,6 +265,45 @@ __global__ __launch_bounds__(256) void amgq_restrict_kernel(AmgLGeo l, const T* 
 #pragma unroll
           for (int c = 0; c < 6; ++c) acc[c] += q[a * 6 + c] * rv;
         }
+      }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) rc[(int64_t)K * 6 + c] = acc[c];
+}
+
+// first stage of the node-level restriction: tmp_n = Q_n^T r_n, one thread per node (coalesced)
+template <typename T>
+__global__ __launch_bounds__(256) void amgq_qtr_kernel(int n, const T* __restrict__ Q, const T* __restrict__ r,
+                                                       T* __restrict__ tmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T* q = Q + (int64_t)i * 36;
+  T rv[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) rv[a] = r[(int64_t)i * 6 + a];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    T s = T(0);
+#pragma unroll
+    for (int a = 0; a < 6; ++a) s += q[a * 6 + c] * rv[a];
+    tmp[(int64_t)i * 6 + c] = s;
+  }
+}
+
+// second stage: rc_K = sum of tmp over the members of K in a fixed order
+template <typename T>
+__global__ __launch_bounds__(256) void amgq_sum_kernel(AmgLGeo l, const T* __restrict__ tmp, T* __restrict__ rc) {
+  const int nc = l.gc[0] * l.gc[1] * l.gc[2];
+  const int K = blockIdx.x * blockDim.x + threadIdx.x;
+  if (K >= nc) return;
+  int s0[3], len[3], kc[3];
+  agg_box(l, K, s0, len, kc);
+  T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int z = 0; z < len[2]; ++z)
+    for (int y = 0; y < len[1]; ++y)
+      for (int x = 0; x < len[0]; ++x) {
+        const int64_t m = ((int64_t)(s0[2] + z) * l.g[1] + (s0[1] + y)) * l.g[0] + (s0[0] + x);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] += tmp[m * 6 + c];
       }
 #pragma unroll
   for (int c = 0; c < 6; ++c) rc[(int64_t)K * 6 + c] = acc[c];
@@ -550,6 +595,16 @@ template <typename T>
 __global__ __launch_bounds__(256) void amg_add_kernel(int64_t n, const T* __restrict__ x, T* __restrict__ y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] += x[i];
+}
+
+// refinement: r32 = (float) r64, and x64 += (double) d32
+__global__ __launch_bounds__(256) void amg_to32_kernel(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+__global__ __launch_bounds__(256) void amg_add64_kernel(int64_t n, const float* __restrict__ d, double* __restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] += (double)d[i];
 }
 
 template <typename T>

@@ -128,3 +128,22 @@ def test_amg_pcg_zero_rhs_and_single_level():
     xs = np.zeros(3 * lat.N)
     xs[fd] = spla.splu(sp.csc_matrix(K[fd][:, fd])).solve(b.reshape(-1)[fd])
     assert rr <= 1e-10 and np.abs(x.cpu().numpy().reshape(-1) - xs).max() <= 1e-8 * np.abs(xs).max()
+
+
+@pytest.mark.gpu
+def test_refined_fp32_amg_reaches_fp64_accuracy():
+    """FP32 AMG-PCG inner solves inside the FP64 refinement reach rel-res 1e-8 on the true FP64 residual, and the
+    displacement agrees with SuperLU within 1e-6."""
+    dims = (21, 20, 19)
+    lat, Kn, Tv, free, b = _problem(dims, seed=5, load="tb_tz")
+    geo = amg3d.AMGGeom(_lat(dims), free, agg=3, coarse_max=100)
+    amg32 = amg3d.AMG3D(geo, Kn, Tv, gamma=2, power_its=5, dtype=torch.float32)
+    x, it, nout, rr = amg32.pcg_refined(torch.as_tensor(b, device="cuda"), Kn, Tv, tol=1e-8, inner_tol=1e-4)
+    assert rr <= 1e-8 and nout >= 1
+    K = R.stiffness(lat, Kn, Tv)
+    fd = np.repeat(free, 3)
+    xs = np.zeros(3 * lat.N)
+    xs[fd] = spla.splu(sp.csc_matrix(K[fd][:, fd])).solve(b.reshape(-1)[fd])
+    assert np.abs(x.cpu().numpy().reshape(-1) - xs).max() <= 1e-6 * np.abs(xs).max()
+    r = b.reshape(-1) - K @ x.cpu().numpy().reshape(-1)
+    assert np.linalg.norm(r[fd]) <= 1.01e-8 * np.linalg.norm(b)
