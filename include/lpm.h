@@ -188,8 +188,9 @@ size_t lpm_serve_arena_free(void* server);
  * prolongator block of a particle is read from the shape templates tmpl + toff[s] ((sx sy sz, 3, 6) row-major, s =
  * bit a set for the last chunk along axis a), the Gram-Schmidt basis of gmg._tentative (timing/t3d/gmg.py:70-143),
  * which depends on the aggregate's shape only.  Stencil level k (k = 0 is the Galerkin operator of level 0) has
- * the node grid g[k], the 27-point 6x6 stencil A[k] stored structure-of-arrays A[(d 36 + r 6 + c) n + I] with
- * d = (dz+1) 9 + (dy+1) 3 + (dx+1), and for k < nlev - 1 the per-node blocks Q[k] (n, 6, 6) towards level k + 1,
+ * the node grid g[k], the symmetric 27-point 6x6 stencil A[k] (offset d = (dz+1) 9 + (dy+1) 3 + (dx+1)) of which the
+ * blocks d = 13 ... 26 are stored structure-of-arrays A[((d - 13) 36 + r 6 + c) n + I] (a block d < 13 is the
+ * transpose of the neighbour's block 26 - d), and for k < nlev - 1 the per-node blocks Q[k] (n, 6, 6) towards level k + 1,
  * the inverses Minv[k] (SoA (36, n)) of its diagonal blocks and the damping w[k] = omega / rho_k.  dinv is the
  * dense inverse of the coarsest stencil.  All pointers are device memory of the call's dtype. */
 #define LPM_AMG_MAXLEV 12
@@ -213,7 +214,7 @@ enum { LPM_AMG_PROLONG_ADD = 0, LPM_AMG_RESTRICT = 1, LPM_AMG_PROBE = 2, LPM_AMG
 
 /* Level-0 transfer: PROLONG_ADD out (N,3) += P in (nagg,6) (amg3d.py:318-331); RESTRICT out (nagg,6) = P^T in (N,3)
  * (amg3d.py:303-315); PROBE out (6,N,3) = columns of P on the aggregates of colour col (0..26, (I mod 3) per axis);
- * GATHER: column j of out[I, d(I, col)] (stencil A[0] layout) = P_I^T in_j for in (6,N,3) = K PROBE: with
+ * GATHER: column j of out[I, d(I, col)] (stencil A[0] layout, stored blocks only) = P_I^T in_j for in (6,N,3) = K PROBE: with
  * lpm_apply this replaces the bond-level Galerkin product _galerkin0 (amg3d.py:151-196). */
 int lpm_amg_level0(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t op, int32_t col, const void* in,
                    void* out, void* stream);

@@ -26,6 +26,7 @@ import _lpmlib as L
 import lpm_ops as ops
 
 MAXLEV = 12
+ST_BLOCKS = 14          # stored stencil blocks per node: the diagonal and the 13 forward offsets (d = 13 ... 26)
 SCRATCH = 4096
 PROLONG_ADD, RESTRICT, PROBE, GATHER = 0, 1, 2, 3
 AMG_CFG = dict(agg=3, coarse_max=1000, nu=1, gamma=2, power_its=5)
@@ -344,7 +345,7 @@ class AMG3D:
         s.w0 = self.omega / self.rho0
         # Galerkin operator of level 0 by probing (replaces _galerkin0, amg3d.py:151-196)
         nagg = int(np.prod(geom.g0))
-        A = torch.zeros((27 * 36, nagg), device=dev, dtype=dtype)
+        A = torch.zeros((ST_BLOCKS * 36, nagg), device=dev, dtype=dtype)
         v = torch.empty((6, N, 3), device=dev, dtype=dtype)
         bc0 = L.Bcast(0, 0)
         for col in range(27):
@@ -371,7 +372,7 @@ class AMG3D:
             s.w[k] = self.omega / r_k
             gc = Lg["gc"]
             nc_ = int(np.prod(gc))
-            Ac = torch.zeros((27 * 36, nc_), device=dev, dtype=dtype)
+            Ac = torch.zeros((ST_BLOCKS * 36, nc_), device=dev, dtype=dtype)
             vq = torch.empty((6, n, 6), device=dev, dtype=dtype)
             wq = torch.empty_like(vq)
             for col in range(27):
@@ -426,9 +427,22 @@ class AMG3D:
         return y
 
     def stencil(self, k):
-        """Galerkin stencil of level k + 1 as (n, 27, 6, 6) (the reference's layout)."""
-        A = self.A[k]
-        return A.reshape(27, 6, 6, -1).permute(3, 0, 1, 2).contiguous()
+        """Galerkin stencil of level k + 1 as (n, 27, 6, 6) (the reference's layout), the blocks d < 13 expanded
+        from the stored transposes."""
+        g = self.geom.grids()[k]
+        n = int(np.prod(g))
+        S = self.A[k].reshape(ST_BLOCKS, 6, 6, n).permute(3, 0, 1, 2).cpu()
+        full = torch.zeros((n, 27, 6, 6), dtype=S.dtype)
+        full[:, 13:] = S
+        ij = np.stack(np.meshgrid(np.arange(g[2]), np.arange(g[1]), np.arange(g[0]), indexing="ij"), -1)
+        ij = ij.reshape(-1, 3)[:, ::-1]
+        for d in range(13):
+            off = np.array([d % 3 - 1, (d // 3) % 3 - 1, d // 9 - 1])
+            j = ij + off
+            ok = np.all((j >= 0) & (j < np.array(g)), 1)
+            J = (j[:, 2] * g[1] + j[:, 1]) * g[0] + j[:, 0]
+            full[torch.as_tensor(np.nonzero(ok)[0]), d] = S[torch.as_tensor(J[ok]), 13 - d].transpose(-1, -2)
+        return full
 
     def vcycle(self, b):
         """z = B b for b (N, 3) zero on fixed particles (amg3d.py:620-646)."""

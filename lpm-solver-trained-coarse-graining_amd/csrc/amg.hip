@@ -1,7 +1,7 @@
 // C ABI of the aggregation AMG preconditioner (declarations and reference citations in include/lpm.h): level
 // transfers, probing assembly of the Galerkin stencils, smoothers, the coarsest dense inverse, and the V/W-cycle and
-// AMG-PCG drivers (timing/t3d/amg3d.py:508-672).  The fine-level residuals go through lpm_residual (the fused K u
-// kernel with the residual epilogue).
+// AMG-PCG drivers (timing/t3d/amg3d.py:508-672).  Fine-level residuals inside the cycle are a plain K u followed by
+// one elementwise pass that also applies the smoother; the FP64 residual of the refinement loop is lpm_residual.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -183,10 +183,31 @@ struct Cycle {
 
   int64_t N() const { return (int64_t)g->nx * g->ny * g->nz; }
 
-  int resid0(const T* rhs, const T* u, T* out) {
+  // plain K u into out
+  int ku0(const T* u, T* out) {
     const int64_t n3 = N() * 3;
-    return lpm_residual(g, 1, 1, sizeof(T) == 4 ? LPM_F32 : LPM_F64, layout, u, lpm_bcast{n3, n3}, kn, tv,
-                        lpm_bcast{0, 0}, mask, lpm_bcast{n3, n3}, rhs, lpm_bcast{n3, n3}, out, rwork, st);
+    return lpm_apply(g, 1, 1, sizeof(T) == 4 ? LPM_F32 : LPM_F64, layout, u, lpm_bcast{n3, n3}, kn, tv,
+                     lpm_bcast{0, 0}, out, st);
+  }
+
+  // out = mask (rhs - K u)
+  int resid0(const T* rhs, const T* u, T* out) {
+    if (int rc = ku0(u, out)) return rc;
+    const int64_t n3 = N() * 3;
+    hipLaunchKernelGGL((amg_resid_kernel<T>), dim3(std::min<unsigned>(blocks_for(n3), 8192)), dim3(256), 0, st, n3,
+                       rhs, mask, out);
+    LAUNCHCHK();
+    return LPM_OK;
+  }
+
+  // z += w M mask (rhs - K z), with r0 as the K z buffer
+  int smooth0(const T* rhs, T* z, double w) {
+    if (int rc = ku0(z, r0)) return rc;
+    const int64_t N_ = N();
+    hipLaunchKernelGGL((amg_resid_bj3_kernel<T>), dim3(blocks_for(N_)), dim3(256), 0, st, N_, (const T*)a->minv0, rhs,
+                       mask, (const T*)r0, (T)w, z);
+    LAUNCHCHK();
+    return LPM_OK;
   }
 
   // out = approximate solution of A_k out = rhs (one cycle from zero); amg3d.py:590-618
@@ -240,18 +261,14 @@ struct Cycle {
     const double w0 = a->w0;
     int rc;
     if ((rc = bj<T>(N_, 3, a->minv0, bb, w0, 0, z, st))) return rc;
-    for (int it = 0; it < a->nu - 1; ++it) {
-      if ((rc = resid0(bb, z, r0))) return rc;
-      if ((rc = bj<T>(N_, 3, a->minv0, r0, w0, 1, z, st))) return rc;
-    }
+    for (int it = 0; it < a->nu - 1; ++it)
+      if ((rc = smooth0(bb, z, w0))) return rc;
     if ((rc = resid0(bb, z, r0))) return rc;
     if ((rc = level0<T>(g, a, LPM_AMG_RESTRICT, 0, r0, b[0], st))) return rc;
     if ((rc = coarse(0, b[0], e[0]))) return rc;
     if ((rc = level0<T>(g, a, LPM_AMG_PROLONG_ADD, 0, e[0], z, st))) return rc;
-    for (int it = 0; it < a->nu; ++it) {
-      if ((rc = resid0(bb, z, r0))) return rc;
-      if ((rc = bj<T>(N_, 3, a->minv0, r0, w0, 1, z, st))) return rc;
-    }
+    for (int it = 0; it < a->nu; ++it)
+      if ((rc = smooth0(bb, z, w0))) return rc;
     return LPM_OK;
   }
 };

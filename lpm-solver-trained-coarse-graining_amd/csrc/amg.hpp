@@ -8,9 +8,11 @@
 // at most 8 shapes) and the particle's place in it: the kernels read it from shape templates.
 //
 // Levels >= 1 are node grids with 6 unknowns per node and a 27-point stencil of 6x6 blocks, offset
-// d = (dz+1) 9 + (dy+1) 3 + (dx+1) (amg3d.py:39).  Stencils are stored structure-of-arrays,
-// A[(d 36 + r 6 + c) n + I], so consecutive nodes read consecutive addresses.  Nodes are aggregated like the
-// particles, with per-node blocks Q (n, 6, 6) row-major.
+// d = (dz+1) 9 + (dy+1) 3 + (dx+1) (amg3d.py:39).  Galerkin operators are symmetric, A[I, d] = A[I + off(d), 26 - d]^T,
+// so only the blocks d = 13 ... 26 (the diagonal and the 13 forward offsets) are stored, e = d - 13, structure-of-
+// arrays A[(e 36 + r 6 + c) n + I] so that consecutive nodes read consecutive addresses; a block with d < 13 is read
+// transposed from the neighbour's e = 13 - d.  Nodes are aggregated like the particles, with per-node blocks
+// Q (n, 6, 6) row-major.
 //
 // Galerkin operators are assembled by probing: with aggregates coloured by (I mod 3) per axis, two aggregates of
 // one colour are at least 3 chunks apart, while the operator couples only neighbouring aggregates (the particle
@@ -28,6 +30,8 @@ struct Amg0Geo {
   int agg;
   int64_t toff[8];  // template of shape s (bit a = last chunk along axis a): (sx sy sz, 3, 6) at tmpl + toff[s]
 };
+
+constexpr int ST_BLOCKS = 14;  // stored stencil blocks per node (d = 13 ... 26)
 
 struct AmgLGeo {
   int g[3];   // node grid of this level
@@ -157,6 +161,7 @@ __global__ __launch_bounds__(256) void amg0_gather_kernel(Amg0Geo a, const T* __
     if (jc[k] < 0) return;
   }
   const int d = (jc[2] - cc[2] + 1) * 9 + (jc[1] - cc[1] + 1) * 3 + (jc[0] - cc[0] + 1);
+  if (d < 13) return;  // stored as the transpose of the partner's block
   int s0[3], len[3], sh = 0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(256) void amg0_gather_kernel(Amg0Geo a, const T* __
           for (int r = 0; r < 6; ++r) acc[r] += pr[r] * w0 + pr[6 + r] * w1 + pr[12 + r] * w2;
         }
 #pragma unroll
-    for (int r = 0; r < 6; ++r) A[((int64_t)d * 36 + r * 6 + j) * nagg + I] = acc[r];
+    for (int r = 0; r < 6; ++r) A[((int64_t)(d - 13) * 36 + r * 6 + j) * nagg + I] = acc[r];
   }
 }
 
@@ -191,7 +196,8 @@ __device__ __forceinline__ void node_xyz(int n, const int g[3], int c[3]) {
 }
 
 // y_s = A x_s (b null) or y_s = b - A x_s on a 27-point 6x6 stencil (amg3d.py:230-253) for NS systems sharing A
-// (probing); b is used with NS = 1 only.  One thread per node: each block of A is read once for all systems.
+// (probing); b is used with NS = 1 only.  One thread per node: each stored block is read once for all systems,
+// the forward blocks of the node and, transposed, the backward ones from its neighbours.
 template <typename T, int NS>
 __global__ __launch_bounds__(256) void amg_st_mv_kernel(int gx, int gy, int gz, const T* __restrict__ A,
                                                         const T* __restrict__ x, const T* __restrict__ b,
@@ -207,14 +213,28 @@ __global__ __launch_bounds__(256) void amg_st_mv_kernel(int gx, int gy, int gz, 
   for (int s = 0; s < NS; ++s)
 #pragma unroll
     for (int r = 0; r < 6; ++r) acc[s][r] = T(0);
-  for (int d = 0; d < 27; ++d) {
-    const int j0 = c[0] + d % 3 - 1, j1 = c[1] + (d / 3) % 3 - 1, j2 = c[2] + d / 9 - 1;
+  for (int q = 0; q < 27; ++q) {
+    const int j0 = c[0] + q % 3 - 1, j1 = c[1] + (q / 3) % 3 - 1, j2 = c[2] + q / 9 - 1;
     if (j0 < 0 || j0 >= gx || j1 < 0 || j1 >= gy || j2 < 0 || j2 >= gz) continue;
     const int J = (j2 * gy + j1) * gx + j0;
+    // block (I, J): stored at I (e = q - 13) for q >= 13, else the transpose of J's block e = 13 - q
+    const bool fwd = q >= 13;
+    const int e = fwd ? q - 13 : 13 - q;
+    const int owner = fwd ? I : J;
     T ab[36];
-    const T* Ad = A + (int64_t)d * 36 * n + I;
+    const T* Ad = A + (int64_t)e * 36 * n + owner;
 #pragma unroll
-    for (int q = 0; q < 36; ++q) ab[q] = Ad[(int64_t)q * n];
+    for (int k = 0; k < 36; ++k) ab[k] = Ad[(int64_t)k * n];
+    if (!fwd) {
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int k = r + 1; k < 6; ++k) {
+          const T t = ab[r * 6 + k];
+          ab[r * 6 + k] = ab[k * 6 + r];
+          ab[k * 6 + r] = t;
+        }
+    }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       T xv[6];
@@ -369,6 +389,7 @@ __global__ __launch_bounds__(256) void amgq_gather_kernel(AmgLGeo l, const T* __
     if (jc[k] < 0) return;
   }
   const int d = (jc[2] - kc[2] + 1) * 9 + (jc[1] - kc[1] + 1) * 3 + (jc[0] - kc[0] + 1);
+  if (d < 13) return;  // stored as the transpose of the partner's block
   for (int j = 0; j < 6; ++j) {
     const T* wj = w + (int64_t)j * n * 6;
     T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
@@ -385,7 +406,7 @@ __global__ __launch_bounds__(256) void amgq_gather_kernel(AmgLGeo l, const T* __
           }
         }
 #pragma unroll
-    for (int r = 0; r < 6; ++r) Ac[((int64_t)d * 36 + r * 6 + j) * nc + K] = acc[r];
+    for (int r = 0; r < 6; ++r) Ac[((int64_t)(d - 13) * 36 + r * 6 + j) * nc + K] = acc[r];
   }
 }
 
@@ -401,7 +422,7 @@ __global__ __launch_bounds__(256) void amg_inv_diag_kernel(int n, const T* __res
   for (int r = 0; r < 6; ++r)
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-      m[r][c] = (double)A[((int64_t)13 * 36 + r * 6 + c) * n + I];
+      m[r][c] = (double)A[((int64_t)0 * 36 + r * 6 + c) * n + I];  // e = 0: the diagonal block
       m[r][6 + c] = r == c ? 1.0 : 0.0;
     }
   for (int k = 0; k < 6; ++k) {
@@ -473,7 +494,32 @@ __global__ __launch_bounds__(256) void amg_bj3_kernel(int64_t N, const T* __rest
   }
 }
 
-// dense (6n)^2 matrix of the coarsest stencil, symmetrised: D = (S + S^T) / 2 (amg3d.py:418-432, 544-546)
+// fine-level residual from a plain K z product kz: r = mask (b - kz), in place over kz (the masked residual of
+// _kx_free, amg3d.py:199-229)
+template <typename T>
+__global__ __launch_bounds__(256) void amg_resid_kernel(int64_t n, const T* __restrict__ b,
+                                                        const uint8_t* __restrict__ mask, T* __restrict__ kz) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    kz[i] = mask[i] ? b[i] - kz[i] : T(0);
+}
+
+// one fine-level smoothing sweep from a plain K z product: z += w M mask (b - kz) (amg3d.py:640-645)
+template <typename T>
+__global__ __launch_bounds__(256) void amg_resid_bj3_kernel(int64_t N, const T* __restrict__ M, const T* __restrict__ b,
+                                                            const uint8_t* __restrict__ mask, const T* __restrict__ kz,
+                                                            T w, T* __restrict__ z) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const T r0 = mask[p * 3] ? b[p * 3] - kz[p * 3] : T(0);
+  const T r1 = mask[p * 3 + 1] ? b[p * 3 + 1] - kz[p * 3 + 1] : T(0);
+  const T r2 = mask[p * 3 + 2] ? b[p * 3 + 2] - kz[p * 3 + 2] : T(0);
+  const T* m = M + p * 6;
+  z[p * 3] += w * (m[0] * r0 + m[5] * r1 + m[4] * r2);
+  z[p * 3 + 1] += w * (m[5] * r0 + m[1] * r1 + m[3] * r2);
+  z[p * 3 + 2] += w * (m[4] * r0 + m[3] * r1 + m[2] * r2);
+}
+
+// dense (6n)^2 matrix of the coarsest stencil (symmetric by its storage; amg3d.py:418-432, 544-546)
 template <typename T>
 __global__ __launch_bounds__(256) void amg_dense_kernel(int gx, int gy, int gz, const T* __restrict__ A,
                                                         T* __restrict__ D) {
@@ -489,8 +535,8 @@ __global__ __launch_bounds__(256) void amg_dense_kernel(int gx, int gy, int gz, 
   const int dx = cj[0] - ci[0], dy = cj[1] - ci[1], dz = cj[2] - ci[2];
   T v = T(0);
   if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
-    const int d = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1), dt = 26 - d;
-    v = T(0.5) * (A[((int64_t)d * 36 + r * 6 + c) * n + I] + A[((int64_t)dt * 36 + c * 6 + r) * n + J]);
+    const int d = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+    v = d >= 13 ? A[((int64_t)(d - 13) * 36 + r * 6 + c) * n + I] : A[((int64_t)(13 - d) * 36 + c * 6 + r) * n + J];
   }
   D[t] = v;
 }
