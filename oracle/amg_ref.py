@@ -332,3 +332,204 @@ class AMG3D:
             p = z + (rz2 / rz) * p
             rz = rz2
         return x, it, rn / bn
+
+
+# ------------------------------------------------------------------------------------------------ large lattices
+def _mgs_batched(B, tol=1e-8):
+    """modified Gram-Schmidt with rank detection (gmg.py:101-130) of a batch B (G, rows, 6) of equal-size blocks."""
+    G, n, m = B.shape
+    Q = np.zeros_like(B)
+    Rf = np.zeros((G, m, m))
+    keep = np.zeros((G, m), bool)
+    scale = np.sqrt((B ** 2).sum(1)).max(1)
+    for c in range(m):
+        v = B[:, :, c].copy()
+        for d in range(c):
+            pr = np.where(keep[:, d], np.einsum("gi,gi->g", Q[:, :, d], v), 0.0)
+            Rf[:, d, c] = pr
+            v -= pr[:, None] * Q[:, :, d]
+        nrm = np.sqrt((v * v).sum(1))
+        ok = (nrm > tol * scale) & (nrm > 0.0)
+        keep[:, c] = ok
+        Rf[:, c, c] = np.where(ok, nrm, 0.0)
+        Q[:, :, c] = np.where(ok[:, None], v / np.where(ok, nrm, 1.0)[:, None], 0.0)
+    assert keep.all(), "rank-deficient aggregate"
+    return Q, Rf
+
+
+def box_hierarchy(dims, lo, L, h=1.0, agg=3, coarse_max=1000, max_levels=8):
+    """AMGGeom of a free box (lo, L) on a full dims lattice without per-aggregate loops, for the CPU baseline on
+    large lattices: level-0 blocks from the Gram-Schmidt of one aggregate per shape (at most 8 shapes; the blocks
+    depend on the shape only, pinned against AMGGeom in tests), the coarse near-null space of every aggregate as the
+    shape's R factor times the centroid shift, and the node levels by batched Gram-Schmidt of equal-size aggregates.
+    Returns dict(g0, tmpl (concatenated (s, 3, 6) blocks, x fastest), toff (8), levels [dict(g, gc, Q (n, 6, 6))])."""
+    dims, lo, L = np.array(dims), np.array(lo), np.array(L)
+    g = [max(1, int(L[d]) // agg) for d in range(3)]
+    lens = [(agg if g[d] > 1 else int(L[d]), int(L[d]) - agg * (g[d] - 1)) for d in range(3)]
+    tm, toff, Rs, off = [], [0] * 8, [], 0
+    for s in range(8):
+        shape = [lens[d][(s >> d) & 1] for d in range(3)]
+        k, j, i = np.meshgrid(np.arange(shape[2]), np.arange(shape[1]), np.arange(shape[0]), indexing="ij")
+        X = np.stack([i.ravel(), j.ravel(), k.ravel()], 1).astype(float) * h
+        Q, Rf = _mgs_batched(rbm(X)[None])
+        tm.append(Q[0].reshape(-1))
+        toff[s] = off
+        off += Q[0].size
+        Rs.append(Rf[0])
+    gi = grid_ijk(tuple(g))
+    last = gi == np.array(g) - 1
+    sid = last[:, 0] * 1 + last[:, 1] * 2 + last[:, 2] * 4
+    size = np.where(last, np.array([lens[d][1] for d in range(3)]), np.array([lens[d][0] for d in range(3)]))
+    D = (lo + agg * gi + 0.5 * (size - 1)) * h - 0.5 * (dims - 1) * h
+    T = np.tile(np.eye(6), (len(gi), 1, 1))
+    T[:, 0, 3], T[:, 1, 3] = -D[:, 1], D[:, 0]
+    T[:, 1, 4], T[:, 2, 4] = -D[:, 2], D[:, 1]
+    T[:, 0, 5], T[:, 2, 5] = D[:, 2], -D[:, 0]
+    Bn = np.einsum("gij,gjk->gik", np.stack(Rs)[sid], T)
+    levels, nagg = [], int(np.prod(g))
+    g = tuple(g)
+    while 6 * nagg > coarse_max and len(levels) < max_levels:
+        chn = [chunk_index(g[d], agg) for d in range(3)]
+        gc = tuple(int(c[1]) for c in chn)
+        ni = grid_ijk(g)
+        a_of = (chn[2][0][ni[:, 2]] * gc[1] + chn[1][0][ni[:, 1]]) * gc[0] + chn[0][0][ni[:, 0]]
+        nc_ = int(np.prod(gc))
+        Qn = np.zeros((nagg, 6, 6))
+        Bc2 = np.zeros((nc_, 6, 6))
+        order = np.argsort(a_of, kind="stable")
+        counts = np.bincount(a_of, minlength=nc_)
+        starts = np.concatenate([[0], np.cumsum(counts)])
+        for cnt in np.unique(counts):
+            ks = np.nonzero(counts == cnt)[0]
+            mem = np.stack([order[starts[k_]:starts[k_] + cnt] for k_ in ks])
+            Qg, Rg = _mgs_batched(Bn[mem].reshape(len(ks), cnt * 6, 6))
+            Qn[mem.reshape(-1)] = Qg.reshape(-1, 6, 6)
+            Bc2[ks] = Rg
+        levels.append(dict(g=g, gc=gc, Q=Qn))
+        g, nagg, Bn = gc, nc_, Bc2
+    return dict(g0=tuple(int(v) for v in (L // agg).clip(min=1)), tmpl=np.concatenate(tm), toff=toff,
+                levels=levels, Bc0=None)
+
+
+# ------------------------------------------------------------------------------------------------ C restatement
+class CAmg:
+    """The C/OpenMP restatement of the AMG-PCG (oracle/lpm_oracle_amg.c) on a full box lattice with a free box:
+    hierarchy geometry from box_hierarchy, numeric setup (rho by power iteration, level-1 stencil by probing with the
+    C K u, coarse stencils by the direct Galerkin product, 6x6 inverses, dense coarsest inverse) and the PCG.
+    nbr (N, 18) int64 (-1 = no bond), Kn / Tv (N, 18) directed, M0 (N, 3, 3) block-Jacobi inverse (zero on fixed
+    particles), free (N,) bool.  starts: {level: start vector} of the power iterations (default_rng(0) normals)."""
+
+    def __init__(self, lib, dims, lo, L, nbr, Kn, Tv, M0, free, h=1.0, agg=3, coarse_max=1000, nu=1, gamma=2,
+                 power_its=5, omega=1.32, w_from=1, starts=None):
+        import ctypes
+        self.lib, self.ct = lib, ctypes
+        P, I64, I32, D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+        lib.orc_apply3.argtypes = [I64, P, P, P, P, P, P]
+        for f in ("orc_amg0_restrict", "orc_amg0_prolong"):
+            getattr(lib, f).argtypes = [P, P, P]
+        lib.orc_amg0_probe.argtypes = [P, ctypes.c_int, P]
+        lib.orc_amg0_gather.argtypes = [P, ctypes.c_int, P, P]
+        lib.orc_q_galerkin.argtypes = [P, P, ctypes.c_int, P, P, P]
+        lib.orc_inv6.argtypes = [I64, P, P]
+        lib.orc_inv6.restype = ctypes.c_int
+        lib.orc_power_rho0.argtypes = [I64, P, P, P, P, P, P, ctypes.c_int]
+        lib.orc_power_rho0.restype = D
+        lib.orc_power_rho_st.argtypes = [P, P, P, P, ctypes.c_int]
+        lib.orc_power_rho_st.restype = D
+        lib.orc_amg_pcg.argtypes = [P, I64, P, P, P, P, P, P, D, ctypes.c_int, P]
+        lib.orc_amg_pcg.restype = ctypes.c_int
+        ptr = lambda a: a.ctypes.data_as(P)
+        self.ptr = ptr
+        starts = starts or {}
+        self.N = int(np.prod(dims))
+        self.keep = [nbr, Kn, Tv, M0]
+        self.nbr, self.Kn, self.Tv = nbr, Kn, Tv
+        self.free = np.ascontiguousarray(np.asarray(free, np.uint8))
+        hb = box_hierarchy(dims, lo, L, h, agg, coarse_max)
+        self.hb = hb
+        self.tmpl = np.ascontiguousarray(hb["tmpl"])
+
+        class A0(ctypes.Structure):
+            _fields_ = [("nx", I64), ("ny", I64), ("nz", I64), ("lo", I32 * 3), ("L", I32 * 3), ("g0", I32 * 3),
+                        ("agg", I32), ("tmpl", P), ("toff", I64 * 8)]
+
+        class AM(ctypes.Structure):
+            _fields_ = [("l0", A0), ("M0", P), ("w0", D), ("nlev", I32), ("nu", I32), ("nu_c", I32),
+                        ("gamma", I32), ("w_from", I32), ("pad_", I32), ("g", (I32 * 3) * 12), ("A", P * 12),
+                        ("Q", P * 12), ("Minv", P * 12), ("w", D * 12), ("dinv", P)]
+
+        s = AM()
+        s.l0.nx, s.l0.ny, s.l0.nz = dims
+        for k in range(3):
+            s.l0.lo[k], s.l0.L[k], s.l0.g0[k] = int(lo[k]), int(L[k]), int(hb["g0"][k])
+        s.l0.agg = agg
+        s.l0.tmpl = ptr(self.tmpl).value
+        for k in range(8):
+            s.l0.toff[k] = int(hb["toff"][k])
+        self.M0 = np.ascontiguousarray(M0, dtype=np.float64)
+        s.M0 = ptr(self.M0).value
+        s.nu, s.nu_c, s.gamma, s.w_from = nu, nu, gamma, w_from
+        grids = [tuple(hb["g0"])] + [Lg["gc"] for Lg in hb["levels"]]
+        s.nlev = len(grids)
+        for k, g in enumerate(grids):
+            for d in range(3):
+                s.g[k][d] = int(g[d])
+        N = self.N
+        st0 = starts.get(0)
+        if st0 is None:
+            st0 = np.random.default_rng(0).standard_normal((N, 3))
+        st0 = np.ascontiguousarray(st0, dtype=np.float64)
+        self.rho0 = lib.orc_power_rho0(N, ptr(nbr), ptr(Kn), ptr(Tv), ptr(self.M0), ptr(self.free), ptr(st0),
+                                       power_its)
+        s.w0 = omega / self.rho0
+        # level-1 stencil by probing
+        g0 = grids[0]
+        nagg = int(np.prod(g0))
+        A = np.zeros((nagg, 27, 6, 6))
+        v = np.empty((6, N, 3))
+        kv = np.empty((6, N, 3))
+        S = np.empty((N, 4))
+        for col in range(27):
+            lib.orc_amg0_probe(ctypes.byref(s.l0), col, ptr(v))
+            for j in range(6):
+                lib.orc_apply3(N, ptr(nbr), ptr(v[j]), ptr(Kn), ptr(Tv), ptr(kv[j]), ptr(S))
+            lib.orc_amg0_gather(ctypes.byref(s.l0), col, ptr(kv), ptr(A))
+        del v, kv, S
+        self.A, self.Minv, self.rho = [A], [], []
+        for k, Lg in enumerate(hb["levels"]):
+            g = np.array(Lg["g"], np.int32)
+            gc = np.array(Lg["gc"], np.int32)
+            n = int(np.prod(g))
+            Mi = np.empty((n, 6, 6))
+            assert lib.orc_inv6(n, ptr(A), ptr(Mi)) == 0
+            stk = starts.get(k + 1)
+            if stk is None:
+                stk = np.random.default_rng(0).standard_normal((n, 6))
+            stk = np.ascontiguousarray(stk, dtype=np.float64)
+            r_k = lib.orc_power_rho_st(ptr(g), ptr(A), ptr(Mi), ptr(stk), power_its)
+            Q = np.ascontiguousarray(Lg["Q"])
+            Ac = np.empty((int(np.prod(gc)), 27, 6, 6))
+            lib.orc_q_galerkin(ptr(g), ptr(gc), agg, ptr(A), ptr(Q), ptr(Ac))
+            self.keep += [Q]
+            self.Minv.append(Mi)
+            self.rho.append(r_k)
+            s.A[k] = ptr(A).value
+            s.Q[k] = ptr(Q).value
+            s.Minv[k] = ptr(Mi).value
+            s.w[k] = omega / r_k
+            self.A.append(Ac)
+            A = Ac
+        s.A[len(grids) - 1] = ptr(A).value
+        Dn = dense_from_stencil(A, grids[-1])
+        self.dinv = np.ascontiguousarray(np.linalg.inv(0.5 * (Dn + Dn.T)))
+        s.dinv = ptr(self.dinv).value
+        self.s = s
+
+    def pcg(self, b, tol, maxit=2000):
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(self.N, 3)
+        x = np.zeros_like(b)
+        rel = np.zeros(1)
+        p = self.ptr
+        it = self.lib.orc_amg_pcg(self.ct.byref(self.s), self.N, p(self.nbr), p(self.Kn), p(self.Tv), p(self.free),
+                                  p(b), p(x), float(tol), int(maxit), p(rel))
+        return x, int(it), float(rel[0])

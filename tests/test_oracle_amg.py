@@ -137,3 +137,52 @@ def test_amg_pcg_matches_superlu_and_beats_block_jacobi():
     assert np.abs(x.reshape(-1) - xs).max() <= 1e-8 * np.abs(xs).max()
     _, itj, _ = R.pcg(lat, b, Kn, Tv, R.precond(lat, Kn, Tv, free), free, tol=1e-10)
     assert it < itj
+
+
+def test_box_hierarchy_equals_per_aggregate_gram_schmidt():
+    """The loop-free hierarchy of the CPU baseline equals AMGGeom's per-aggregate Gram-Schmidt."""
+    dims = (17, 11, 16)
+    lat, Kn, Tv, free, _ = _problem(dims)
+    geom = A.AMGGeom(lat, free, agg=3, coarse_max=50)
+    hb = A.box_hierarchy(dims, geom.lo, geom.L, agg=3, coarse_max=50)
+    assert tuple(hb["g0"]) == tuple(geom.l0["g"]) and len(hb["levels"]) == len(geom.levels)
+    fi = np.nonzero(free)[0]
+    g0 = np.array(hb["g0"])
+    for p in fi:
+        rel = lat.ijk[p] - geom.lo
+        cc = np.minimum(rel // 3, g0 - 1)
+        loc = rel - 3 * cc
+        s = sum(1 << d for d in range(3) if cc[d] == g0[d] - 1)
+        ln = [geom.L[d] - 3 * (g0[d] - 1) if cc[d] == g0[d] - 1 else 3 for d in range(3)]
+        row = (loc[2] * ln[1] + loc[1]) * ln[0] + loc[0]
+        blk = hb["tmpl"][hb["toff"][s] + 18 * row: hb["toff"][s] + 18 * row + 18].reshape(3, 6)
+        assert np.abs(blk - geom.l0["P"][p]).max() < 1e-12
+    for Lh, Lr in zip(hb["levels"], geom.levels):
+        assert tuple(Lh["gc"]) == tuple(Lr["gc"]) and np.abs(Lh["Q"] - Lr["Q"]).max() < 1e-10
+
+
+def test_c_amg_pcg_matches_numpy_oracle():
+    """The C/OpenMP AMG-PCG (CPU baseline) reproduces the NumPy oracle: same rho, stencils, iterations, solution."""
+    import ctypes
+    import build_oracle
+    build_oracle.build()
+    lib = ctypes.CDLL(build_oracle.LIB)
+    dims = (20, 19, 21)
+    lat, Kn, Tv, free, up = _problem(dims, seed=2)
+    fixed = ~free
+    b = -R.apply(lat, up * fixed[:, None], Kn, Tv) * free[:, None]
+    geom = A.AMGGeom(lat, free, agg=3, coarse_max=100)
+    ref = A.AMG3D(geom, Kn, Tv, nu=1, gamma=2, power_its=5)
+    M0 = R.precond(lat, Kn, Tv, free)
+    nbr = np.ascontiguousarray(lat.nbr.astype(np.int64))
+    c = A.CAmg(lib, dims, geom.lo, geom.L, nbr, np.ascontiguousarray(Kn), np.ascontiguousarray(Tv), M0, free,
+               agg=3, coarse_max=100, nu=1, gamma=2, power_its=5)
+    assert abs(c.rho0 - ref.rho0) <= 1e-10 * ref.rho0
+    for a_, b_ in zip(c.rho, ref.rho):
+        assert abs(a_ - b_) <= 1e-10 * b_
+    for k in range(len(ref.A)):
+        assert np.abs(c.A[k] - ref.A[k]).max() <= 1e-10 * np.abs(ref.A[k]).max()
+    x, it, rr = c.pcg(b, 1e-10, 500)
+    xr, itr, _ = ref.pcg(b, 1e-10, 500)
+    assert rr <= 1e-10 and abs(it - itr) <= 1
+    assert np.abs(x - xr).max() <= 1e-8 * np.abs(xr).max()
