@@ -542,6 +542,116 @@ def bench_patch(a):
     print(json.dumps(out), flush=True)
 
 
+def _bondnet_setup(n, s, seed):
+    """Two-phase voxel field (fraction 0.4, contrast 4, nu 0.3 / 0.2, seeded), its voxel features and per-cell Voigt
+    parameters, and a bond network of the reference architecture (ch 24, D 12, hid 64) with random weights (last
+    layer N(0, 0.05), so that the outputs are not the Voigt values)."""
+    import torch
+    import geom3d as G
+    import net3d
+    rng = np.random.default_rng(seed)
+    phase = (rng.random(n ** 3) < 0.4).astype(np.int64)
+    X = net3d.voxel_features(phase, 4.0, 0.3, n, s)
+    Cv = G.iso_C(np.where(phase == 1, 4.0, 1.0), np.where(phase == 1, 0.3, 0.2))
+    torch.manual_seed(seed)
+    net = net3d.BondNet(s)
+    with torch.no_grad():
+        for _, _, l3 in net.heads:
+            l3.weight.normal_(0.0, 0.05)
+            l3.bias.normal_(0.0, 0.05)
+    return net, X, Cv, phase
+
+
+def bondnet_cpu_baseline(s, seed, n_sample=48):
+    """The oracle forward + post (NumPy with BLAS on the host's cores) on an n_sample^3 field of the same kind;
+    the caller scales its time by the bond count."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+    import bondnet_ref as BR
+    net, X, Cv, _ = _bondnet_setup(n_sample, s, seed)
+    P = {k: v.detach().double().numpy() for k, v in net.state_dict().items() if not k.endswith("masks")}
+    nc = n_sample // s
+    cp = BR.voigt_cell_params(Cv, n_sample, s)
+    t0 = time.perf_counter()
+    outs, groups = BR.forward(X[None], nc, s, P)
+    BR.post([o[0] for o in outs], groups, cp, nc)
+    dt = time.perf_counter() - t0
+    return dt, sum(len(g["bonds"]) for g in groups)
+
+
+def bench_infer(a):
+    """Bond-network inference (coarse3d/net3d.py + the post of timing/t3d/infer_fast.py): a 256^3 two-phase voxel
+    field, s = 2 -> the Kn / Tv (128^3, 18) of the coarse lattice.  The voxel features and per-cell Voigt parameters
+    are on the GPU before timing; a step is trunk (3 IsoConv3d + 1x1), cell means, both head groups and the
+    scatter.  Roofline of the dominant launch (the face-diagonal heads): first-layer FLOPs 2 nb H nin hid on the
+    FP32 matrix cores."""
+    import torch
+    import net3d
+    n, s = a.n, 2
+    nc = n // s
+    dev = torch.device("cuda:0")
+    net, X, Cv, _ = _bondnet_setup(n, s, a.seed)
+    net = net.to(dev)
+    Xd = torch.as_tensor(X[None], device=dev)
+    cp = net3d.voigt_cell_params(torch.as_tensor(Cv, device=dev), n, s)
+    for w in range(a.warmup):
+        Kn, Tv = net.bond_params(Xd, nc, cp)
+    torch.cuda.synchronize()
+    lat = []
+    ev_feat, ev_heads = [], []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ts = time.perf_counter()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        Fv, Fc = net.features(Xd)
+        e1.record()
+        ev_feat.append((e0, e1))
+        Kn = torch.zeros((1, nc ** 3, 18), device=dev, dtype=torch.float64)
+        Tv = torch.zeros_like(Kn)
+        evs = []
+        net.heads_run(Fv, Fc, nc, cp=cp[None].contiguous(), Kn=Kn, Tv=Tv, events=evs)
+        ev_heads.append(evs)
+        del Fv, Fc
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+        print("[bench] infer step: %.1f ms" % (lat[-1] * 1e3), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    feat_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev_feat]))
+    head_ms = [float(np.mean([ev[gi][0].elapsed_time(ev[gi][1]) for ev in ev_heads])) for gi in range(2)]
+    D, hid = net.D, net.heads[0][0].weight.shape[0]
+    groups = net.tables.groups
+    nbs = [net._device_tables(nc, dev)[gi]["nb"] for gi in range(2)]
+    flops = [2.0 * nbs[gi] * len(groups[gi]["perms"]) * (groups[gi]["nv"] + groups[gi]["ncell"]) * D * hid
+             for gi in range(2)]
+    peak = 157.3
+    ach = flops[1] / (head_ms[1] * 1e-3) / 1e12
+    lat_ms = np.array(lat) * 1e3
+    out = {"metric": "bond-network inferences/s (fine 256^3 voxels -> coarse 128^3 Kn/Tv)",
+           "value": a.steps / total, "unit": "inferences/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": total / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "BondNet s=2 (ch 24, D 12, hid 64, random weights) on a %d^3 two-phase voxel field "
+                                  "(phi 0.4, contrast 4) -> Kn/Tv of the %d^3 coarse lattice" % (n, nc),
+                      "n": n, "s": s, "seed": a.seed, "bonds": nbs},
+           "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+           "stages_ms": {"trunk_and_pool": feat_ms, "heads_axial": head_ms[0], "heads_diagonal": head_ms[1]},
+           "roofline": {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                        "traffic": None, "kernel": "bond_heads_kernel<2,4> (face-diagonal group)",
+                        "alg_flops_per_launch": flops[1], "avg_us": head_ms[1] * 1e3,
+                        "axial": {"alg_flops_per_launch": flops[0], "avg_us": head_ms[0] * 1e3,
+                                  "achieved": flops[0] / (head_ms[0] * 1e-3) / 1e12}}}
+    if not a.no_cpu:
+        print("[bench] cpu baseline (oracle forward on a 48^3 field) ...", file=sys.stderr, flush=True)
+        dt, nb_s = bondnet_cpu_baseline(s, a.seed)
+        scale = sum(nbs) / nb_s
+        out["cpu_baseline"] = {"value": 1.0 / (dt * scale), "unit": "inferences/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+                               "sample": "oracle forward + post (NumPy, BLAS) of a 48^3 field (nc 24, %d bonds) in "
+                                         "%.2f s, scaled by the bond count (x%.1f)" % (nb_s, dt, scale)}
+    print(json.dumps(out), flush=True)
+
+
 def bench_porous256(a):
     """256^3 cement-like microstructure with the pore particles removed (bond words), tension z, FP32 PCG + FP64
     refinement to 1e-8; the masked K p kernel's roofline and the C oracle's PCG on the same reduced lattice."""
@@ -800,7 +910,8 @@ def main():
     ap.add_argument("--probe", type=int, default=0, help="run the profiler probe on an n^3 lattice and exit")
     ap.add_argument("--probe-reps", type=int, default=10)
     ap.add_argument("--config", default="cube256",
-                    choices=["cube256", "plain2d", "kubc3d", "serve", "train3d", "porous256", "amg256", "patch"])
+                    choices=["cube256", "plain2d", "kubc3d", "serve", "train3d", "porous256", "amg256", "patch",
+                             "infer"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--slots", type=int, default=512)
     ap.add_argument("--budget", type=int, default=400000)
@@ -818,7 +929,8 @@ def main():
         sys.exit(2)
     if a.config != "cube256":
         {"plain2d": bench_plain2d, "kubc3d": bench_kubc3d, "serve": bench_serve, "train3d": bench_train3d,
-         "porous256": bench_porous256, "amg256": bench_amg256, "patch": bench_patch}[a.config](a)
+         "porous256": bench_porous256, "amg256": bench_amg256, "patch": bench_patch,
+         "infer": bench_infer}[a.config](a)
         return
     import torch
     dev = torch.device("cuda:0")

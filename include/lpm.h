@@ -259,6 +259,52 @@ int lpm_amg_pcg_refined(const lpm_grid* g, const lpm_amg* a32, int32_t layout64,
                         void* x64, double tol, double inner_tol, int32_t max_outer, int32_t maxit, void* work,
                         int32_t* iters, int32_t* outer, double* relres, void* stream);
 
+/* ---------------------------------------------------------------------------------------------------------------
+ * Bond-network inference: the producer of the coarse Kn / Tv (coarse3d/net3d.py, timing/t3d/infer3d.py).
+ * Feature fields are FP32.  Voxel grids are (nz, ny, nx) with x fastest.
+ * --------------------------------------------------------------------------------------------------------------- */
+
+/* One O_h-invariant 3x3x3 convolution with zero padding (IsoConv3d.forward, net3d.py:35-49): per output channel o
+ * y = b_o + sum_i (w[o,i,0] centre + w[o,i,1] sum of the 6 face + w[o,i,2] sum of the 12 edge + w[o,i,3] sum of the
+ * 8 corner neighbours of x_i), then GELU (erf form) when gelu != 0.  x (B, cin, nz, ny, nx), w (cout, cin, 4), b
+ * (cout).  With dproj == 0 the output is y (B, cout, nz, ny, nx); with dproj > 0 the 1x1 convolution that ends the
+ * trunk (net3d.py:125-126) is applied as well, pw (dproj, cout), pb (dproj), and written voxel-major:
+ * y (B, nz*ny*nx, dstride) with channels dproj..dstride-1 set to zero.  cout <= 32, dproj <= dstride <= 16. */
+int lpm_iso_conv(int32_t B, int32_t cin, int32_t cout, int32_t nx, int32_t ny, int32_t nz, const float* x,
+                 const float* w, const float* b, int32_t gelu, int32_t dproj, int32_t dstride, const float* pw,
+                 const float* pb, float* y, void* stream);
+
+/* Cell means of voxel-major features (F.avg_pool3d(Fv, s, s), net3d.py:154): fv (B, (ncp s)^3, dstride) ->
+ * fc (B, ncp^3, dstride). */
+int lpm_cell_pool(int32_t B, int32_t ncp, int32_t s, int32_t dstride, const float* fv, float* fc, void* stream);
+
+/* One direction group of bond heads (BondNet.forward, net3d.py:151-166, with the bond list of BondWindows,
+ * net3d.py:74-118, and the post-processing of infer_fast.py:160-177 / train3d.py:100-108).  Bonds are numbered as
+ * the reference lists them: direction by direction, and within a direction the cells I of the nc^3 box whose
+ * neighbour I + o is in the box, x fastest.  Window slot j of a bond in direction d reads voxel
+ * (I + 2) s + voff[d, j] (j < nv) or cell I + 2 + coff[d, j - nv] (offsets as linear ids in the padded voxel grid of
+ * side npd = (nc + 4) s, resp. the padded cell grid of side nc + 4).  inv (H, nslot): slot of the window read for
+ * slot j under stabiliser element g, so that z_g = W1 P_g x.  w1f: W1 (hid, nslot * D) rearranged by
+ * lpm_bond_w1_layout.  out (B, nb, 2) raw outputs; when cp (B, nc^3, 10) (per-cell 9 direction Kn and Tv) is given,
+ * kn / tv (B, nc^3, 18) receive Kn = Kn_V exp(o1), Tv = Tv_V + Kn_V o2 in both directed slots of every bond, with
+ * (Kn_V, Tv_V) the two-end means of cp (geom3d.py:214-220); other slots are left untouched. */
+typedef struct {
+  int32_t ndir;         /* directions of the group (3 axial or 6 face-diagonal) */
+  int32_t nv, ncell;    /* voxel slots (2 s^3) and cell slots of the window */
+  int32_t H;            /* stabiliser order: 16 (axial) or 8 (diagonal) */
+  int32_t D, dstride;   /* feature channels and their stride in fv / fc (a multiple of 4, <= 16) */
+  int32_t hid;          /* hidden width, <= 64 */
+  int32_t slot[6];      /* forward slot k (even, 0..16) of every direction, in bond order */
+  const int32_t* voff;  /* device (ndir, nv) */
+  const int32_t* coff;  /* device (ndir, ncell) */
+  const int32_t* inv;   /* device (H, nv + ncell) */
+} lpm_bond_group;
+size_t lpm_bond_w1_size(const lpm_bond_group* grp);
+int lpm_bond_w1_layout(const lpm_bond_group* grp, const float* w1, float* w1f, void* stream);
+int lpm_bond_heads(const lpm_bond_group* grp, int32_t B, int32_t nc, int32_t s, const float* fv, const float* fc,
+                   const float* w1f, const float* b1, const float* w2, const float* b2, const float* w3,
+                   const float* b3, float* out, const double* cp, double* kn, double* tv, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,554 @@
+// Bond-network inference (declarations and reference citations in include/lpm.h): the O_h-invariant 3x3x3
+// convolutions of the trunk as class sums staged in LDS followed by a channel mix in registers, the cell means, and
+// the bond heads as one fused kernel per direction group: window gather into LDS, the stabiliser-expanded first
+// layer on the FP32 matrix cores (v_mfma_f32_16x16x4f32), GELU and the mean over the stabiliser, the two small
+// layers, and the Voigt-based scatter of Kn / Tv.
+#include <algorithm>
+#include <cmath>
+
+#include "capi_common.hpp"
+
+using lpmc::fail;
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float gelu_erf(float v) { return 0.5f * v * (1.0f + erff(v * 0.70710678118654752440f)); }
+
+// ------------------------------------------------------------------------------------------------ trunk layers
+// Block = 64 x 4 threads over (x, y), each thread 4 consecutive z voxels; per input channel the (4+2) x (4+2) x
+// (64+2) zero-padded neighbourhood is staged in LDS (the next channel's values are loaded into registers while the
+// current one is mixed).  Per plane a thread forms its in-plane centre c, face f (4 neighbours) and corner k (4
+// diagonal neighbours) sums; the 3D classes of voxel z are then
+//   centre = c_z, faces = f_z + c_{z-1} + c_{z+1}, edges = k_z + f_{z-1} + f_{z+1}, corners = k_{z-1} + k_{z+1}.
+// The weights (cin x COP float4, zero-padded output channels) sit in LDS and are read as broadcasts.
+constexpr int CT_X = 64, CT_Y = 4, CT_Z = 4;
+constexpr int CT_HALO = (CT_Z + 2) * (CT_Y + 2) * (CT_X + 2);
+constexpr int CT_PER = (CT_HALO + 255) / 256;
+constexpr int CT_MAXCIN = 32;
+
+template <int COP, bool PROJ>
+__global__ __launch_bounds__(256) void iso_conv_kernel(int cin, int cout, int nx, int ny, int nz,
+                                                       const float* __restrict__ x, const float* __restrict__ w,
+                                                       const float* __restrict__ b, int gelu, int dproj,
+                                                       int dstride, const float* __restrict__ pw,
+                                                       const float* __restrict__ pb, float* __restrict__ y) {
+  __shared__ float tile[CT_Z + 2][CT_Y + 2][CT_X + 2];
+  __shared__ f4 wl[CT_MAXCIN * COP];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * CT_X, y0 = blockIdx.y * CT_Y;
+  const int zt = (nz + CT_Z - 1) / CT_Z;
+  const int bi = blockIdx.z / zt, z0 = (blockIdx.z % zt) * CT_Z;
+  const int64_t V = (int64_t)nx * ny * nz;
+  const float* xb = x + (int64_t)bi * cin * V;
+
+  for (int i = threadIdx.x; i < cin * COP; i += 256) {
+    const int ci = i / COP, co = i % COP;
+    f4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (co < cout) v = *(const f4*)(w + ((int64_t)co * cin + ci) * 4);
+    wl[i] = v;
+  }
+  // this thread's halo elements: channel-relative offsets (-1 = outside the box, read as zero)
+  int hoff[CT_PER];
+#pragma unroll
+  for (int r = 0; r < CT_PER; ++r) {
+    const int idx = threadIdx.x + 256 * r;
+    hoff[r] = -1;
+    if (idx < CT_HALO) {
+      const int xx = idx % (CT_X + 2), q = idx / (CT_X + 2);
+      const int yy = q % (CT_Y + 2), zz = q / (CT_Y + 2);
+      const int gx = x0 + xx - 1, gy = y0 + yy - 1, gz = z0 + zz - 1;
+      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) hoff[r] = (gz * ny + gy) * nx + gx;
+    }
+  }
+  float hv[CT_PER];
+#pragma unroll
+  for (int r = 0; r < CT_PER; ++r) hv[r] = hoff[r] >= 0 ? xb[hoff[r]] : 0.0f;
+
+  float acc[CT_Z][COP];
+#pragma unroll
+  for (int co = 0; co < COP; ++co) {
+    const float bias = co < cout ? b[co] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < CT_Z; ++j) acc[j][co] = bias;
+  }
+  float* tflat = &tile[0][0][0];
+
+  for (int ci = 0; ci < cin; ++ci) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < CT_PER; ++r) {
+      const int idx = threadIdx.x + 256 * r;
+      if (idx < CT_HALO) tflat[idx] = hv[r];
+    }
+    __syncthreads();
+    if (ci + 1 < cin) {
+      const float* xn = xb + (int64_t)(ci + 1) * V;
+#pragma unroll
+      for (int r = 0; r < CT_PER; ++r) hv[r] = hoff[r] >= 0 ? xn[hoff[r]] : 0.0f;
+    }
+    float c[CT_Z + 2], f[CT_Z + 2], k[CT_Z + 2];
+#pragma unroll
+    for (int p = 0; p < CT_Z + 2; ++p) {
+      c[p] = tile[p][ty + 1][tx + 1];
+      f[p] = (tile[p][ty + 1][tx] + tile[p][ty + 1][tx + 2]) + (tile[p][ty][tx + 1] + tile[p][ty + 2][tx + 1]);
+      k[p] = (tile[p][ty][tx] + tile[p][ty][tx + 2]) + (tile[p][ty + 2][tx] + tile[p][ty + 2][tx + 2]);
+    }
+    const f4* wc = wl + ci * COP;
+#pragma unroll
+    for (int co = 0; co < COP; ++co) {
+      const f4 wv = wc[co];
+#pragma unroll
+      for (int j = 0; j < CT_Z; ++j) {
+        const float sc = c[j + 1];
+        const float sf = f[j + 1] + (c[j] + c[j + 2]);
+        const float se = k[j + 1] + (f[j] + f[j + 2]);
+        const float sk = k[j] + k[j + 2];
+        float a = acc[j][co];
+        a = fmaf(wv[0], sc, a);
+        a = fmaf(wv[1], sf, a);
+        a = fmaf(wv[2], se, a);
+        a = fmaf(wv[3], sk, a);
+        acc[j][co] = a;
+      }
+    }
+  }
+
+  const int gx = x0 + tx, gy = y0 + ty;
+  if (gx >= nx || gy >= ny) return;
+#pragma unroll
+  for (int j = 0; j < CT_Z; ++j) {
+    const int gz = z0 + j;
+    if (gz >= nz) break;
+    const int64_t vox = ((int64_t)gz * ny + gy) * nx + gx;
+    float a[COP];
+#pragma unroll
+    for (int co = 0; co < COP; ++co) a[co] = gelu ? gelu_erf(acc[j][co]) : acc[j][co];
+    if constexpr (PROJ) {
+      float* yo = y + ((int64_t)bi * V + vox) * dstride;
+      for (int d = 0; d < dstride; ++d) {
+        float v = 0.0f;
+        if (d < dproj) {
+          v = pb[d];
+#pragma unroll
+          for (int co = 0; co < COP; ++co)
+            if (co < cout) v = fmaf(pw[d * cout + co], a[co], v);
+        }
+        yo[d] = v;
+      }
+    } else {
+#pragma unroll
+      for (int co = 0; co < COP; ++co)
+        if (co < cout) y[((int64_t)bi * cout + co) * V + vox] = a[co];
+    }
+  }
+}
+
+template <int COP>
+int launch_iso(int B, int cin, int cout, int nx, int ny, int nz, const float* x, const float* w, const float* b,
+               int gelu, int dproj, int dstride, const float* pw, const float* pb, float* y, hipStream_t st) {
+  dim3 grid((nx + CT_X - 1) / CT_X, (ny + CT_Y - 1) / CT_Y, (unsigned)(B * ((nz + CT_Z - 1) / CT_Z)));
+  if (dproj > 0)
+    iso_conv_kernel<COP, true><<<grid, 256, 0, st>>>(cin, cout, nx, ny, nz, x, w, b, gelu, dproj, dstride, pw, pb, y);
+  else
+    iso_conv_kernel<COP, false><<<grid, 256, 0, st>>>(cin, cout, nx, ny, nz, x, w, b, gelu, 0, 0, nullptr, nullptr,
+                                                       y);
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+__global__ void cell_pool_kernel(int B, int ncp, int s, int ds, const float* __restrict__ fv, float* __restrict__ fc) {
+  const int64_t nce = (int64_t)ncp * ncp * ncp;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)B * nce * ds) return;
+  const int d = (int)(idx % ds);
+  const int64_t cell = (idx / ds) % nce, bi = idx / (ds * nce);
+  const int cx = (int)(cell % ncp), cy = (int)((cell / ncp) % ncp), cz = (int)(cell / ((int64_t)ncp * ncp));
+  const int64_t npd = (int64_t)ncp * s;
+  const float* src = fv + bi * npd * npd * npd * ds;
+  float sum = 0.0f;
+  for (int a = 0; a < s; ++a)
+    for (int bb = 0; bb < s; ++bb)
+      for (int c = 0; c < s; ++c)
+        sum += src[(((int64_t)(cz * s + a) * npd + (cy * s + bb)) * npd + (cx * s + c)) * ds + d];
+  fc[idx] = sum / (float)(s * s * s);
+}
+
+// ------------------------------------------------------------------------------------------------- bond heads
+// A block takes HB = 16 consecutive bonds of one group.  The first layer is a GEMM whose rows are the HB * H
+// (bond, stabiliser element) pairs, K = nslot * dstride window features and N = hid (padded to NT = 1, 2 or 4
+// column tiles of 16): row (b, g) reads window slot inv[g, j] where W1 expects slot j.  Four waves, each RT = H / 4
+// row tiles of 16 rows and all NT column tiles, so that one k-step of 4 costs RT LDS reads (A) and NT global loads
+// (B) for RT x NT MFMAs; the fragments of the next k-step are loaded before the MFMAs of the current one.
+// v_mfma_f32_16x16x4f32 lane layout: A[lane % 16][lane / 16], B[lane / 16][lane % 16], D[4 (lane / 16) + i][lane % 16].
+// W1 comes pre-arranged (lpm_bond_w1_layout) so that every B fragment is one coalesced 256-byte load.
+constexpr int HB = 16;
+constexpr int HMAX = 64;
+
+struct HeadArgs {
+  int ndir, nv, ncell, nslot, H, D, ds, hid, Q, xs_floats;
+  int nc, s, npd, ncp, B;
+  int dk[6], dx[6], dy[6], dz[6];
+  int64_t nb;
+  int64_t cnt[7];
+  const int* voff;
+  const int* coff;
+  const int* inv;
+  const float* fv;
+  const float* fc;
+  const float* w1f;
+  const float* b1;
+  const float* w2;
+  const float* b2;
+  const float* w3;
+  const float* b3;
+  float* out;
+  const double* cp;
+  double* kn;
+  double* tv;
+};
+
+struct BondMeta {
+  int dir, vbase, cbase, p, q, k;
+};
+
+template <int H, int NT>
+__global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
+  extern __shared__ __align__(16) float smem[];
+  static_assert((H == 8 || H == 16) && (NT == 1 || NT == 2 || NT == 4), "tile shape");
+  constexpr int RT = H / 4;
+  const int nslot = a.nslot, ds = a.ds;
+  float* xs = smem;                                             // [HB][nslot][ds]
+  int* inv_s = (int*)(xs + a.xs_floats);                        // [H][nslot]
+  BondMeta* meta = (BondMeta*)(inv_s + H * nslot);              // [HB]
+  float* hbuf = xs;                                             // [HB][HMAX], after the GEMM
+  float* h2buf = xs + HB * HMAX;                                // [HB][HMAX]
+  float* obuf = xs + 2 * HB * HMAX;                             // [HB][2]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int bi = blockIdx.y;
+  const int64_t b0 = (int64_t)blockIdx.x * HB;
+
+  if (tid < HB) {
+    BondMeta m{-1, 0, 0, 0, 0, 0};
+    const int64_t gb = b0 + tid;
+    if (gb < a.nb) {
+      int d = 0;
+      while (gb >= a.cnt[d + 1]) ++d;
+      const int ox = a.dx[d], oy = a.dy[d], oz = a.dz[d];
+      const int ex = a.nc - abs(ox), ey = a.nc - abs(oy);
+      const int64_t loc = gb - a.cnt[d];
+      const int ix = (int)(loc % ex) + (ox < 0), iy = (int)((loc / ex) % ey) + (oy < 0);
+      const int iz = (int)(loc / ((int64_t)ex * ey)) + (oz < 0);
+      m.dir = d;
+      m.k = a.dk[d];
+      m.vbase = (((iz + 2) * a.s) * a.npd + (iy + 2) * a.s) * a.npd + (ix + 2) * a.s;
+      m.cbase = ((iz + 2) * a.ncp + (iy + 2)) * a.ncp + (ix + 2);
+      m.p = (iz * a.nc + iy) * a.nc + ix;
+      m.q = ((iz + oz) * a.nc + (iy + oy)) * a.nc + (ix + ox);
+    }
+    meta[tid] = m;
+  }
+  for (int i = tid; i < H * nslot; i += 256) inv_s[i] = a.inv[i];
+  __syncthreads();
+
+  // window gather: HB x nslot x (ds / 4) float4
+  {
+    const int64_t nvox = (int64_t)a.npd * a.npd * a.npd, ncel = (int64_t)a.ncp * a.ncp * a.ncp;
+    const float* fvb = a.fv + (int64_t)bi * nvox * ds;
+    const float* fcb = a.fc + (int64_t)bi * ncel * ds;
+    const int total = HB * nslot * a.Q;
+    for (int idx = tid; idx < total; idx += 256) {
+      const int c = idx % a.Q, r = idx / a.Q;
+      const int slot = r % nslot, bl = r / nslot;
+      const BondMeta m = meta[bl];
+      f4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (m.dir >= 0) {
+        const float* src = slot < a.nv ? fvb + (int64_t)(m.vbase + a.voff[m.dir * a.nv + slot]) * ds
+                                       : fcb + (int64_t)(m.cbase + a.coff[m.dir * a.ncell + slot - a.nv]) * ds;
+        v = *(const f4*)(src + 4 * c);
+      }
+      *(f4*)(xs + (bl * nslot + slot) * ds + 4 * c) = v;
+    }
+  }
+  __syncthreads();
+
+  f4 acc[RT][NT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[t][n] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  // rows of tile t: wave * RT * 16 + t * 16 + lane % 16 -> bond row / H, stabiliser element row % H
+  int xrow[RT], grow[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const int row = wave * RT * 16 + t * 16 + (lane & 15);
+    xrow[t] = (row / H) * nslot * ds + (lane >> 4);
+    grow[t] = (row % H) * nslot;
+  }
+  const float* w1l = a.w1f + lane;
+  const int ksteps = nslot * a.Q;
+  int slot = 0, j = 0;
+  int aoff[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) aoff[t] = xrow[t] + inv_s[grow[t]] * ds;
+  float bf[NT], af[RT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) bf[n] = w1l[n * 64];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) af[t] = xs[aoff[t]];
+  for (int kk = 0; kk < ksteps; ++kk) {
+    float bc[NT], ac[RT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) bc[n] = bf[n];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) ac[t] = af[t];
+    if (kk + 1 < ksteps) {
+      if (++j == a.Q) {
+        j = 0;
+        ++slot;
+#pragma unroll
+        for (int t = 0; t < RT; ++t) aoff[t] = xrow[t] + inv_s[grow[t] + slot] * ds;
+      }
+      const float* wk = w1l + (int64_t)(kk + 1) * NT * 64;
+#pragma unroll
+      for (int n = 0; n < NT; ++n) bf[n] = wk[n * 64];
+#pragma unroll
+      for (int t = 0; t < RT; ++t) af[t] = xs[aoff[t] + 4 * j];
+    }
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int n = 0; n < NT; ++n) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t], bc[n], acc[t][n], 0, 0, 0);
+  }
+  __syncthreads();  // xs is reused below
+
+  // + b1, GELU, mean over the stabiliser (rows of one bond)
+  {
+    const float invH = 1.0f / (float)H;
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        const int col = n * 16 + (lane & 15);
+        const float bias = col < a.hid ? a.b1[col] : 0.0f;
+        float sum = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sum += gelu_erf(acc[t][n][i] + bias);
+        sum += __shfl_xor(sum, 16);
+        if constexpr (H == 16) {
+          sum += __shfl_xor(sum, 32);
+          if ((lane >> 4) == 0 && col < a.hid) hbuf[(wave * 4 + t) * HMAX + col] = sum * invH;
+        } else {
+          if ((lane & 16) == 0 && col < a.hid) hbuf[(wave * 4 + t * 2 + (lane >> 5)) * HMAX + col] = sum * invH;
+        }
+      }
+  }
+  __syncthreads();
+
+  {
+    const int bl = tid >> 4;
+    for (int j = tid & 15; j < a.hid; j += 16) {
+      float v = a.b2[j];
+      const float* wr = a.w2 + j * a.hid;
+      for (int m = 0; m < a.hid; ++m) v = fmaf(wr[m], hbuf[bl * HMAX + m], v);
+      h2buf[bl * HMAX + j] = gelu_erf(v);
+    }
+  }
+  __syncthreads();
+  if (tid < 2 * HB) {
+    const int bl = tid >> 1, c = tid & 1;
+    float v = a.b3[c];
+    const float* wr = a.w3 + c * a.hid;
+    for (int m = 0; m < a.hid; ++m) v = fmaf(wr[m], h2buf[bl * HMAX + m], v);
+    obuf[bl * 2 + c] = v;
+    if (meta[bl].dir >= 0) a.out[((int64_t)bi * a.nb + b0 + bl) * 2 + c] = v;
+  }
+  if (!a.cp) return;
+  __syncthreads();
+  if (tid < HB && meta[tid].dir >= 0) {
+    const BondMeta m = meta[tid];
+    const int64_t nce = (int64_t)a.nc * a.nc * a.nc;
+    const double* cpb = a.cp + (int64_t)bi * nce * 10;
+    const double kv = 0.5 * (cpb[(int64_t)m.p * 10 + m.k / 2] + cpb[(int64_t)m.q * 10 + m.k / 2]);
+    const double tb = 0.5 * (cpb[(int64_t)m.p * 10 + 9] + cpb[(int64_t)m.q * 10 + 9]);
+    const double knv = kv * exp((double)obuf[tid * 2]);
+    const double tvv = tb + kv * (double)obuf[tid * 2 + 1];
+    double* knb = a.kn + (int64_t)bi * nce * 18;
+    double* tvb = a.tv + (int64_t)bi * nce * 18;
+    knb[(int64_t)m.p * 18 + m.k] = knv;
+    knb[(int64_t)m.q * 18 + (m.k ^ 1)] = knv;
+    tvb[(int64_t)m.p * 18 + m.k] = tvv;
+    tvb[(int64_t)m.q * 18 + (m.k ^ 1)] = tvv;
+  }
+}
+
+__global__ void w1_layout_kernel(int nslot, int D, int Q, int NT, int hid, const float* __restrict__ w1,
+                                 float* __restrict__ w1f) {
+  const int64_t total = (int64_t)nslot * Q * NT * 64;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int lane = (int)(idx & 63);
+  int64_t r = idx >> 6;
+  const int n = (int)(r % NT);
+  r /= NT;
+  const int j = (int)(r % Q);
+  const int slot = (int)(r / Q);
+  const int row = n * 16 + (lane & 15), d = 4 * j + (lane >> 4);
+  w1f[idx] = (row < hid && d < D) ? w1[(int64_t)row * nslot * D + (int64_t)slot * D + d] : 0.0f;
+}
+
+// column tiles of the first layer: 1, 2 or 4 (hidden widths above 32 are padded to 64 with zero weights)
+inline int col_tiles(int hid) { return hid <= 16 ? 1 : hid <= 32 ? 2 : 4; }
+
+int check_group(const lpm_bond_group* g) {
+  if (!g) return fail(LPM_E_BADARG, "bond heads: null group");
+  if (!((g->ndir == 3 && g->H == 16) || (g->ndir == 6 && g->H == 8)))
+    return fail(LPM_E_BADARG, "bond heads: %d directions with stabiliser order %d", g->ndir, g->H);
+  if (g->nv < 1 || g->ncell < 1 || g->nv + g->ncell > 512) return fail(LPM_E_BADARG, "bond heads: window slots");
+  if (g->D < 1 || g->dstride < g->D || g->dstride % 4 || g->dstride > 16)
+    return fail(LPM_E_BADARG, "bond heads: %d channels with stride %d", g->D, g->dstride);
+  if (g->hid < 1 || g->hid > HMAX) return fail(LPM_E_BADARG, "bond heads: hidden width %d (1..%d)", g->hid, HMAX);
+  if (!g->voff || !g->coff || !g->inv) return fail(LPM_E_BADARG, "bond heads: null table");
+  for (int d = 0; d < g->ndir; ++d) {
+    const int k = g->slot[d];
+    if (k < 0 || k >= 18 || (k & 1) || (g->H == 16) != (k < 6))
+      return fail(LPM_E_BADARG, "bond heads: forward slot %d in a group of stabiliser order %d", k, g->H);
+  }
+  return LPM_OK;
+}
+
+// floats of the LDS area that holds the window features during the GEMM and the hidden layers after it
+int head_xs_floats(const lpm_bond_group* g) {
+  return std::max(HB * (g->nv + g->ncell) * g->dstride, 2 * HB * HMAX + 2 * HB);
+}
+
+size_t head_lds(const lpm_bond_group* g) {
+  return head_xs_floats(g) * sizeof(float) + (size_t)g->H * (g->nv + g->ncell) * sizeof(int) + HB * sizeof(BondMeta);
+}
+
+// slot offsets of the SC lattice (coarse3d/geom3d.py:16-20)
+constexpr int SC_OFF[18][3] = {{1, 0, 0},  {-1, 0, 0},  {0, 1, 0},  {0, -1, 0}, {0, 0, 1},  {0, 0, -1},
+                               {1, 1, 0},  {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 0, 1},  {-1, 0, -1},
+                               {1, 0, -1}, {-1, 0, 1},  {0, 1, 1},  {0, -1, -1}, {0, 1, -1}, {0, -1, 1}};
+
+template <int H, int NT>
+int launch_heads(const HeadArgs& a, size_t lds, hipStream_t st) {
+  dim3 grid((unsigned)((a.nb + HB - 1) / HB), (unsigned)a.B);
+  bond_heads_kernel<H, NT><<<grid, 256, lds, st>>>(a);
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+template <int H>
+int launch_heads_nt(const HeadArgs& a, size_t lds, hipStream_t st) {
+  const int nt = (a.hid + 15) / 16;
+  if (nt == 1) return launch_heads<H, 1>(a, lds, st);
+  if (nt == 2) return launch_heads<H, 2>(a, lds, st);
+  return launch_heads<H, 4>(a, lds, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpm_iso_conv(int32_t B, int32_t cin, int32_t cout, int32_t nx, int32_t ny, int32_t nz, const float* x,
+                 const float* w, const float* b, int32_t gelu, int32_t dproj, int32_t dstride, const float* pw,
+                 const float* pb, float* y, void* stream) {
+  if (B < 1 || cin < 1 || cin > CT_MAXCIN || cout < 1 || cout > 32 || nx < 1 || ny < 1 || nz < 1 || !x || !w || !b ||
+      !y)
+    return fail(LPM_E_BADARG, "lpm_iso_conv: bad arguments (cin %d, cout %d: 1..32)", cin, cout);
+  if ((int64_t)nx * ny * nz >= ((int64_t)1 << 31)) return fail(LPM_E_BADARG, "lpm_iso_conv: grid too large");
+  if (dproj < 0 || (dproj > 0 && (dstride < dproj || dstride > 16 || !pw || !pb)))
+    return fail(LPM_E_BADARG, "lpm_iso_conv: projection to %d channels with stride %d", dproj, dstride);
+  if ((int64_t)B * ((nz + CT_Z - 1) / CT_Z) > 65535) return fail(LPM_E_BADARG, "lpm_iso_conv: grid too deep");
+  hipStream_t st = (hipStream_t)stream;
+  if (cout <= 8) return launch_iso<8>(B, cin, cout, nx, ny, nz, x, w, b, gelu, dproj, dstride, pw, pb, y, st);
+  if (cout <= 16) return launch_iso<16>(B, cin, cout, nx, ny, nz, x, w, b, gelu, dproj, dstride, pw, pb, y, st);
+  if (cout <= 24) return launch_iso<24>(B, cin, cout, nx, ny, nz, x, w, b, gelu, dproj, dstride, pw, pb, y, st);
+  return launch_iso<32>(B, cin, cout, nx, ny, nz, x, w, b, gelu, dproj, dstride, pw, pb, y, st);
+}
+
+int lpm_cell_pool(int32_t B, int32_t ncp, int32_t s, int32_t dstride, const float* fv, float* fc, void* stream) {
+  if (B < 1 || ncp < 1 || s < 1 || dstride < 1 || !fv || !fc) return fail(LPM_E_BADARG, "lpm_cell_pool: bad arguments");
+  const int64_t n = (int64_t)B * ncp * ncp * ncp * dstride;
+  cell_pool_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(B, ncp, s, dstride, fv, fc);
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+size_t lpm_bond_w1_size(const lpm_bond_group* g) {
+  if (!g || g->dstride < 4 || g->hid < 1) return 0;
+  return (size_t)(g->nv + g->ncell) * (g->dstride / 4) * col_tiles(g->hid) * 64 * sizeof(float);
+}
+
+int lpm_bond_w1_layout(const lpm_bond_group* g, const float* w1, float* w1f, void* stream) {
+  if (int rc = check_group(g)) return rc;
+  if (!w1 || !w1f) return fail(LPM_E_BADARG, "lpm_bond_w1_layout: null weights");
+  const int nslot = g->nv + g->ncell, Q = g->dstride / 4, NT = col_tiles(g->hid);
+  const int64_t n = (int64_t)nslot * Q * NT * 64;
+  w1_layout_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(nslot, g->D, Q, NT, g->hid, w1, w1f);
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+int lpm_bond_heads(const lpm_bond_group* g, int32_t B, int32_t nc, int32_t s, const float* fv, const float* fc,
+                   const float* w1f, const float* b1, const float* w2, const float* b2, const float* w3,
+                   const float* b3, float* out, const double* cp, double* kn, double* tv, void* stream) {
+  if (int rc = check_group(g)) return rc;
+  if (B < 1 || B > 65535 || nc < 2 || s < 1 || !fv || !fc || !w1f || !b1 || !w2 || !b2 || !w3 || !b3 || !out)
+    return fail(LPM_E_BADARG, "lpm_bond_heads: bad arguments");
+  if (cp && (!kn || !tv)) return fail(LPM_E_BADARG, "lpm_bond_heads: cp given without kn / tv");
+  const int64_t npd = (int64_t)(nc + 4) * s;
+  if (npd * npd * npd >= ((int64_t)1 << 31)) return fail(LPM_E_BADARG, "lpm_bond_heads: padded grid too large");
+  const size_t lds = head_lds(g);
+  if (lds > 65536) return fail(LPM_E_BADARG, "lpm_bond_heads: window needs %zu bytes of LDS (64 KiB)", lds);
+  HeadArgs a{};
+  a.ndir = g->ndir;
+  a.nv = g->nv;
+  a.ncell = g->ncell;
+  a.nslot = g->nv + g->ncell;
+  a.H = g->H;
+  a.D = g->D;
+  a.ds = g->dstride;
+  a.hid = g->hid;
+  a.Q = g->dstride / 4;
+  a.nc = nc;
+  a.s = s;
+  a.npd = (int)npd;
+  a.ncp = nc + 4;
+  a.B = B;
+  a.xs_floats = head_xs_floats(g);
+  a.cnt[0] = 0;
+  for (int d = 0; d < 6; ++d) {
+    const int k = d < g->ndir ? g->slot[d] : 0;
+    a.dk[d] = k;
+    a.dx[d] = SC_OFF[k][0];
+    a.dy[d] = SC_OFF[k][1];
+    a.dz[d] = SC_OFF[k][2];
+  }
+  // bond counts per direction: the cells whose neighbour stays in the box
+  for (int d = 0; d < g->ndir; ++d)
+    a.cnt[d + 1] = a.cnt[d] + (int64_t)(nc - std::abs(a.dx[d])) * (nc - std::abs(a.dy[d])) * (nc - std::abs(a.dz[d]));
+  for (int d = g->ndir + 1; d < 7; ++d) a.cnt[d] = a.cnt[g->ndir];
+  a.nb = a.cnt[g->ndir];
+  a.voff = g->voff;
+  a.coff = g->coff;
+  a.inv = g->inv;
+  a.fv = fv;
+  a.fc = fc;
+  a.w1f = w1f;
+  a.b1 = b1;
+  a.w2 = w2;
+  a.b2 = b2;
+  a.w3 = w3;
+  a.b3 = b3;
+  a.out = out;
+  a.cp = cp;
+  a.kn = kn;
+  a.tv = tv;
+  hipStream_t st = (hipStream_t)stream;
+  return g->H == 16 ? launch_heads_nt<16>(a, lds, st) : launch_heads_nt<8>(a, lds, st);
+}
+
+}  // extern "C"

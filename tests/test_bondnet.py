@@ -1,0 +1,236 @@
+"""Bond-network inference (coarse3d/net3d.py): the oracle (oracle/bondnet_ref.py) pinned by the reference's own outputs
+(tests/golden/bondnet_*.npz, written by tests/golden/make_bondnet_golden.py), the class-sum convolution against
+torch's conv3d with the expanded kernel, and the cube-symmetry equivariance that the reference guarantees; the
+product's window tables against the reference's BondWindows; the HIP path (net3d.py of the package) against the oracle
+and the fixtures.  GPU results are FP32 against FP64 references: tolerances are relative to the largest output."""
+import os
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+import bondnet_ref as BR
+import net3d
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+CASES = ["bondnet_s2_small", "bondnet_s3_small", "bondnet_s2_full"]
+TOL32 = 2e-5          # FP32 kernels against the FP64 oracle, relative to the largest magnitude
+
+
+def _fixture(name):
+    f = np.load(os.path.join(GOLD, name + ".npz"))
+    P = {k[2:]: f[k].astype(np.float64) for k in f.files if k.startswith("w/")}
+    return f, P
+
+
+def _net_from(P, s, ch, D, hid, device):
+    net = net3d.BondNet(s, ch=ch, D=D, hid=hid)
+    net.load_state_dict({k: torch.as_tensor(v, dtype=torch.float32) for k, v in P.items()}, strict=False)
+    return net.to(device)
+
+
+def _random_net(s, ch, D, hid, seed, head_scale=0.05):
+    torch.manual_seed(seed)
+    net = net3d.BondNet(s, ch=ch, D=D, hid=hid)
+    with torch.no_grad():
+        for l1, l2, l3 in net.heads:
+            l3.weight.normal_(0.0, head_scale)
+            l3.bias.normal_(0.0, head_scale)
+        for m in (net.trunk[0], net.trunk[2], net.trunk[4]):
+            m.b.normal_(0.0, 0.1)
+    P = {k: v.detach().double().numpy() for k, v in net.state_dict().items() if not k.endswith("masks")}
+    return net, P
+
+
+def _micro(n, s, seed, contrast=4.0):
+    rng = np.random.default_rng(seed)
+    phase = (rng.random(n ** 3) < 0.4).astype(np.int64)
+    return phase, BR.voxel_features(phase, contrast, 0.3, n, s)
+
+
+# ------------------------------------------------------------------------------------------------- symmetry helpers
+def _rotate(Fld, g):
+    """Field over the last three axes (z, y, x) of a cube, rotated about its centre: out[g v] = Fld[v]."""
+    n = Fld.shape[-1]
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    v2 = 2 * np.stack([x, y, z], -1) - (n - 1)                   # doubled coordinates of the output voxel
+    src = ((v2 @ g) + (n - 1)) // 2                               # g^T v' (g orthogonal)
+    return Fld[..., src[..., 2], src[..., 1], src[..., 0]]
+
+
+def _rotate_bonds(A, g, nc):
+    """Directed per-slot field (nc^3, 18): out[g p, k'] = A[p, k] with offset(k') = g offset(k)."""
+    off = BR.SC_OFF
+    kmap = np.array([next(j for j in range(18) if np.array_equal(off[j], g @ off[k])) for k in range(18)])
+    ijk = np.stack(np.meshgrid(np.arange(nc), np.arange(nc), np.arange(nc), indexing="ij"), -1).reshape(-1, 3)[:, ::-1]
+    d2 = (2 * ijk - (nc - 1)) @ g.T + (nc - 1)
+    q = ((d2[:, 2] // 2) * nc + d2[:, 1] // 2) * nc + d2[:, 0] // 2
+    out = np.zeros_like(A)
+    out[q[:, None], kmap[None, :]] = A
+    return out
+
+
+# -------------------------------------------------------------------------------------------------------- CPU
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_matches_reference_fixture(name):
+    f, P = _fixture(name)
+    outs, groups = BR.forward(f["X"], int(f["nc"]), int(f["s"]), P)
+    Fv = BR.trunk(f["X"][0].astype(float), P)
+    assert np.abs(Fv - f["Fv"][0]).max() <= 1e-10 * np.abs(f["Fv"][0]).max()
+    for gi in range(2):
+        for key in ("iv", "ic", "bonds", "perms"):
+            assert np.array_equal(groups[gi][key], f["g%d/%s" % (gi, key)])
+        ref = f["g%d/out" % gi]
+        assert np.abs(outs[gi] - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_oracle_iso_conv_is_conv3d_with_the_class_kernel():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((5, 7, 6, 9))
+    w = rng.standard_normal((4, 5, 4))
+    b = rng.standard_normal(4)
+    off = np.array(np.meshgrid(*(np.arange(-1, 2),) * 3, indexing="ij")).reshape(3, -1).T
+    cls = np.abs(off).sum(1)
+    K = w[:, :, cls].reshape(4, 5, 3, 3, 3)
+    ref = F.conv3d(torch.as_tensor(x)[None], torch.as_tensor(K), torch.as_tensor(b), padding=1)[0].numpy()
+    assert np.abs(BR.iso_conv(x, w, b) - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("gidx", [9, 20, 47])
+def test_oracle_is_cube_equivariant(gidx):
+    net, P = _random_net(2, 6, 4, 16, seed=5, head_scale=0.3)
+    nc, s = 3, 2
+    _, X = _micro(nc * s, s, 7)
+    g = BR.OH[gidx]
+    cp = np.ones((nc ** 3, 10))
+    outs, groups = BR.forward(X[None], nc, s, P)
+    Kn, Tv = BR.post([o[0] for o in outs], groups, cp, nc)
+    outs_r, _ = BR.forward(_rotate(X, g)[None], nc, s, P)
+    Kn_r, Tv_r = BR.post([o[0] for o in outs_r], groups, cp, nc)
+    assert np.abs(Kn_r - _rotate_bonds(Kn, g, nc)).max() < 1e-10 * np.abs(Kn).max()
+    assert np.abs(Tv_r - _rotate_bonds(Tv, g, nc)).max() < 1e-10 * np.abs(Tv).max()
+    # the rotation is not the identity on this field: the check has something to detect
+    assert np.abs(Kn_r - Kn).max() > 1e-6
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_product_windows_match_reference(name):
+    f, _ = _fixture(name)
+    bw = net3d.BondWindows(int(f["nc"]), int(f["s"]))
+    for gi, g in enumerate(bw.groups):
+        for key in ("iv", "ic", "bonds", "perms"):
+            assert np.array_equal(g[key], f["g%d/%s" % (gi, key)])
+
+
+def test_checkpoint_names_load():
+    f, P = _fixture("bondnet_s2_full")
+    net = net3d.BondNet(2)
+    res = net.load_state_dict({k: torch.as_tensor(v, dtype=torch.float32) for k, v in P.items()}, strict=False)
+    assert not res.unexpected_keys and all(k.endswith("masks") for k in res.missing_keys)
+
+
+def test_voigt_cell_params_match_general_calcKnTv():
+    import geom3d as G
+    rng = np.random.default_rng(2)
+    n, s = 6, 2
+    Cv = G.iso_C(rng.uniform(1, 5, n ** 3), rng.uniform(0.1, 0.35, n ** 3))
+    ref = G.cell_params_from_C66(G.voigt_cells(Cv, n, s), s / 2.0)
+    assert np.abs(net3d.voigt_cell_params(Cv, n, s).numpy() - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.abs(BR.voigt_cell_params(Cv, n, s) - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_zero_last_layer_gives_voigt():
+    net, P = _random_net(2, 6, 4, 16, seed=1, head_scale=0.0)
+    nc, s = 3, 2
+    _, X = _micro(nc * s, s, 3)
+    outs, groups = BR.forward(X[None], nc, s, P)
+    cp = np.random.default_rng(0).uniform(0.5, 1.5, (nc ** 3, 10))
+    Kn, Tv = BR.post([o[0] for o in outs], groups, cp, nc)
+    import geom3d as G
+    KnV, TvV = G.bond_params_from_cells(G.LatticeGeom(nc, nc, nc, s), cp)
+    assert np.abs(Kn - KnV).max() == 0.0 and np.abs(Tv - TvV).max() < 1e-15
+
+
+# -------------------------------------------------------------------------------------------------------- GPU
+@pytest.mark.gpu
+@pytest.mark.parametrize("cin,cout,dims,gelu,proj", [(3, 24, (9, 70, 5), True, 0), (24, 24, (13, 5, 66), True, 0),
+                                                     (8, 6, (4, 4, 4), False, 0), (24, 24, (7, 9, 65), True, 12),
+                                                     (6, 30, (5, 6, 7), True, 5)])
+def test_gpu_iso_conv(dev, cin, cout, dims, gelu, proj):
+    rng = np.random.default_rng(cin * 100 + cout)
+    nz, ny, nx = dims
+    x = rng.standard_normal((2, cin, nz, ny, nx)).astype(np.float32)
+    w = (rng.standard_normal((cout, cin, 4)) * 0.2).astype(np.float32)
+    b = rng.standard_normal(cout).astype(np.float32)
+    refs = []
+    for xb in x.astype(float):
+        r = BR.iso_conv(xb, w.astype(float), b.astype(float))
+        refs.append(BR.gelu(r) if gelu else r)
+    ref = np.stack(refs)
+    wt, bt = torch.as_tensor(w, device=dev), torch.as_tensor(b, device=dev)
+    if proj:
+        pw = (rng.standard_normal((proj, cout)) * 0.3).astype(np.float32)
+        pb = rng.standard_normal(proj).astype(np.float32)
+        ref = np.einsum("dc,bczyx->bzyxd", pw, ref) + pb
+        y = net3d.iso_conv(torch.as_tensor(x, device=dev), wt, bt, gelu=gelu,
+                           proj=(torch.as_tensor(pw, device=dev), torch.as_tensor(pb, device=dev)), dstride=16)
+        y = y.cpu().numpy().reshape(2, nz, ny, nx, 16)
+        assert np.all(y[..., proj:] == 0)
+        y = y[..., :proj]
+    else:
+        y = net3d.iso_conv(torch.as_tensor(x, device=dev), wt, bt, gelu=gelu).cpu().numpy()
+    assert np.abs(y - ref).max() <= TOL32 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_forward_matches_reference_fixture(dev, name):
+    f, P = _fixture(name)
+    net = _net_from(P, int(f["s"]), int(f["ch"]), int(f["D"]), int(f["hid"]), dev)
+    outs = net(torch.as_tensor(f["X"], device=dev), int(f["nc"]))
+    for gi in range(2):
+        ref = f["g%d/out" % gi]
+        got = outs[gi].cpu().numpy()
+        assert got.shape == ref.shape
+        assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max(), (gi, np.abs(got - ref).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nc,s,B", [(7, 2, 2), (5, 3, 1)])
+def test_gpu_bond_params_match_oracle(dev, nc, s, B):
+    """Kn / Tv of the fused heads + post against oracle forward + post; bond counts that are not multiples of the
+    16-bond block and blocks that straddle two directions."""
+    net, P = _random_net(s, 8, 4, 32, seed=nc)
+    n = nc * s
+    Xs, cps = [], []
+    import geom3d as G
+    for b in range(B):
+        phase, X = _micro(n, s, 10 + b, contrast=3.0 + b)
+        Xs.append(X)
+        Cv = G.iso_C(np.where(phase == 1, 3.0 + b, 1.0), np.where(phase == 1, 0.3, 0.2))
+        cps.append(net3d.voigt_cell_params(Cv, n, s).numpy())
+    X = np.stack(Xs)
+    Kn, Tv = net.to(dev).bond_params(torch.as_tensor(X, device=dev), nc, torch.as_tensor(np.stack(cps), device=dev))
+    outs, groups = BR.forward(X, nc, s, P)
+    for b in range(B):
+        KnR, TvR = BR.post([o[b] for o in outs], groups, cps[b], nc)
+        assert np.abs(Kn[b].cpu().numpy() - KnR).max() <= TOL32 * 10 * np.abs(KnR).max()
+        assert np.abs(Tv[b].cpu().numpy() - TvR).max() <= TOL32 * 10 * np.abs(KnR).max()
+        # slots leaving the box stay zero, every in-box slot is set (Kn > 0)
+        lat = G.LatticeGeom(nc, nc, nc, s)
+        assert np.all((Kn[b].cpu().numpy() > 0) == lat.valid)
+
+
+@pytest.mark.gpu
+def test_gpu_forward_is_cube_equivariant(dev):
+    net, _ = _random_net(2, 24, 12, 64, seed=3, head_scale=0.3)
+    net = net.to(dev)
+    nc, s = 6, 2
+    _, X = _micro(nc * s, s, 9)
+    cp = torch.ones((nc ** 3, 10), device=dev, dtype=torch.float64)
+    Kn, _ = net.bond_params(torch.as_tensor(X[None], device=dev), nc, cp)
+    g = BR.OH[21]
+    Kn_r, _ = net.bond_params(torch.as_tensor(_rotate(X, g)[None].copy(), device=dev), nc, cp)
+    A = Kn[0].cpu().numpy()
+    assert np.abs(Kn_r[0].cpu().numpy() - _rotate_bonds(A, g, nc)).max() <= 1e-4 * np.abs(A).max()
