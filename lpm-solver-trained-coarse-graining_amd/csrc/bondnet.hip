@@ -180,11 +180,14 @@ __global__ void cell_pool_kernel(int B, int ncp, int s, int ds, const float* __r
 // (bond, stabiliser element) pairs, K = nslot * dstride window features and N = hid (padded to NT = 1, 2 or 4
 // column tiles of 16): row (b, g) reads window slot inv[g, j] where W1 expects slot j.  Four waves, each RT = H / 4
 // row tiles of 16 rows and all NT column tiles, so that one k-step of 4 costs RT LDS reads (A) and NT global loads
-// (B) for RT x NT MFMAs; the fragments of the next k-step are loaded before the MFMAs of the current one.
+// (B) for RT x NT MFMAs; two k-steps of fragments are in flight ahead of the MFMAs that use them.
 // v_mfma_f32_16x16x4f32 lane layout: A[lane % 16][lane / 16], B[lane / 16][lane % 16], D[4 (lane / 16) + i][lane % 16].
 // W1 comes pre-arranged (lpm_bond_w1_layout) so that every B fragment is one coalesced 256-byte load.
 constexpr int HB = 16;
 constexpr int HMAX = 64;
+
+// k-steps of the first layer rounded up to even (the loop runs two steps per iteration)
+__host__ __device__ inline int kpad_of(int ksteps) { return ksteps + (ksteps & 1); }
 
 struct HeadArgs {
   int ndir, nv, ncell, nslot, H, D, ds, hid, Q, xs_floats;
@@ -287,39 +290,43 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
     grow[t] = (row % H) * nslot;
   }
   const float* w1l = a.w1f + lane;
-  const int ksteps = nslot * a.Q;
-  int slot = 0, j = 0;
+  // k-steps rounded up to even; W1 holds zero fragments for the steps past nslot * Q and two more beyond, so the
+  // loop has no conditional load (A reads past the last slot repeat the last slot's offsets, in bounds)
+  const int ksteps = kpad_of(nslot * a.Q);
+  // load cursor: k-step lk = (lslot, lj); two fragment sets in flight (steps kk and kk + 1)
+  int lk = 0, lslot = 0, lj = 0;
   int aoff[RT];
 #pragma unroll
   for (int t = 0; t < RT; ++t) aoff[t] = xrow[t] + inv_s[grow[t]] * ds;
-  float bf[NT], af[RT];
+  auto load = [&](float (&bb)[NT], float (&aa)[RT]) {
+    const float* wk = w1l + (int64_t)lk * NT * 64;
 #pragma unroll
-  for (int n = 0; n < NT; ++n) bf[n] = w1l[n * 64];
+    for (int n = 0; n < NT; ++n) bb[n] = wk[n * 64];
 #pragma unroll
-  for (int t = 0; t < RT; ++t) af[t] = xs[aoff[t]];
-  for (int kk = 0; kk < ksteps; ++kk) {
-    float bc[NT], ac[RT];
+    for (int t = 0; t < RT; ++t) aa[t] = xs[aoff[t] + 4 * lj];
+    ++lk;
+    if (++lj == a.Q) {
+      lj = 0;
+      if (++lslot < nslot) {
 #pragma unroll
-    for (int n = 0; n < NT; ++n) bc[n] = bf[n];
-#pragma unroll
-    for (int t = 0; t < RT; ++t) ac[t] = af[t];
-    if (kk + 1 < ksteps) {
-      if (++j == a.Q) {
-        j = 0;
-        ++slot;
-#pragma unroll
-        for (int t = 0; t < RT; ++t) aoff[t] = xrow[t] + inv_s[grow[t] + slot] * ds;
+        for (int t = 0; t < RT; ++t) aoff[t] = xrow[t] + inv_s[grow[t] + lslot] * ds;
       }
-      const float* wk = w1l + (int64_t)(kk + 1) * NT * 64;
-#pragma unroll
-      for (int n = 0; n < NT; ++n) bf[n] = wk[n * 64];
-#pragma unroll
-      for (int t = 0; t < RT; ++t) af[t] = xs[aoff[t] + 4 * j];
     }
+  };
+  auto step = [&](const float (&bb)[NT], const float (&aa)[RT]) {
 #pragma unroll
     for (int t = 0; t < RT; ++t)
 #pragma unroll
-      for (int n = 0; n < NT; ++n) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t], bc[n], acc[t][n], 0, 0, 0);
+      for (int n = 0; n < NT; ++n) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(aa[t], bb[n], acc[t][n], 0, 0, 0);
+  };
+  float fb0[NT], fa0[RT], fb1[NT], fa1[RT];
+  load(fb0, fa0);
+  load(fb1, fa1);
+  for (int kk = 0; kk < ksteps; kk += 2) {
+    step(fb0, fa0);
+    load(fb0, fa0);
+    step(fb1, fa1);
+    load(fb1, fa1);
   }
   __syncthreads();  // xs is reused below
 
@@ -385,7 +392,7 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
 
 __global__ void w1_layout_kernel(int nslot, int D, int Q, int NT, int hid, const float* __restrict__ w1,
                                  float* __restrict__ w1f) {
-  const int64_t total = (int64_t)nslot * Q * NT * 64;
+  const int64_t total = (int64_t)(kpad_of(nslot * Q) + 2) * NT * 64;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= total) return;
   const int lane = (int)(idx & 63);
@@ -395,7 +402,7 @@ __global__ void w1_layout_kernel(int nslot, int D, int Q, int NT, int hid, const
   const int j = (int)(r % Q);
   const int slot = (int)(r / Q);
   const int row = n * 16 + (lane & 15), d = 4 * j + (lane >> 4);
-  w1f[idx] = (row < hid && d < D) ? w1[(int64_t)row * nslot * D + (int64_t)slot * D + d] : 0.0f;
+  w1f[idx] = (slot < nslot && row < hid && d < D) ? w1[(int64_t)row * nslot * D + (int64_t)slot * D + d] : 0.0f;
 }
 
 // column tiles of the first layer: 1, 2 or 4 (hidden widths above 32 are padded to 64 with zero weights)
@@ -479,14 +486,14 @@ int lpm_cell_pool(int32_t B, int32_t ncp, int32_t s, int32_t dstride, const floa
 
 size_t lpm_bond_w1_size(const lpm_bond_group* g) {
   if (!g || g->dstride < 4 || g->hid < 1) return 0;
-  return (size_t)(g->nv + g->ncell) * (g->dstride / 4) * col_tiles(g->hid) * 64 * sizeof(float);
+  return (size_t)(kpad_of((g->nv + g->ncell) * (g->dstride / 4)) + 2) * col_tiles(g->hid) * 64 * sizeof(float);
 }
 
 int lpm_bond_w1_layout(const lpm_bond_group* g, const float* w1, float* w1f, void* stream) {
   if (int rc = check_group(g)) return rc;
   if (!w1 || !w1f) return fail(LPM_E_BADARG, "lpm_bond_w1_layout: null weights");
   const int nslot = g->nv + g->ncell, Q = g->dstride / 4, NT = col_tiles(g->hid);
-  const int64_t n = (int64_t)nslot * Q * NT * 64;
+  const int64_t n = (int64_t)(kpad_of(nslot * Q) + 2) * NT * 64;
   w1_layout_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(nslot, g->D, Q, NT, g->hid, w1, w1f);
   LAUNCHCHK();
   return LPM_OK;
