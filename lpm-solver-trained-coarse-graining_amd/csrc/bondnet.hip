@@ -180,7 +180,7 @@ __global__ void cell_pool_kernel(int B, int ncp, int s, int ds, const float* __r
 // (bond, stabiliser element) pairs, K = nslot * dstride window features and N = hid (padded to NT = 1, 2 or 4
 // column tiles of 16): row (b, g) reads window slot inv[g, j] where W1 expects slot j.  Four waves, each RT = H / 4
 // row tiles of 16 rows and all NT column tiles, so that one k-step of 4 costs RT LDS reads (A) and NT global loads
-// (B) for RT x NT MFMAs; two k-steps of fragments are in flight ahead of the MFMAs that use them.
+// (B) for RT x NT MFMAs; the fragments of two window slots are in flight ahead of the MFMAs that use them.
 // v_mfma_f32_16x16x4f32 lane layout: A[lane % 16][lane / 16], B[lane / 16][lane % 16], D[4 (lane / 16) + i][lane % 16].
 // W1 comes pre-arranged (lpm_bond_w1_layout) so that every B fragment is one coalesced 256-byte load.
 constexpr int HB = 16;
@@ -216,10 +216,10 @@ struct BondMeta {
   int dir, vbase, cbase, p, q, k;
 };
 
-template <int H, int NT>
+template <int H, int NT, int Q>
 __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
   extern __shared__ __align__(16) float smem[];
-  static_assert((H == 8 || H == 16) && (NT == 1 || NT == 2 || NT == 4), "tile shape");
+  static_assert((H == 8 || H == 16) && (NT == 1 || NT == 2 || NT == 4) && Q >= 1 && Q <= 4, "tile shape");
   constexpr int RT = H / 4;
   const int nslot = a.nslot, ds = a.ds;
   float* xs = smem;                                             // [HB][nslot][ds]
@@ -281,52 +281,55 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
   for (int t = 0; t < RT; ++t)
 #pragma unroll
     for (int n = 0; n < NT; ++n) acc[t][n] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-  // rows of tile t: wave * RT * 16 + t * 16 + lane % 16 -> bond row / H, stabiliser element row % H
+  // rows of tile t: wave * RT * 16 + t * 16 + lane % 16 -> bond row / H, stabiliser element row % H; within a
+  // window slot, k-step j of lane group c = lane / 16 is channel c Q + j, so a lane's Q values of a slot are
+  // contiguous in LDS (A) and in the fragment-ordered W1 (B: one Q-wide load per column tile)
   int xrow[RT], grow[RT];
 #pragma unroll
   for (int t = 0; t < RT; ++t) {
     const int row = wave * RT * 16 + t * 16 + (lane & 15);
-    xrow[t] = (row / H) * nslot * ds + (lane >> 4);
+    xrow[t] = (row / H) * nslot * ds + (lane >> 4) * Q;
     grow[t] = (row % H) * nslot;
   }
-  const float* w1l = a.w1f + lane;
-  // k-steps rounded up to even; W1 holds zero fragments for the steps past nslot * Q and two more beyond, so the
-  // loop has no conditional load (A reads past the last slot repeat the last slot's offsets, in bounds)
-  const int ksteps = kpad_of(nslot * a.Q);
-  // load cursor: k-step lk = (lslot, lj); two fragment sets in flight (steps kk and kk + 1)
-  int lk = 0, lslot = 0, lj = 0;
-  int aoff[RT];
+  const float* w1l = a.w1f + lane * Q;
+  // slots rounded up to even; W1 holds zero fragments for the slots past nslot and two more beyond, and A reads
+  // past the last slot repeat it, so the loop has no conditional load
+  const int nsl = kpad_of(nslot);
+  int lsl = 0;
+  auto load = [&](float (&bb)[NT][Q], float (&aa)[RT][Q]) {
+    const float* wk = w1l + (int64_t)lsl * NT * 64 * Q;
 #pragma unroll
-  for (int t = 0; t < RT; ++t) aoff[t] = xrow[t] + inv_s[grow[t]] * ds;
-  auto load = [&](float (&bb)[NT], float (&aa)[RT]) {
-    const float* wk = w1l + (int64_t)lk * NT * 64;
+    for (int n = 0; n < NT; ++n)
 #pragma unroll
-    for (int n = 0; n < NT; ++n) bb[n] = wk[n * 64];
+      for (int j = 0; j < Q; ++j) bb[n][j] = wk[n * 64 * Q + j];
+    const int sl = lsl < nslot ? lsl : nslot - 1;
 #pragma unroll
-    for (int t = 0; t < RT; ++t) aa[t] = xs[aoff[t] + 4 * lj];
-    ++lk;
-    if (++lj == a.Q) {
-      lj = 0;
-      if (++lslot < nslot) {
+    for (int t = 0; t < RT; ++t) {
+      const int base = xrow[t] + inv_s[grow[t] + sl] * ds;
 #pragma unroll
-        for (int t = 0; t < RT; ++t) aoff[t] = xrow[t] + inv_s[grow[t] + lslot] * ds;
-      }
+      for (int j = 0; j < Q; ++j) aa[t][j] = xs[base + j];
     }
+    ++lsl;
   };
-  auto step = [&](const float (&bb)[NT], const float (&aa)[RT]) {
+  auto step = [&](const float (&bb)[NT][Q], const float (&aa)[RT][Q]) {
 #pragma unroll
-    for (int t = 0; t < RT; ++t)
+    for (int j = 0; j < Q; ++j)
 #pragma unroll
-      for (int n = 0; n < NT; ++n) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(aa[t], bb[n], acc[t][n], 0, 0, 0);
+      for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+          acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(aa[t][j], bb[n][j], acc[t][n], 0, 0, 0);
   };
-  float fb0[NT], fa0[RT], fb1[NT], fa1[RT];
+  float fb0[NT][Q], fa0[RT][Q], fb1[NT][Q], fa1[RT][Q];
   load(fb0, fa0);
   load(fb1, fa1);
-  for (int kk = 0; kk < ksteps; kk += 2) {
+  for (int sl = 0; sl < nsl; sl += 2) {
     step(fb0, fa0);
     load(fb0, fa0);
+    __builtin_amdgcn_sched_barrier(0);  // keep each slot's loads between the two halves
     step(fb1, fa1);
     load(fb1, fa1);
+    __builtin_amdgcn_sched_barrier(0);
   }
   __syncthreads();  // xs is reused below
 
@@ -390,18 +393,19 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
   }
 }
 
+// fragment order [slot][column tile][lane][j]: lane's k-step j of a slot is channel (lane / 16) Q + j
 __global__ void w1_layout_kernel(int nslot, int D, int Q, int NT, int hid, const float* __restrict__ w1,
                                  float* __restrict__ w1f) {
-  const int64_t total = (int64_t)(kpad_of(nslot * Q) + 2) * NT * 64;
+  const int64_t total = (int64_t)(kpad_of(nslot) + 2) * NT * 64 * Q;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= total) return;
-  const int lane = (int)(idx & 63);
-  int64_t r = idx >> 6;
+  const int j = (int)(idx % Q);
+  int64_t r = idx / Q;
+  const int lane = (int)(r & 63);
+  r >>= 6;
   const int n = (int)(r % NT);
-  r /= NT;
-  const int j = (int)(r % Q);
-  const int slot = (int)(r / Q);
-  const int row = n * 16 + (lane & 15), d = 4 * j + (lane >> 4);
+  const int slot = (int)(r / NT);
+  const int row = n * 16 + (lane & 15), d = (lane >> 4) * Q + j;
   w1f[idx] = (slot < nslot && row < hid && d < D) ? w1[(int64_t)row * nslot * D + (int64_t)slot * D + d] : 0.0f;
 }
 
@@ -439,20 +443,30 @@ constexpr int SC_OFF[18][3] = {{1, 0, 0},  {-1, 0, 0},  {0, 1, 0},  {0, -1, 0}, 
                                {1, 1, 0},  {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 0, 1},  {-1, 0, -1},
                                {1, 0, -1}, {-1, 0, 1},  {0, 1, 1},  {0, -1, -1}, {0, 1, -1}, {0, -1, 1}};
 
-template <int H, int NT>
+template <int H, int NT, int Q>
 int launch_heads(const HeadArgs& a, size_t lds, hipStream_t st) {
   dim3 grid((unsigned)((a.nb + HB - 1) / HB), (unsigned)a.B);
-  bond_heads_kernel<H, NT><<<grid, 256, lds, st>>>(a);
+  bond_heads_kernel<H, NT, Q><<<grid, 256, lds, st>>>(a);
   LAUNCHCHK();
   return LPM_OK;
+}
+
+template <int H, int NT>
+int launch_heads_q(const HeadArgs& a, size_t lds, hipStream_t st) {
+  switch (a.Q) {
+    case 1: return launch_heads<H, NT, 1>(a, lds, st);
+    case 2: return launch_heads<H, NT, 2>(a, lds, st);
+    case 3: return launch_heads<H, NT, 3>(a, lds, st);
+    default: return launch_heads<H, NT, 4>(a, lds, st);
+  }
 }
 
 template <int H>
 int launch_heads_nt(const HeadArgs& a, size_t lds, hipStream_t st) {
   const int nt = (a.hid + 15) / 16;
-  if (nt == 1) return launch_heads<H, 1>(a, lds, st);
-  if (nt == 2) return launch_heads<H, 2>(a, lds, st);
-  return launch_heads<H, 4>(a, lds, st);
+  if (nt == 1) return launch_heads_q<H, 1>(a, lds, st);
+  if (nt == 2) return launch_heads_q<H, 2>(a, lds, st);
+  return launch_heads_q<H, 4>(a, lds, st);
 }
 
 }  // namespace
@@ -486,14 +500,14 @@ int lpm_cell_pool(int32_t B, int32_t ncp, int32_t s, int32_t dstride, const floa
 
 size_t lpm_bond_w1_size(const lpm_bond_group* g) {
   if (!g || g->dstride < 4 || g->hid < 1) return 0;
-  return (size_t)(kpad_of((g->nv + g->ncell) * (g->dstride / 4)) + 2) * col_tiles(g->hid) * 64 * sizeof(float);
+  return (size_t)(kpad_of(g->nv + g->ncell) + 2) * col_tiles(g->hid) * 64 * (g->dstride / 4) * sizeof(float);
 }
 
 int lpm_bond_w1_layout(const lpm_bond_group* g, const float* w1, float* w1f, void* stream) {
   if (int rc = check_group(g)) return rc;
   if (!w1 || !w1f) return fail(LPM_E_BADARG, "lpm_bond_w1_layout: null weights");
   const int nslot = g->nv + g->ncell, Q = g->dstride / 4, NT = col_tiles(g->hid);
-  const int64_t n = (int64_t)(kpad_of(nslot * Q) + 2) * NT * 64;
+  const int64_t n = (int64_t)(kpad_of(nslot) + 2) * NT * 64 * Q;
   w1_layout_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(nslot, g->D, Q, NT, g->hid, w1, w1f);
   LAUNCHCHK();
   return LPM_OK;
