@@ -234,3 +234,29 @@ def test_gpu_forward_is_cube_equivariant(dev):
     Kn_r, _ = net.bond_params(torch.as_tensor(_rotate(X, g)[None].copy(), device=dev), nc, cp)
     A = Kn[0].cpu().numpy()
     assert np.abs(Kn_r[0].cpu().numpy() - _rotate_bonds(A, g, nc)).max() <= 1e-4 * np.abs(A).max()
+
+
+@pytest.mark.gpu
+def test_gpu_inferred_parameters_feed_the_solve(dev):
+    """The network's Kn / Tv of a 6^3 coarse lattice (h = s) go straight into the GPU solve (train3d.py:100-108 feeds
+    CoarseT the same way), which matches SuperLU on the oracle's K built from the same parameters."""
+    import geom3d
+    import lpm_ref as R
+    import torch_lpm3d as T3
+    nc, s = 6, 2
+    net, _ = _random_net(s, 8, 4, 32, seed=4)
+    phase, X = _micro(nc * s, s, 21)
+    Cv = geom3d.iso_C(np.where(phase == 1, 4.0, 1.0), np.where(phase == 1, 0.3, 0.2))
+    cp = net3d.voigt_cell_params(Cv, nc * s, s)
+    Kn, Tv = net.to(dev).bond_params(torch.as_tensor(X[None], device=dev), nc, cp.to(dev))
+    Kn, Tv = Kn[0], Tv[0]
+    dims, h = (nc, nc, nc), float(s)
+    lat = R.Box(dims, h)
+    fixed, up, _ = R.prescribed3("k_xx", lat.x, lat.ijk, dims, 1, h)
+    ud = R.solve_direct(R.stiffness(lat, Kn.cpu().numpy(), Tv.cpu().numpy()), np.repeat(fixed, 3), up)
+    op = T3.SCT(geom3d.LatticeGeom(*dims, h), device=dev)
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    u, it, relres = op.solve(Kn, Tv, t(up), t((~fixed).astype(float)), tol=1e-12)
+    ud = ud.reshape(lat.N, 3)
+    assert relres <= 1e-12
+    assert np.abs(u.cpu().numpy() - ud).max() <= 1e-9 * np.abs(ud).max()
