@@ -295,9 +295,9 @@ typedef struct {
   int32_t D, dstride;   /* feature channels and their stride in fv / fc (a multiple of 4, <= 16) */
   int32_t hid;          /* hidden width, <= 64 */
   int32_t slot[6];      /* forward slot k (even, 0..16) of every direction, in bond order */
-  const int32_t* voff;  /* device (ndir, nv) */
-  const int32_t* coff;  /* device (ndir, ncell) */
-  const int32_t* inv;   /* device (H, nv + ncell) */
+  const int32_t* voff;  /* device (ndir, nv); every window must stay inside the padded voxel grid */
+  const int32_t* coff;  /* device (ndir, ncell); every window must stay inside the padded cell grid */
+  const int32_t* inv;   /* device (H, nv + ncell), entries in [0, nv + ncell): the inverse stabiliser perms */
 } lpm_bond_group;
 size_t lpm_bond_w1_size(const lpm_bond_group* grp);
 int lpm_bond_w1_layout(const lpm_bond_group* grp, const float* w1, float* w1f, void* stream);

@@ -340,3 +340,45 @@ def voigt_cell_params(Cvox, n, s):
     kax = (r * (C[:, 0] - 2 * C[:, 2] - T))[:, None].expand(-1, 3)
     kdg = (r * C[:, 2])[:, None].expand(-1, 6)
     return torch.cat([kax, kdg, (r * T / 18.0)[:, None]], 1).contiguous()
+
+
+class Post:
+    """Voigt bond bases of the bond list and the scatter of raw outputs to per-slot Kn / Tv, on the GPU, with the
+    interface of timing/t3d/infer_fast.py:150-177: `bases(Cv)` -> per group (Kn_V, Tv_V) of every listed bond,
+    `scatter(outs, base)` -> Kn, Tv (nc^3, 18) float64 with Kn = Kn_V exp(o1), Tv = Tv_V + Kn_V o2 on both directed
+    slots (raw outputs of specimen 0, as the reference takes).  `BondNet.bond_params` does the same inside the heads
+    kernel."""
+
+    def __init__(self, n, s, net, device=None):
+        self.n, self.s, self.nc = n, s, n // s
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        nc = self.nc
+        self.bonds, self.q = [], []
+        for g in net.windows(nc):
+            b = g["bonds"]
+            ijk = np.stack([b[:, 0] % nc, (b[:, 0] // nc) % nc, b[:, 0] // (nc * nc)], 1) + SC_OFFS[b[:, 1]]
+            q = (ijk[:, 2] * nc + ijk[:, 1]) * nc + ijk[:, 0]
+            self.bonds.append(torch.as_tensor(b, device=self.device))
+            self.q.append(torch.as_tensor(q, device=self.device))
+
+    def bases(self, Cv):
+        cp = voigt_cell_params(Cv, self.n, self.s).to(self.device)
+        out = []
+        for b, q in zip(self.bonds, self.q):
+            p, d = b[:, 0], b[:, 1] // 2
+            out.append((0.5 * (cp[p, d] + cp[q, d]), 0.5 * (cp[p, 9] + cp[q, 9])))
+        return out
+
+    def scatter(self, outs, base):
+        N = self.nc ** 3
+        Kn = torch.zeros((N, 18), device=self.device, dtype=torch.float64)
+        Tv = torch.zeros_like(Kn)
+        for b, q, o, (kv, tv) in zip(self.bonds, self.q, outs, base):
+            o = o[0].to(self.device, torch.float64)
+            kn = kv * torch.exp(o[:, 0])
+            tt = tv + kv * o[:, 1]
+            Kn[b[:, 0], b[:, 1]] = kn
+            Kn[q, b[:, 1] ^ 1] = kn
+            Tv[b[:, 0], b[:, 1]] = tt
+            Tv[q, b[:, 1] ^ 1] = tt
+        return Kn, Tv

@@ -260,3 +260,19 @@ def test_gpu_inferred_parameters_feed_the_solve(dev):
     ud = ud.reshape(lat.N, 3)
     assert relres <= 1e-12
     assert np.abs(u.cpu().numpy() - ud).max() <= 1e-9 * np.abs(ud).max()
+
+
+@pytest.mark.gpu
+def test_gpu_post_matches_fused_bond_params(dev):
+    """infer_fast.Post's two-step interface (bases, scatter of the raw outputs) equals the fused heads + post."""
+    import geom3d
+    nc, s = 5, 2
+    net, _ = _random_net(s, 8, 4, 32, seed=8)
+    net = net.to(dev)
+    phase, X = _micro(nc * s, s, 31)
+    Cv = geom3d.iso_C(np.where(phase == 1, 4.0, 1.0), np.where(phase == 1, 0.3, 0.2))
+    Xd = torch.as_tensor(X[None], device=dev)
+    post = net3d.Post(nc * s, s, net, device=dev)
+    Kn, Tv = post.scatter(net(Xd, nc), post.bases(Cv))
+    Kf, Tf = net.bond_params(Xd, nc, net3d.voigt_cell_params(Cv, nc * s, s).to(dev))
+    assert torch.allclose(Kn, Kf[0], rtol=1e-12, atol=0) and torch.allclose(Tv, Tf[0], rtol=1e-12, atol=1e-15)
