@@ -179,15 +179,17 @@ __global__ void cell_pool_kernel(int B, int ncp, int s, int ds, const float* __r
 // A block takes HB = 16 consecutive bonds of one group.  The first layer is a GEMM whose rows are the HB * H
 // (bond, stabiliser element) pairs, K = nslot * dstride window features and N = hid (padded to NT = 1, 2 or 4
 // column tiles of 16): row (b, g) reads window slot inv[g, j] where W1 expects slot j.  Four waves, each RT = H / 4
-// row tiles of 16 rows and all NT column tiles, so that one k-step of 4 costs RT LDS reads (A) and NT global loads
-// (B) for RT x NT MFMAs; the fragments of two window slots are in flight ahead of the MFMAs that use them.
+// row tiles of 16 rows and all NT column tiles.  A window slot is Q = dstride / 4 k-steps of 4; k-step j of lane
+// group c = lane / 16 is channel c Q + j, so per slot a lane makes RT x Q LDS reads (A) and NT Q-wide global loads
+// (B) for RT x NT x Q MFMAs, and the fragments of two slots are in flight ahead of the MFMAs that use them.
 // v_mfma_f32_16x16x4f32 lane layout: A[lane % 16][lane / 16], B[lane / 16][lane % 16], D[4 (lane / 16) + i][lane % 16].
-// W1 comes pre-arranged (lpm_bond_w1_layout) so that every B fragment is one coalesced 256-byte load.
+// W1 comes pre-arranged (lpm_bond_w1_layout, [slot][column tile][lane][j]) so that a wave's B load of one column
+// tile and slot is one contiguous 64 x Q x 4-byte run.
 constexpr int HB = 16;
 constexpr int HMAX = 64;
 
-// k-steps of the first layer rounded up to even (the loop runs two steps per iteration)
-__host__ __device__ inline int kpad_of(int ksteps) { return ksteps + (ksteps & 1); }
+// window slots rounded up to even (the first-layer loop runs two slots per iteration)
+__host__ __device__ inline int kpad_of(int nslot) { return nslot + (nslot & 1); }
 
 struct HeadArgs {
   int ndir, nv, ncell, nslot, H, D, ds, hid, Q, xs_floats;
