@@ -542,6 +542,11 @@ def bench_patch(a):
     print(json.dumps(out), flush=True)
 
 
+# HBM bytes per launch of the face-diagonal heads at n = 256 (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, KiB per
+# dispatch, scaled by what the same counters read for a copy kernel of known size: x2.0 and x1.0)
+HEADS_DIAG_TRAFFIC_256 = 4591490 * 1024 * 2.0 + 1399950 * 1024 * 1.0
+
+
 def _bondnet_setup(n, s, seed):
     """Two-phase voxel field (fraction 0.4, contrast 4, nu 0.3 / 0.2, seeded), its voxel features and per-cell Voigt
     parameters, and a bond network of the reference architecture (ch 24, D 12, hid 64) with random weights (last
@@ -637,7 +642,8 @@ def bench_infer(a):
            "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
            "stages_ms": {"trunk_and_pool": feat_ms, "heads_axial": head_ms[0], "heads_diagonal": head_ms[1]},
            "roofline": {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-                        "traffic": None, "kernel": "bond_heads_kernel<2,4> (face-diagonal group)",
+                        "traffic": HEADS_DIAG_TRAFFIC_256 if n == 256 else None,
+                        "kernel": "bond_heads_kernel<8,4,3> (face-diagonal group)",
                         "alg_flops_per_launch": flops[1], "avg_us": head_ms[1] * 1e3,
                         "axial": {"alg_flops_per_launch": flops[0], "avg_us": head_ms[0] * 1e3,
                                   "achieved": flops[0] / (head_ms[0] * 1e-3) / 1e12}}}
