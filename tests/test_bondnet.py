@@ -276,3 +276,19 @@ def test_gpu_post_matches_fused_bond_params(dev):
     Kn, Tv = post.scatter(net(Xd, nc), post.bases(Cv))
     Kf, Tf = net.bond_params(Xd, nc, net3d.voigt_cell_params(Cv, nc * s, s).to(dev))
     assert torch.allclose(Kn, Kf[0], rtol=1e-12, atol=0) and torch.allclose(Tv, Tf[0], rtol=1e-12, atol=1e-15)
+
+
+def test_post_glue_matches_oracle_on_host_tensors():
+    """net3d.Post (torch glue around the kernels) scatters given raw outputs exactly as the oracle's post."""
+    import geom3d
+    nc, s = 4, 2
+    net = net3d.BondNet(s, ch=6, D=4, hid=16)
+    rng = np.random.default_rng(12)
+    Cv = geom3d.iso_C(rng.uniform(1, 4, (nc * s) ** 3), rng.uniform(0.15, 0.3, (nc * s) ** 3))
+    groups = BR.bond_windows(nc, s)
+    outs = [rng.normal(0, 0.1, (1, len(g["bonds"]), 2)) for g in groups]
+    post = net3d.Post(nc * s, s, net, device="cpu")
+    Kn, Tv = post.scatter([torch.as_tensor(o) for o in outs], post.bases(Cv))
+    KnR, TvR = BR.post([o[0] for o in outs], groups, BR.voigt_cell_params(Cv, nc * s, s), nc)
+    assert np.abs(Kn.numpy() - KnR).max() <= 1e-14 * np.abs(KnR).max()
+    assert np.abs(Tv.numpy() - TvR).max() <= 1e-14 * np.abs(KnR).max()
