@@ -226,7 +226,9 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
   const int nslot = a.nslot, ds = a.ds;
   float* xs = smem;                                             // [HB][nslot][ds]
   int* inv_s = (int*)(xs + a.xs_floats);                        // [H][nslot]
-  BondMeta* meta = (BondMeta*)(inv_s + H * nslot);              // [HB]
+  int* voff_s = inv_s + H * nslot;                              // [ndir][nv]
+  int* coff_s = voff_s + a.ndir * a.nv;                         // [ndir][ncell]
+  BondMeta* meta = (BondMeta*)(coff_s + a.ndir * a.ncell);      // [HB]
   float* hbuf = xs;                                             // [HB][HMAX], after the GEMM
   float* h2buf = xs + HB * HMAX;                                // [HB][HMAX]
   float* obuf = xs + 2 * HB * HMAX;                             // [HB][2]
@@ -255,25 +257,40 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
     meta[tid] = m;
   }
   for (int i = tid; i < H * nslot; i += 256) inv_s[i] = a.inv[i];
+  for (int i = tid; i < a.ndir * a.nv; i += 256) voff_s[i] = a.voff[i];
+  for (int i = tid; i < a.ndir * a.ncell; i += 256) coff_s[i] = a.coff[i];
   __syncthreads();
 
-  // window gather: HB x nslot x (ds / 4) float4
+  // window gather: HB x nslot x (ds / 4) float4, GB independent loads in flight per thread
   {
+    constexpr int GB = 8;
     const int64_t nvox = (int64_t)a.npd * a.npd * a.npd, ncel = (int64_t)a.ncp * a.ncp * a.ncp;
     const float* fvb = a.fv + (int64_t)bi * nvox * ds;
     const float* fcb = a.fc + (int64_t)bi * ncel * ds;
     const int total = HB * nslot * a.Q;
-    for (int idx = tid; idx < total; idx += 256) {
-      const int c = idx % a.Q, r = idx / a.Q;
-      const int slot = r % nslot, bl = r / nslot;
-      const BondMeta m = meta[bl];
-      f4 v = {0.0f, 0.0f, 0.0f, 0.0f};
-      if (m.dir >= 0) {
-        const float* src = slot < a.nv ? fvb + (int64_t)(m.vbase + a.voff[m.dir * a.nv + slot]) * ds
-                                       : fcb + (int64_t)(m.cbase + a.coff[m.dir * a.ncell + slot - a.nv]) * ds;
-        v = *(const f4*)(src + 4 * c);
+    for (int i0 = 0; i0 < total; i0 += 256 * GB) {
+      f4 v[GB];
+      int dst[GB];
+#pragma unroll
+      for (int u = 0; u < GB; ++u) {
+        const int idx = i0 + u * 256 + tid;
+        v[u] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        dst[u] = -1;
+        if (idx < total) {
+          const int c = idx % a.Q, r = idx / a.Q;
+          const int slot = r % nslot, bl = r / nslot;
+          const BondMeta m = meta[bl];
+          dst[u] = (bl * nslot + slot) * ds + 4 * c;
+          if (m.dir >= 0) {
+            const float* src = slot < a.nv ? fvb + (int64_t)(m.vbase + voff_s[m.dir * a.nv + slot]) * ds
+                                           : fcb + (int64_t)(m.cbase + coff_s[m.dir * a.ncell + slot - a.nv]) * ds;
+            v[u] = *(const f4*)(src + 4 * c);
+          }
+        }
       }
-      *(f4*)(xs + (bl * nslot + slot) * ds + 4 * c) = v;
+#pragma unroll
+      for (int u = 0; u < GB; ++u)
+        if (dst[u] >= 0) *(f4*)(xs + dst[u]) = v[u];
     }
   }
   __syncthreads();
@@ -437,7 +454,8 @@ int head_xs_floats(const lpm_bond_group* g) {
 }
 
 size_t head_lds(const lpm_bond_group* g) {
-  return head_xs_floats(g) * sizeof(float) + (size_t)g->H * (g->nv + g->ncell) * sizeof(int) + HB * sizeof(BondMeta);
+  return head_xs_floats(g) * sizeof(float) + (size_t)g->H * (g->nv + g->ncell) * sizeof(int) +
+         (size_t)g->ndir * (g->nv + g->ncell) * sizeof(int) + HB * sizeof(BondMeta);
 }
 
 // slot offsets of the SC lattice (coarse3d/geom3d.py:16-20)
