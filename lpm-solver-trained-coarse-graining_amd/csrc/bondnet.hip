@@ -17,13 +17,14 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float gelu_erf(float v) { return 0.5f * v * (1.0f + erff(v * 0.70710678118654752440f)); }
 
 // ------------------------------------------------------------------------------------------------ trunk layers
-// Block = 64 x 4 threads over (x, y), each thread 4 consecutive z voxels; per input channel the (4+2) x (4+2) x
-// (64+2) zero-padded neighbourhood is staged in LDS (the next channel's values are loaded into registers while the
-// current one is mixed).  Per plane a thread forms its in-plane centre c, face f (4 neighbours) and corner k (4
+// Block = 64 x 4 threads over (x, y), each thread CT_Z = 2 consecutive z voxels (two accumulator sets of COP: 4
+// waves per SIMD with 24 channels; four voxels measured slower at 2 waves per SIMD); per input channel the
+// (CT_Z+2) x (4+2) x (64+2) zero-padded neighbourhood is staged in LDS (the next channel's values are loaded into
+// registers while the current one is mixed).  Per plane a thread forms its in-plane centre c, face f (4 neighbours) and corner k (4
 // diagonal neighbours) sums; the 3D classes of voxel z are then
 //   centre = c_z, faces = f_z + c_{z-1} + c_{z+1}, edges = k_z + f_{z-1} + f_{z+1}, corners = k_{z-1} + k_{z+1}.
 // The weights (cin x COP float4, zero-padded output channels) sit in LDS and are read as broadcasts.
-constexpr int CT_X = 64, CT_Y = 4, CT_Z = 4;
+constexpr int CT_X = 64, CT_Y = 4, CT_Z = 2;
 constexpr int CT_HALO = (CT_Z + 2) * (CT_Y + 2) * (CT_X + 2);
 constexpr int CT_PER = (CT_HALO + 255) / 256;
 constexpr int CT_MAXCIN = 32;
