@@ -292,3 +292,27 @@ def test_post_glue_matches_oracle_on_host_tensors():
     KnR, TvR = BR.post([o[0] for o in outs], groups, BR.voigt_cell_params(Cv, nc * s, s), nc)
     assert np.abs(Kn.numpy() - KnR).max() <= 1e-14 * np.abs(KnR).max()
     assert np.abs(Tv.numpy() - TvR).max() <= 1e-14 * np.abs(KnR).max()
+
+
+def test_bond_abi_rejects_bad_arguments_without_a_gpu():
+    """The bond-network entry points validate shapes and tables before any HIP call."""
+    import ctypes
+    import _lpmlib as L
+    lib = net3d._lib()
+    assert lib.lpm_iso_conv(1, 3, 0, 8, 8, 8, None, None, None, 1, 0, 0, None, None, None, None) == L.LPM_E_BADARG
+    assert lib.lpm_iso_conv(1, 40, 8, 8, 8, 8, None, None, None, 1, 0, 0, None, None, None, None) == L.LPM_E_BADARG
+    assert lib.lpm_bond_heads(None, 1, 4, 2, *([None] * 13)) == L.LPM_E_BADARG
+    grp = net3d.BondGroup()
+    grp.ndir, grp.nv, grp.ncell, grp.H, grp.D, grp.dstride, grp.hid = 3, 16, 34, 8, 12, 12, 64
+    for d, k in enumerate((0, 2, 4)):
+        grp.slot[d] = k
+    assert lib.lpm_bond_w1_layout(ctypes.byref(grp), None, None, None) == L.LPM_E_BADARG   # axial with H = 8
+    assert b"stabiliser" in lib.lpm_last_error() or b"directions" in lib.lpm_last_error()
+    grp.H = 16
+    grp.voff = grp.coff = grp.inv = 16                                   # non-null dummies: never dereferenced
+    grp.slot[1] = 7                                                      # odd slot
+    assert lib.lpm_bond_w1_layout(ctypes.byref(grp), None, None, None) == L.LPM_E_BADARG
+    assert b"forward slot" in lib.lpm_last_error()
+    grp.slot[1] = 2
+    grp.dstride = 10                                                     # not a multiple of 4
+    assert lib.lpm_bond_w1_layout(ctypes.byref(grp), None, None, None) == L.LPM_E_BADARG
