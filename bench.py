@@ -641,10 +641,18 @@ def bench_infer(a):
         Kn, Tv = net.bond_params(Xd, nc, cp)
     torch.cuda.synchronize()
     lat = []
-    ev_feat, ev_heads = [], []
     t0 = time.perf_counter()
     for _ in range(a.steps):
         ts = time.perf_counter()
+        Kn, Tv = net.bond_params(Xd, nc, cp)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+        print("[bench] infer step: %.1f ms" % (lat[-1] * 1e3), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    # per-launch timing (HIP events on the launching stream)
+    ev_feat, ev_heads = [], []
+    for _ in range(2):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         Fv, Fc = net.features(Xd)
@@ -656,11 +664,7 @@ def bench_infer(a):
         net.heads_run(Fv, Fc, nc, cp=cp[None].contiguous(), Kn=Kn, Tv=Tv, events=evs)
         ev_heads.append(evs)
         del Fv, Fc
-        torch.cuda.synchronize()
-        lat.append(time.perf_counter() - ts)
-        print("[bench] infer step: %.1f ms" % (lat[-1] * 1e3), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    total = time.perf_counter() - t0
     feat_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev_feat]))
     head_ms = [float(np.mean([ev[gi][0].elapsed_time(ev[gi][1]) for ev in ev_heads])) for gi in range(2)]
     D, hid = net.D, net.heads[0][0].weight.shape[0]
