@@ -167,6 +167,9 @@ def iso_conv(x, w, b, gelu=False, proj=None, dstride=0):
     x = x.contiguous()
     B, cin, nz, ny, nx = x.shape
     cout = w.shape[0]
+    if tuple(w.shape) != (cout, cin, 4) or tuple(b.shape) != (cout,):
+        raise ValueError("iso_conv: weights %s / bias %s do not match %d input channels"
+                         % (tuple(w.shape), tuple(b.shape), cin))
     w = w.detach().to(x.device, torch.float32).contiguous()
     b = b.detach().to(x.device, torch.float32).contiguous()
     if proj is None:
@@ -177,6 +180,9 @@ def iso_conv(x, w, b, gelu=False, proj=None, dstride=0):
         pw = proj[0].detach().to(x.device, torch.float32).reshape(proj[0].shape[0], -1).contiguous()
         pb = proj[1].detach().to(x.device, torch.float32).contiguous()
         dproj = pw.shape[0]
+        if pw.shape[1] != cout or tuple(pb.shape) != (dproj,):
+            raise ValueError("iso_conv: projection %s / %s does not match %d channels"
+                             % (tuple(pw.shape), tuple(pb.shape), cout))
         dstride = dstride or 4 * ((dproj + 3) // 4)
         y = torch.empty((B, nz * ny * nx, dstride), device=x.device, dtype=torch.float32)
     L.check(_lib().lpm_iso_conv(B, cin, cout, nx, ny, nz, _ptr(x), _ptr(w), _ptr(b), int(bool(gelu)), dproj, dstride,
@@ -264,6 +270,18 @@ class BondNet(nn.Module):
         """Raw outputs per group from the features; with cp, Kn / Tv are filled as in bond_params.  `events`: an
         optional list that receives a (start, end) pair of CUDA events around every heads launch."""
         B = Fv.shape[0]
+        s, ds = self.s, self.dstride
+        npd, ncp, N = (nc + 4) * s, nc + 4, nc ** 3
+        L.require_gpu(Fv, Fc, cp, Kn, Tv)
+        if (tuple(Fv.shape) != (B, npd ** 3, ds) or tuple(Fc.shape) != (B, ncp ** 3, ds)
+                or Fv.dtype != torch.float32 or Fc.dtype != torch.float32
+                or not Fv.is_contiguous() or not Fc.is_contiguous()):
+            raise ValueError("heads_run: features must be float32 (B, %d, %d) and (B, %d, %d) for nc = %d"
+                             % (npd ** 3, ds, ncp ** 3, ds, nc))
+        if cp is not None:
+            for t_, shp in ((cp, (B, N, 10)), (Kn, (B, N, 18)), (Tv, (B, N, 18))):
+                if t_ is None or tuple(t_.shape) != shp or t_.dtype != torch.float64 or not t_.is_contiguous():
+                    raise ValueError("heads_run: cp / Kn / Tv must be contiguous float64 (B, %d, 10 | 18)" % N)
         outs = []
         lib = _lib()
         st = L.stream_of(Fv)

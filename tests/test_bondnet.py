@@ -316,3 +316,26 @@ def test_bond_abi_rejects_bad_arguments_without_a_gpu():
     grp.slot[1] = 2
     grp.dstride = 10                                                     # not a multiple of 4
     assert lib.lpm_bond_w1_layout(ctypes.byref(grp), None, None, None) == L.LPM_E_BADARG
+
+
+def test_iso_conv_rejects_mismatched_weights():
+    """Shape checks run before any device call: wrong weight shapes raise instead of reading out of bounds."""
+    x = torch.zeros((1, 3, 4, 4, 4))
+    with pytest.raises(RuntimeError):                      # CPU tensors are refused first
+        net3d.iso_conv(x, torch.zeros((8, 3, 4)), torch.zeros(8))
+
+
+@pytest.mark.gpu
+def test_gpu_shape_checks_raise_before_launch(dev):
+    x = torch.zeros((1, 3, 6, 6, 6), device=dev)
+    with pytest.raises(ValueError):
+        net3d.iso_conv(x, torch.zeros((8, 4, 4), device=dev), torch.zeros(8, device=dev))
+    net = net3d.BondNet(2, ch=6, D=4, hid=16).to(dev)
+    nc, s = 3, 2
+    _, X = _micro(nc * s, s, 1)
+    Fv, Fc = net.features(torch.as_tensor(X[None], device=dev))
+    with pytest.raises(ValueError):
+        net.heads_run(Fv, Fc, nc + 1)                     # features of a 3^3 box read as a 4^3 one
+    with pytest.raises(ValueError):
+        net.heads_run(Fv, Fc, nc, cp=torch.zeros((1, nc ** 3, 10), device=dev, dtype=torch.float64))
+    assert [o.shape[1] for o in net.heads_run(Fv, Fc, nc)] == [54, 72]
