@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
   BondMeta* meta = (BondMeta*)(coff_s + a.ndir * a.ncell);      // [HB]
   float* hbuf = xs;                                             // [HB][HMAX], after the GEMM
   float* h2buf = xs + HB * HMAX;                                // [HB][HMAX]
-  float* obuf = xs + 2 * HB * HMAX;                             // [HB][2]
+  float* obuf = xs + 2 * HB * HMAX;                             // [HB][2], then W2 (below)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bi = blockIdx.y;
   const int64_t b0 = (int64_t)blockIdx.x * HB;
@@ -352,6 +352,9 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
     __builtin_amdgcn_sched_barrier(0);
   }
   __syncthreads();  // xs is reused below
+  // W2 transposed into the freed window area, so that the second layer reads it from LDS
+  float* w2s = xs + 2 * HB * HMAX + 2 * HB;                     // [HMAX][HMAX]: w2s[m][j] = W2[j][m]
+  for (int i = tid; i < a.hid * a.hid; i += 256) w2s[(i % a.hid) * HMAX + i / a.hid] = a.w2[i];
 
   // + b1, GELU, mean over the stabiliser (rows of one bond)
   {
@@ -377,13 +380,21 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
   __syncthreads();
 
   {
-    const int bl = tid >> 4;
-    for (int j = tid & 15; j < a.hid; j += 16) {
-      float v = a.b2[j];
-      const float* wr = a.w2 + j * a.hid;
-      for (int m = 0; m < a.hid; ++m) v = fmaf(wr[m], hbuf[bl * HMAX + m], v);
-      h2buf[bl * HMAX + j] = gelu_erf(v);
+    // 16 threads per bond, each the outputs j0, j0 + 16, j0 + 32, j0 + 48 as independent chains
+    constexpr int NQ = HMAX / 16;
+    const int bl = tid >> 4, j0 = tid & 15;
+    float v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q] = j0 + 16 * q < a.hid ? a.b2[j0 + 16 * q] : 0.0f;
+    for (int m = 0; m < a.hid; ++m) {
+      const float hm = hbuf[bl * HMAX + m];
+      const float* wm = w2s + m * HMAX + j0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v[q] = fmaf(wm[16 * q], hm, v[q]);
     }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      if (j0 + 16 * q < a.hid) h2buf[bl * HMAX + j0 + 16 * q] = gelu_erf(v[q]);
   }
   __syncthreads();
   if (tid < 2 * HB) {
@@ -451,7 +462,7 @@ int check_group(const lpm_bond_group* g) {
 
 // floats of the LDS area that holds the window features during the GEMM and the hidden layers after it
 int head_xs_floats(const lpm_bond_group* g) {
-  return std::max(HB * (g->nv + g->ncell) * g->dstride, 2 * HB * HMAX + 2 * HB);
+  return std::max(HB * (g->nv + g->ncell) * g->dstride, 2 * HB * HMAX + 2 * HB + HMAX * HMAX);
 }
 
 size_t head_lds(const lpm_bond_group* g) {
