@@ -400,3 +400,45 @@ class Post:
             Tv[b[:, 0], b[:, 1]] = tt
             Tv[q, b[:, 1] ^ 1] = tt
         return Kn, Tv
+
+
+# ------------------------------------------------------------------------------------ specimens with internal voids
+KMIN = 1e-3                       # minimum stiffness of a kept bond, relative to the mean (timing/t3d/infer3d.py:34)
+
+
+def voxel_features_v(E, nu, solid, n, s):
+    """Voxel features of a specimen with internal voids (timing/t3d/infer3d.py:38-50): void voxels look like the
+    padding (log term and Tv term 0, void flag 1); the reference stiffness is the mean shear modulus of the solid."""
+    solid = np.asarray(solid, bool)
+    E = np.asarray(E, float)
+    nu = np.asarray(nu, float)
+    mu = np.where(solid, E / (2 * (1 + nu)), 1.0)
+    lam = np.where(solid, E * nu / ((1 + nu) * (1 - 2 * np.where(solid, nu, 0.0))), 1.0)
+    kref = mu[solid].mean()
+    pad = 2 * s
+    X = np.zeros((3,) + (n + 2 * pad,) * 3, np.float32)
+    X[2] = 1.0
+    core = (slice(pad, pad + n),) * 3
+    X[(0,) + core] = np.where(solid, np.log(mu / kref), 0.0).reshape(n, n, n)
+    X[(1,) + core] = np.where(solid, (lam - mu) / (18 * mu), 0.0).reshape(n, n, n)
+    X[(2,) + core] = (~solid).reshape(n, n, n)
+    return X
+
+
+def mask_cells(Kn, Tv, cell, nc, kmin=KMIN):
+    """Coarse Kn / Tv (nc^3, 18) of a specimen with void cells (timing/t3d/infer3d.py:95-103): a slot is kept when it
+    stays in the box and both of its cells are solid; other slots are zeroed, and kept Kn are raised to at least
+    kmin x (mean Kn of the kept slots).  `cell` (nc^3,) bool; tensors on any device, returned on the same."""
+    dev = Kn.device
+    cell = torch.as_tensor(np.asarray(cell, bool), device=dev)
+    ijk = torch.stack(torch.meshgrid(*(torch.arange(nc, device=dev),) * 3, indexing="ij"), -1).reshape(-1, 3).flip(1)
+    off = torch.as_tensor(SC_OFFS, device=dev)
+    nb = ijk[:, None, :] + off[None]
+    inside = ((nb >= 0) & (nb < nc)).all(-1)
+    q = ((nb[..., 2] * nc + nb[..., 1]) * nc + nb[..., 0]).clamp(0, nc ** 3 - 1)
+    v = inside & cell[:, None] & cell[q]
+    Kn = Kn * v
+    Tv = Tv * v
+    km = kmin * Kn.sum() / max(int(v.sum()), 1)
+    Kn = torch.where(v, torch.maximum(Kn, km), Kn)
+    return Kn, Tv

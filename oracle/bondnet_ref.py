@@ -208,3 +208,23 @@ def post(outs, groups, cp, nc):
         Tv[p, k] = tt
         Tv[q, k ^ 1] = tt
     return Kn, Tv
+
+
+def mask_cells(Kn, Tv, cell, nc, kmin=1e-3):
+    """Void-cell masking of the coarse parameters with the minimum-stiffness floor (timing/t3d/infer3d.py:95-103),
+    written as explicit loops over particles and slots."""
+    Kn = np.array(Kn, float)
+    Tv = np.array(Tv, float)
+    keep = np.zeros(Kn.shape, bool)
+    for p in range(nc ** 3):
+        x, y, z = p % nc, (p // nc) % nc, p // (nc * nc)
+        for k in range(18):
+            ox, oy, oz = SC_OFF[k]
+            X, Y, Z = x + ox, y + oy, z + oz
+            if 0 <= X < nc and 0 <= Y < nc and 0 <= Z < nc:
+                keep[p, k] = cell[p] and cell[(Z * nc + Y) * nc + X]
+    Kn[~keep] = 0.0
+    Tv[~keep] = 0.0
+    km = kmin * Kn.sum() / max(keep.sum(), 1)
+    Kn[keep] = np.maximum(Kn[keep], km)
+    return Kn, Tv

@@ -339,3 +339,56 @@ def test_gpu_shape_checks_raise_before_launch(dev):
     with pytest.raises(ValueError):
         net.heads_run(Fv, Fc, nc, cp=torch.zeros((1, nc ** 3, 10), device=dev, dtype=torch.float64))
     assert [o.shape[1] for o in net.heads_run(Fv, Fc, nc)] == [54, 72]
+
+
+def test_void_features_reduce_to_the_two_phase_features():
+    """voxel_features_v (infer3d.py:38-50) with every voxel solid equals voxel_features of the same phases; void voxels
+    carry the padding's values."""
+    n, s = 6, 2
+    rng = np.random.default_rng(4)
+    phase = (rng.random(n ** 3) < 0.4).astype(np.int64)
+    E = np.where(phase == 1, 3.0, 1.0)
+    nu = np.where(phase == 1, 0.3, 0.2)
+    X = net3d.voxel_features_v(E, nu, np.ones(n ** 3, bool), n, s)
+    assert np.array_equal(X, net3d.voxel_features(phase, 3.0, 0.3, n, s))
+    solid = rng.random(n ** 3) > 0.2
+    Xv = net3d.voxel_features_v(E, nu, solid, n, s)
+    core = Xv[:, 2 * s:2 * s + n, 2 * s:2 * s + n, 2 * s:2 * s + n].reshape(3, -1)
+    assert np.all(core[:, ~solid] == np.array([[0.0], [0.0], [1.0]]))
+    mu = E / (2 * (1 + nu))
+    assert abs(np.log(mu[solid] / mu[solid].mean()).astype(np.float32) - core[0, solid]).max() == 0
+
+
+def test_mask_cells_matches_oracle_and_its_rules():
+    import geom3d
+    nc = 5
+    rng = np.random.default_rng(6)
+    lat = geom3d.LatticeGeom(nc, nc, nc, 2)
+    Kn = rng.uniform(0.0, 2.0, (nc ** 3, 18)) * lat.valid
+    Kn[rng.random(Kn.shape) < 0.1] = 1e-9                       # a few weak bonds that the floor must raise
+    Tv = rng.normal(0.0, 0.1, (nc ** 3, 18)) * lat.valid
+    cell = rng.random(nc ** 3) > 0.25
+    K, T = net3d.mask_cells(torch.as_tensor(Kn), torch.as_tensor(Tv), cell, nc)
+    KR, TR = BR.mask_cells(Kn, Tv, cell, nc)
+    assert np.array_equal(K.numpy(), KR) and np.array_equal(T.numpy(), TR)
+    q = np.where(lat.valid, lat.nbr, 0)
+    keep = lat.valid & cell[:, None] & cell[q]                  # the rule, from the lattice tables
+    km = 1e-3 * (Kn * keep).sum() / keep.sum()                   # floor from the kept values before flooring
+    assert np.all(KR[~keep] == 0) and np.all(TR[~keep] == 0)
+    assert np.array_equal(KR[keep], np.maximum(Kn[keep], km)) and np.array_equal(TR[keep], Tv[keep])
+    assert np.sum(Kn[keep] < km) > 0                             # the floor acted on some bonds
+
+
+@pytest.mark.gpu
+def test_gpu_mask_cells_matches_oracle(dev):
+    import geom3d
+    nc = 6
+    rng = np.random.default_rng(9)
+    lat = geom3d.LatticeGeom(nc, nc, nc, 2)
+    Kn = rng.uniform(0.0, 2.0, (nc ** 3, 18)) * lat.valid
+    Tv = rng.normal(0.0, 0.1, (nc ** 3, 18)) * lat.valid
+    cell = rng.random(nc ** 3) > 0.3
+    K, T = net3d.mask_cells(torch.as_tensor(Kn, device=dev), torch.as_tensor(Tv, device=dev), cell, nc)
+    KR, TR = BR.mask_cells(Kn, Tv, cell, nc)
+    assert np.abs(K.cpu().numpy() - KR).max() <= 1e-15 * np.abs(KR).max()
+    assert np.abs(T.cpu().numpy() - TR).max() <= 1e-15 * np.abs(TR).max()
