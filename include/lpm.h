@@ -305,6 +305,17 @@ int lpm_bond_heads(const lpm_bond_group* grp, int32_t B, int32_t nc, int32_t s, 
                    const float* w1f, const float* b1, const float* w2, const float* b2, const float* w3,
                    const float* b3, float* out, const double* cp, double* kn, double* tv, void* stream);
 
+/* Same heads with the first layer in split BF16 (opt-in): every FP32 operand x = hi + lo (hi = bf16(x), lo =
+ * bf16(x - hi)) and W1 x ~ Whi xhi + Whi xlo + Wlo xhi accumulated in FP32 on the BF16 matrix cores; D = dstride =
+ * 12 (LPM_E_UNSUPPORTED otherwise).  w1h / w1l (lpm_bond_w1_split_size bytes each) from lpm_bond_w1_split_layout.
+ * The reference computes this layer in FP32 (net3d.py:160-162); no counterpart for the split. */
+size_t lpm_bond_w1_split_size(const lpm_bond_group* grp);
+int lpm_bond_w1_split_layout(const lpm_bond_group* grp, const float* w1, void* w1h, void* w1l, void* stream);
+int lpm_bond_heads_split(const lpm_bond_group* grp, int32_t B, int32_t nc, int32_t s, const float* fv,
+                         const float* fc, const void* w1h, const void* w1l, const float* b1, const float* w2,
+                         const float* b2, const float* w3, const float* b3, float* out, const double* cp, double* kn,
+                         double* tv, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

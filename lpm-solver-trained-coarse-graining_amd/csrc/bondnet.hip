@@ -219,24 +219,25 @@ struct BondMeta {
   int dir, vbase, cbase, p, q, k;
 };
 
-template <int H, int NT, int Q>
-__global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
-  extern __shared__ __align__(16) float smem[];
-  static_assert((H == 8 || H == 16) && (NT == 1 || NT == 2 || NT == 4) && Q >= 1 && Q <= 4, "tile shape");
-  constexpr int RT = H / 4;
-  const int nslot = a.nslot, ds = a.ds;
-  float* xs = smem;                                             // [HB][nslot][ds]
-  int* inv_s = (int*)(xs + a.xs_floats);                        // [H][nslot]
-  int* voff_s = inv_s + H * nslot;                              // [ndir][nv]
-  int* coff_s = voff_s + a.ndir * a.nv;                         // [ndir][ncell]
-  BondMeta* meta = (BondMeta*)(coff_s + a.ndir * a.ncell);      // [HB]
-  float* hbuf = xs;                                             // [HB][HMAX], after the GEMM
-  float* h2buf = xs + HB * HMAX;                                // [HB][HMAX]
-  float* obuf = xs + 2 * HB * HMAX;                             // [HB][2], then W2 (below)
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int bi = blockIdx.y;
-  const int64_t b0 = (int64_t)blockIdx.x * HB;
+// LDS of a heads block: the window area (xs_floats floats; hidden layers and W2 after the first layer), then the
+// inverse stabiliser perms [H][nslot], the window offsets [ndir][nv] and [ndir][ncell], and the bonds' meta [HB]
+struct HeadSmem {
+  float* xs;
+  int* inv_s;
+  int* voff_s;
+  int* coff_s;
+  BondMeta* meta;
+};
 
+// bond list decode (direction by direction, x fastest) and the tables into LDS
+template <int H>
+__device__ __forceinline__ HeadSmem heads_setup(const HeadArgs& a, float* smem, int tid, int64_t b0) {
+  HeadSmem sm;
+  sm.xs = smem;
+  sm.inv_s = (int*)(smem + a.xs_floats);
+  sm.voff_s = sm.inv_s + H * a.nslot;
+  sm.coff_s = sm.voff_s + a.ndir * a.nv;
+  sm.meta = (BondMeta*)(sm.coff_s + a.ndir * a.ncell);
   if (tid < HB) {
     BondMeta m{-1, 0, 0, 0, 0, 0};
     const int64_t gb = b0 + tid;
@@ -255,46 +256,144 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
       m.p = (iz * a.nc + iy) * a.nc + ix;
       m.q = ((iz + oz) * a.nc + (iy + oy)) * a.nc + (ix + ox);
     }
-    meta[tid] = m;
+    sm.meta[tid] = m;
   }
-  for (int i = tid; i < H * nslot; i += 256) inv_s[i] = a.inv[i];
-  for (int i = tid; i < a.ndir * a.nv; i += 256) voff_s[i] = a.voff[i];
-  for (int i = tid; i < a.ndir * a.ncell; i += 256) coff_s[i] = a.coff[i];
+  for (int i = tid; i < H * a.nslot; i += 256) sm.inv_s[i] = a.inv[i];
+  for (int i = tid; i < a.ndir * a.nv; i += 256) sm.voff_s[i] = a.voff[i];
+  for (int i = tid; i < a.ndir * a.ncell; i += 256) sm.coff_s[i] = a.coff[i];
   __syncthreads();
+  return sm;
+}
 
-  // window gather: HB x nslot x (ds / 4) float4, GB independent loads in flight per thread
-  {
-    constexpr int GB = 8;
-    const int64_t nvox = (int64_t)a.npd * a.npd * a.npd, ncel = (int64_t)a.ncp * a.ncp * a.ncp;
-    const float* fvb = a.fv + (int64_t)bi * nvox * ds;
-    const float* fcb = a.fc + (int64_t)bi * ncel * ds;
-    const int total = HB * nslot * a.Q;
-    for (int i0 = 0; i0 < total; i0 += 256 * GB) {
-      f4 v[GB];
-      int dst[GB];
+// window gather: HB x nslot x (ds / 4) float4 with GB independent loads in flight per thread; store(e, v) puts the
+// four channels 4c..4c+3 of (bond, slot) at element e = (bond nslot + slot) ds + 4c of the window area
+template <class Store>
+__device__ __forceinline__ void heads_gather(const HeadArgs& a, const HeadSmem& sm, int tid, int bi, Store store) {
+  constexpr int GB = 8;
+  const int nslot = a.nslot, ds = a.ds;
+  const int64_t nvox = (int64_t)a.npd * a.npd * a.npd, ncel = (int64_t)a.ncp * a.ncp * a.ncp;
+  const float* fvb = a.fv + (int64_t)bi * nvox * ds;
+  const float* fcb = a.fc + (int64_t)bi * ncel * ds;
+  const int total = HB * nslot * a.Q;
+  for (int i0 = 0; i0 < total; i0 += 256 * GB) {
+    f4 v[GB];
+    int dst[GB];
 #pragma unroll
-      for (int u = 0; u < GB; ++u) {
-        const int idx = i0 + u * 256 + tid;
-        v[u] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        dst[u] = -1;
-        if (idx < total) {
-          const int c = idx % a.Q, r = idx / a.Q;
-          const int slot = r % nslot, bl = r / nslot;
-          const BondMeta m = meta[bl];
-          dst[u] = (bl * nslot + slot) * ds + 4 * c;
-          if (m.dir >= 0) {
-            const float* src = slot < a.nv ? fvb + (int64_t)(m.vbase + voff_s[m.dir * a.nv + slot]) * ds
-                                           : fcb + (int64_t)(m.cbase + coff_s[m.dir * a.ncell + slot - a.nv]) * ds;
-            v[u] = *(const f4*)(src + 4 * c);
-          }
+    for (int u = 0; u < GB; ++u) {
+      const int idx = i0 + u * 256 + tid;
+      v[u] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+      dst[u] = -1;
+      if (idx < total) {
+        const int c = idx % a.Q, r = idx / a.Q;
+        const int slot = r % nslot, bl = r / nslot;
+        const BondMeta m = sm.meta[bl];
+        dst[u] = (bl * nslot + slot) * ds + 4 * c;
+        if (m.dir >= 0) {
+          const float* src = slot < a.nv ? fvb + (int64_t)(m.vbase + sm.voff_s[m.dir * a.nv + slot]) * ds
+                                         : fcb + (int64_t)(m.cbase + sm.coff_s[m.dir * a.ncell + slot - a.nv]) * ds;
+          v[u] = *(const f4*)(src + 4 * c);
         }
       }
-#pragma unroll
-      for (int u = 0; u < GB; ++u)
-        if (dst[u] >= 0) *(f4*)(xs + dst[u]) = v[u];
     }
+#pragma unroll
+    for (int u = 0; u < GB; ++u)
+      if (dst[u] >= 0) store(dst[u], v[u]);
   }
   __syncthreads();
+}
+
+// after the first layer: + b1, GELU, mean over the stabiliser, second layer (W2 from LDS), third layer, raw outputs
+// and, with cp, Kn / Tv on both slots of every bond.  acc: the wave's RT x NT tiles of first-layer sums.
+template <int H, int NT>
+__device__ __forceinline__ void heads_epilogue(const HeadArgs& a, const HeadSmem& sm, const f4 (&acc)[H / 4][NT],
+                                               int tid, int bi, int64_t b0) {
+  constexpr int RT = H / 4;
+  const int lane = tid & 63, wave = tid >> 6;
+  float* hbuf = sm.xs;                                          // [HB][HMAX]
+  float* h2buf = sm.xs + HB * HMAX;                             // [HB][HMAX]
+  float* obuf = sm.xs + 2 * HB * HMAX;                          // [HB][2]
+  float* w2s = sm.xs + 2 * HB * HMAX + 2 * HB;                  // [HMAX][HMAX]: w2s[m][j] = W2[j][m]
+  __syncthreads();  // the window area is reused from here on
+  for (int i = tid; i < a.hid * a.hid; i += 256) w2s[(i % a.hid) * HMAX + i / a.hid] = a.w2[i];
+  {
+    const float invH = 1.0f / (float)H;
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        const int col = n * 16 + (lane & 15);
+        const float bias = col < a.hid ? a.b1[col] : 0.0f;
+        float sum = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sum += gelu_erf(acc[t][n][i] + bias);
+        sum += __shfl_xor(sum, 16);
+        if constexpr (H == 16) {
+          sum += __shfl_xor(sum, 32);
+          if ((lane >> 4) == 0 && col < a.hid) hbuf[(wave * 4 + t) * HMAX + col] = sum * invH;
+        } else {
+          if ((lane & 16) == 0 && col < a.hid) hbuf[(wave * 4 + t * 2 + (lane >> 5)) * HMAX + col] = sum * invH;
+        }
+      }
+  }
+  __syncthreads();
+  {
+    // 16 threads per bond, each the outputs j0, j0 + 16, j0 + 32, j0 + 48 as independent chains
+    constexpr int NQ = HMAX / 16;
+    const int bl = tid >> 4, j0 = tid & 15;
+    float v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q] = j0 + 16 * q < a.hid ? a.b2[j0 + 16 * q] : 0.0f;
+    for (int m = 0; m < a.hid; ++m) {
+      const float hm = hbuf[bl * HMAX + m];
+      const float* wm = w2s + m * HMAX + j0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v[q] = fmaf(wm[16 * q], hm, v[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      if (j0 + 16 * q < a.hid) h2buf[bl * HMAX + j0 + 16 * q] = gelu_erf(v[q]);
+  }
+  __syncthreads();
+  if (tid < 2 * HB) {
+    const int bl = tid >> 1, c = tid & 1;
+    float v = a.b3[c];
+    const float* wr = a.w3 + c * a.hid;
+    for (int m = 0; m < a.hid; ++m) v = fmaf(wr[m], h2buf[bl * HMAX + m], v);
+    obuf[bl * 2 + c] = v;
+    if (sm.meta[bl].dir >= 0) a.out[((int64_t)bi * a.nb + b0 + bl) * 2 + c] = v;
+  }
+  if (!a.cp) return;
+  __syncthreads();
+  if (tid < HB && sm.meta[tid].dir >= 0) {
+    const BondMeta m = sm.meta[tid];
+    const int64_t nce = (int64_t)a.nc * a.nc * a.nc;
+    const double* cpb = a.cp + (int64_t)bi * nce * 10;
+    const double kv = 0.5 * (cpb[(int64_t)m.p * 10 + m.k / 2] + cpb[(int64_t)m.q * 10 + m.k / 2]);
+    const double tb = 0.5 * (cpb[(int64_t)m.p * 10 + 9] + cpb[(int64_t)m.q * 10 + 9]);
+    const double knv = kv * exp((double)obuf[tid * 2]);
+    const double tvv = tb + kv * (double)obuf[tid * 2 + 1];
+    double* knb = a.kn + (int64_t)bi * nce * 18;
+    double* tvb = a.tv + (int64_t)bi * nce * 18;
+    knb[(int64_t)m.p * 18 + m.k] = knv;
+    knb[(int64_t)m.q * 18 + (m.k ^ 1)] = knv;
+    tvb[(int64_t)m.p * 18 + m.k] = tvv;
+    tvb[(int64_t)m.q * 18 + (m.k ^ 1)] = tvv;
+  }
+}
+
+template <int H, int NT, int Q>
+__global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
+  extern __shared__ __align__(16) float smem[];
+  static_assert((H == 8 || H == 16) && (NT == 1 || NT == 2 || NT == 4) && Q >= 1 && Q <= 4, "tile shape");
+  constexpr int RT = H / 4;
+  const int nslot = a.nslot, ds = a.ds;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int bi = blockIdx.y;
+  const int64_t b0 = (int64_t)blockIdx.x * HB;
+  const HeadSmem sm = heads_setup<H>(a, smem, tid, b0);
+  float* xs = sm.xs;                                            // [HB][nslot][ds]
+  const int* inv_s = sm.inv_s;
+  heads_gather(a, sm, tid, bi, [&](int e, const f4& v) { *(f4*)(xs + e) = v; });
 
   f4 acc[RT][NT];
 #pragma unroll
@@ -351,77 +450,144 @@ __global__ __launch_bounds__(256) void bond_heads_kernel(HeadArgs a) {
     load(fb1, fa1);
     __builtin_amdgcn_sched_barrier(0);
   }
-  __syncthreads();  // xs is reused below
-  // W2 transposed into the freed window area, so that the second layer reads it from LDS
-  float* w2s = xs + 2 * HB * HMAX + 2 * HB;                     // [HMAX][HMAX]: w2s[m][j] = W2[j][m]
-  for (int i = tid; i < a.hid * a.hid; i += 256) w2s[(i % a.hid) * HMAX + i / a.hid] = a.w2[i];
+  heads_epilogue<H, NT>(a, sm, acc, tid, bi, b0);
+}
 
-  // + b1, GELU, mean over the stabiliser (rows of one bond)
-  {
-    const float invH = 1.0f / (float)H;
+// ---- first layer in split BF16 (opt-in): every FP32 operand x = hi + lo with hi = bf16(x), lo = bf16(x - hi), and
+// W1 x = Whi xhi + Whi xlo + Wlo xhi accumulated in FP32 by v_mfma_f32_16x16x32_bf16 (lane layout A[lane % 16][8
+// (lane / 16) + i], B[8 (lane / 16) + i][lane % 16], D as above).  One instruction covers two window slots of 12
+// channels: lane group c reads slot 2 p + (c >> 1), channels 8 (c & 1) + 0..7 (channels 12..15 are zero).  The
+// window is stored as hi and lo BF16 planes (the same bytes as FP32); W1 comes split and fragment-ordered
+// (lpm_bond_w1_split_layout, [slot pair][column tile][lane] x 8 BF16).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+template <int H, int NT>
+__global__ __launch_bounds__(256) void bond_heads_split_kernel(HeadArgs a, const bf16x8* __restrict__ wh,
+                                                               const bf16x8* __restrict__ wl) {
+  extern __shared__ __align__(16) float smem[];
+  static_assert((H == 8 || H == 16) && (NT == 1 || NT == 2 || NT == 4), "tile shape");
+  constexpr int RT = H / 4;
+  constexpr int DS = 12;
+  const int nslot = a.nslot;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int bi = blockIdx.y;
+  const int64_t b0 = (int64_t)blockIdx.x * HB;
+  const HeadSmem sm = heads_setup<H>(a, smem, tid, b0);
+  __bf16* xh = (__bf16*)sm.xs;                                  // [HB][nslot][12] hi
+  __bf16* xl = xh + HB * nslot * DS;                            // [HB][nslot][12] lo
+  const int* inv_s = sm.inv_s;
+  heads_gather(a, sm, tid, bi, [&](int e, const f4& v) {
+    bf16x4 h, l;
 #pragma unroll
-    for (int t = 0; t < RT; ++t)
+    for (int i = 0; i < 4; ++i) {
+      const __bf16 hb = (__bf16)v[i];
+      h[i] = hb;
+      l[i] = (__bf16)(v[i] - (float)hb);
+    }
+    *(bf16x4*)(xh + e) = h;
+    *(bf16x4*)(xl + e) = l;
+  });
+
+  f4 acc[RT][NT];
 #pragma unroll
-      for (int n = 0; n < NT; ++n) {
-        const int col = n * 16 + (lane & 15);
-        const float bias = col < a.hid ? a.b1[col] : 0.0f;
-        float sum = 0.0f;
+  for (int t = 0; t < RT; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) sum += gelu_erf(acc[t][n][i] + bias);
-        sum += __shfl_xor(sum, 16);
-        if constexpr (H == 16) {
-          sum += __shfl_xor(sum, 32);
-          if ((lane >> 4) == 0 && col < a.hid) hbuf[(wave * 4 + t) * HMAX + col] = sum * invH;
-        } else {
-          if ((lane & 16) == 0 && col < a.hid) hbuf[(wave * 4 + t * 2 + (lane >> 5)) * HMAX + col] = sum * invH;
+    for (int n = 0; n < NT; ++n) acc[t][n] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int half = (lane >> 4) & 1, sub = lane >> 5;
+  int xrow[RT], grow[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const int row = wave * RT * 16 + t * 16 + (lane & 15);
+    xrow[t] = (row / H) * nslot * DS + 8 * half;
+    grow[t] = (row % H) * nslot;
+  }
+  const int npair = (nslot + 1) / 2;
+  int lp = 0;
+  // W1 fragments two slot pairs ahead (W1 holds two zero pairs past the end); window fragments at their step
+  auto loadb = [&](bf16x8 (&bh)[NT], bf16x8 (&bl)[NT]) {
+    const int64_t wo = (int64_t)lp * NT * 64 + lane;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      bh[n] = wh[wo + n * 64];
+      bl[n] = wl[wo + n * 64];
+    }
+    ++lp;
+  };
+  auto loada = [&](int pr, int t, bf16x8& ah, bf16x8& al) {
+    const int slot = 2 * pr + sub;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      ah[i] = (__bf16)0.0f;
+      al[i] = (__bf16)0.0f;
+    }
+    if (slot < nslot) {                                         // an odd last pair has no second slot
+      const int base = xrow[t] + inv_s[grow[t] + slot] * DS;
+      const bf16x4 h0 = *(const bf16x4*)(xh + base), l0 = *(const bf16x4*)(xl + base);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        ah[i] = h0[i];
+        al[i] = l0[i];
+      }
+      if (half == 0) {
+        const bf16x4 h1 = *(const bf16x4*)(xh + base + 4), l1 = *(const bf16x4*)(xl + base + 4);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          ah[4 + i] = h1[i];
+          al[4 + i] = l1[i];
         }
       }
-  }
-  __syncthreads();
-
-  {
-    // 16 threads per bond, each the outputs j0, j0 + 16, j0 + 32, j0 + 48 as independent chains
-    constexpr int NQ = HMAX / 16;
-    const int bl = tid >> 4, j0 = tid & 15;
-    float v[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) v[q] = j0 + 16 * q < a.hid ? a.b2[j0 + 16 * q] : 0.0f;
-    for (int m = 0; m < a.hid; ++m) {
-      const float hm = hbuf[bl * HMAX + m];
-      const float* wm = w2s + m * HMAX + j0;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) v[q] = fmaf(wm[16 * q], hm, v[q]);
     }
+  };
+  auto step = [&](int pr, const bf16x8 (&bh)[NT], const bf16x8 (&bl)[NT]) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
-      if (j0 + 16 * q < a.hid) h2buf[bl * HMAX + j0 + 16 * q] = gelu_erf(v[q]);
+    for (int t = 0; t < RT; ++t) {
+      bf16x8 ah, al;
+      loada(pr, t, ah, al);
+#pragma unroll
+      for (int n = 0; n < NT; ++n) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[n], acc[t][n], 0, 0, 0);
+#pragma unroll
+      for (int n = 0; n < NT; ++n) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[n], acc[t][n], 0, 0, 0);
+#pragma unroll
+      for (int n = 0; n < NT; ++n) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[n], acc[t][n], 0, 0, 0);
+    }
+  };
+  bf16x8 bh0[NT], bl0[NT], bh1[NT], bl1[NT];
+  loadb(bh0, bl0);
+  loadb(bh1, bl1);
+  for (int pr = 0; pr < npair; pr += 2) {
+    step(pr, bh0, bl0);
+    loadb(bh0, bl0);
+    __builtin_amdgcn_sched_barrier(0);
+    if (pr + 1 < npair) step(pr + 1, bh1, bl1);
+    loadb(bh1, bl1);
+    __builtin_amdgcn_sched_barrier(0);
   }
-  __syncthreads();
-  if (tid < 2 * HB) {
-    const int bl = tid >> 1, c = tid & 1;
-    float v = a.b3[c];
-    const float* wr = a.w3 + c * a.hid;
-    for (int m = 0; m < a.hid; ++m) v = fmaf(wr[m], h2buf[bl * HMAX + m], v);
-    obuf[bl * 2 + c] = v;
-    if (meta[bl].dir >= 0) a.out[((int64_t)bi * a.nb + b0 + bl) * 2 + c] = v;
+  heads_epilogue<H, NT>(a, sm, acc, tid, bi, b0);
+}
+
+// W1 split into hi / lo BF16 fragments: [slot pair][column tile][lane] x 8, two zero pairs past the end
+__global__ void w1_split_kernel(int nslot, int NT, int hid, const float* __restrict__ w1, bf16x8* __restrict__ wh,
+                                bf16x8* __restrict__ wl) {
+  const int npair = (nslot + 1) / 2;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)(npair + 2) * NT * 64) return;
+  const int lane = (int)(idx & 63);
+  const int64_t r = idx >> 6;
+  const int n = (int)(r % NT), pr = (int)(r / NT);
+  const int col = n * 16 + (lane & 15);
+  const int slot = 2 * pr + (lane >> 5);
+  bf16x8 h, l;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int ch = 8 * ((lane >> 4) & 1) + i;
+    const float v = (slot < nslot && col < hid && ch < 12) ? w1[(int64_t)col * nslot * 12 + slot * 12 + ch] : 0.0f;
+    const __bf16 hb = (__bf16)v;
+    h[i] = hb;
+    l[i] = (__bf16)(v - (float)hb);
   }
-  if (!a.cp) return;
-  __syncthreads();
-  if (tid < HB && meta[tid].dir >= 0) {
-    const BondMeta m = meta[tid];
-    const int64_t nce = (int64_t)a.nc * a.nc * a.nc;
-    const double* cpb = a.cp + (int64_t)bi * nce * 10;
-    const double kv = 0.5 * (cpb[(int64_t)m.p * 10 + m.k / 2] + cpb[(int64_t)m.q * 10 + m.k / 2]);
-    const double tb = 0.5 * (cpb[(int64_t)m.p * 10 + 9] + cpb[(int64_t)m.q * 10 + 9]);
-    const double knv = kv * exp((double)obuf[tid * 2]);
-    const double tvv = tb + kv * (double)obuf[tid * 2 + 1];
-    double* knb = a.kn + (int64_t)bi * nce * 18;
-    double* tvb = a.tv + (int64_t)bi * nce * 18;
-    knb[(int64_t)m.p * 18 + m.k] = knv;
-    knb[(int64_t)m.q * 18 + (m.k ^ 1)] = knv;
-    tvb[(int64_t)m.p * 18 + m.k] = tvv;
-    tvb[(int64_t)m.q * 18 + (m.k ^ 1)] = tvv;
-  }
+  wh[idx] = h;
+  wl[idx] = l;
 }
 
 // fragment order [slot][column tile][lane][j]: lane's k-step j of a slot is channel (lane / 16) Q + j
@@ -501,6 +667,65 @@ int launch_heads_nt(const HeadArgs& a, size_t lds, hipStream_t st) {
   return launch_heads_q<H, 4>(a, lds, st);
 }
 
+// checks and kernel arguments shared by the FP32 and the split-BF16 heads
+int head_args(const lpm_bond_group* g, int32_t B, int32_t nc, int32_t s, const float* fv, const float* fc,
+              const float* w1, const float* b1, const float* w2, const float* b2, const float* w3, const float* b3,
+              float* out, const double* cp, double* kn, double* tv, const char* who, HeadArgs& a, size_t& lds) {
+  if (int rc = check_group(g)) return rc;
+  if (B < 1 || B > 65535 || nc < 2 || s < 1 || !fv || !fc || !w1 || !b1 || !w2 || !b2 || !w3 || !b3 || !out)
+    return fail(LPM_E_BADARG, "%s: bad arguments", who);
+  if (cp && (!kn || !tv)) return fail(LPM_E_BADARG, "%s: cp given without kn / tv", who);
+  const int64_t npd = (int64_t)(nc + 4) * s;
+  if (npd * npd * npd >= ((int64_t)1 << 31)) return fail(LPM_E_BADARG, "%s: padded grid too large", who);
+  lds = head_lds(g);
+  if (lds > 65536) return fail(LPM_E_BADARG, "%s: window needs %zu bytes of LDS (64 KiB)", who, lds);
+  a = HeadArgs{};
+  a.ndir = g->ndir;
+  a.nv = g->nv;
+  a.ncell = g->ncell;
+  a.nslot = g->nv + g->ncell;
+  a.H = g->H;
+  a.D = g->D;
+  a.ds = g->dstride;
+  a.hid = g->hid;
+  a.Q = g->dstride / 4;
+  a.nc = nc;
+  a.s = s;
+  a.npd = (int)npd;
+  a.ncp = nc + 4;
+  a.B = B;
+  a.xs_floats = head_xs_floats(g);
+  a.cnt[0] = 0;
+  for (int d = 0; d < 6; ++d) {
+    const int k = d < g->ndir ? g->slot[d] : 0;
+    a.dk[d] = k;
+    a.dx[d] = SC_OFF[k][0];
+    a.dy[d] = SC_OFF[k][1];
+    a.dz[d] = SC_OFF[k][2];
+  }
+  // bond counts per direction: the cells whose neighbour stays in the box
+  for (int d = 0; d < g->ndir; ++d)
+    a.cnt[d + 1] = a.cnt[d] + (int64_t)(nc - std::abs(a.dx[d])) * (nc - std::abs(a.dy[d])) * (nc - std::abs(a.dz[d]));
+  for (int d = g->ndir + 1; d < 7; ++d) a.cnt[d] = a.cnt[g->ndir];
+  a.nb = a.cnt[g->ndir];
+  a.voff = g->voff;
+  a.coff = g->coff;
+  a.inv = g->inv;
+  a.fv = fv;
+  a.fc = fc;
+  a.w1f = w1;
+  a.b1 = b1;
+  a.w2 = w2;
+  a.b2 = b2;
+  a.w3 = w3;
+  a.b3 = b3;
+  a.out = out;
+  a.cp = cp;
+  a.kn = kn;
+  a.tv = tv;
+  return LPM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -548,60 +773,60 @@ int lpm_bond_w1_layout(const lpm_bond_group* g, const float* w1, float* w1f, voi
 int lpm_bond_heads(const lpm_bond_group* g, int32_t B, int32_t nc, int32_t s, const float* fv, const float* fc,
                    const float* w1f, const float* b1, const float* w2, const float* b2, const float* w3,
                    const float* b3, float* out, const double* cp, double* kn, double* tv, void* stream) {
-  if (int rc = check_group(g)) return rc;
-  if (B < 1 || B > 65535 || nc < 2 || s < 1 || !fv || !fc || !w1f || !b1 || !w2 || !b2 || !w3 || !b3 || !out)
-    return fail(LPM_E_BADARG, "lpm_bond_heads: bad arguments");
-  if (cp && (!kn || !tv)) return fail(LPM_E_BADARG, "lpm_bond_heads: cp given without kn / tv");
-  const int64_t npd = (int64_t)(nc + 4) * s;
-  if (npd * npd * npd >= ((int64_t)1 << 31)) return fail(LPM_E_BADARG, "lpm_bond_heads: padded grid too large");
-  const size_t lds = head_lds(g);
-  if (lds > 65536) return fail(LPM_E_BADARG, "lpm_bond_heads: window needs %zu bytes of LDS (64 KiB)", lds);
   HeadArgs a{};
-  a.ndir = g->ndir;
-  a.nv = g->nv;
-  a.ncell = g->ncell;
-  a.nslot = g->nv + g->ncell;
-  a.H = g->H;
-  a.D = g->D;
-  a.ds = g->dstride;
-  a.hid = g->hid;
-  a.Q = g->dstride / 4;
-  a.nc = nc;
-  a.s = s;
-  a.npd = (int)npd;
-  a.ncp = nc + 4;
-  a.B = B;
-  a.xs_floats = head_xs_floats(g);
-  a.cnt[0] = 0;
-  for (int d = 0; d < 6; ++d) {
-    const int k = d < g->ndir ? g->slot[d] : 0;
-    a.dk[d] = k;
-    a.dx[d] = SC_OFF[k][0];
-    a.dy[d] = SC_OFF[k][1];
-    a.dz[d] = SC_OFF[k][2];
-  }
-  // bond counts per direction: the cells whose neighbour stays in the box
-  for (int d = 0; d < g->ndir; ++d)
-    a.cnt[d + 1] = a.cnt[d] + (int64_t)(nc - std::abs(a.dx[d])) * (nc - std::abs(a.dy[d])) * (nc - std::abs(a.dz[d]));
-  for (int d = g->ndir + 1; d < 7; ++d) a.cnt[d] = a.cnt[g->ndir];
-  a.nb = a.cnt[g->ndir];
-  a.voff = g->voff;
-  a.coff = g->coff;
-  a.inv = g->inv;
-  a.fv = fv;
-  a.fc = fc;
-  a.w1f = w1f;
-  a.b1 = b1;
-  a.w2 = w2;
-  a.b2 = b2;
-  a.w3 = w3;
-  a.b3 = b3;
-  a.out = out;
-  a.cp = cp;
-  a.kn = kn;
-  a.tv = tv;
+  size_t lds = 0;
+  if (int rc = head_args(g, B, nc, s, fv, fc, w1f, b1, w2, b2, w3, b3, out, cp, kn, tv, "lpm_bond_heads", a, lds))
+    return rc;
   hipStream_t st = (hipStream_t)stream;
   return g->H == 16 ? launch_heads_nt<16>(a, lds, st) : launch_heads_nt<8>(a, lds, st);
+}
+
+size_t lpm_bond_w1_split_size(const lpm_bond_group* g) {
+  if (!g || g->hid < 1) return 0;
+  return (size_t)((g->nv + g->ncell + 1) / 2 + 2) * col_tiles(g->hid) * 64 * sizeof(bf16x8);
+}
+
+int lpm_bond_w1_split_layout(const lpm_bond_group* g, const float* w1, void* w1h, void* w1l, void* stream) {
+  if (int rc = check_group(g)) return rc;
+  if (g->D != 12 || g->dstride != 12)
+    return fail(LPM_E_UNSUPPORTED, "split BF16 heads: %d channels (12 supported)", g->D);
+  if (!w1 || !w1h || !w1l) return fail(LPM_E_BADARG, "lpm_bond_w1_split_layout: null weights");
+  const int nslot = g->nv + g->ncell, NT = col_tiles(g->hid);
+  const int64_t n = (int64_t)((nslot + 1) / 2 + 2) * NT * 64;
+  w1_split_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(nslot, NT, g->hid, w1, (bf16x8*)w1h,
+                                                                                (bf16x8*)w1l);
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+int lpm_bond_heads_split(const lpm_bond_group* g, int32_t B, int32_t nc, int32_t s, const float* fv, const float* fc,
+                         const void* w1h, const void* w1l, const float* b1, const float* w2, const float* b2,
+                         const float* w3, const float* b3, float* out, const double* cp, double* kn, double* tv,
+                         void* stream) {
+  if (g && (g->D != 12 || g->dstride != 12))
+    return fail(LPM_E_UNSUPPORTED, "split BF16 heads: %d channels (12 supported)", g->D);
+  HeadArgs a{};
+  size_t lds = 0;
+  if (int rc = head_args(g, B, nc, s, fv, fc, (const float*)w1h, b1, w2, b2, w3, b3, out, cp, kn, tv,
+                         "lpm_bond_heads_split", a, lds))
+    return rc;
+  if (!w1l) return fail(LPM_E_BADARG, "lpm_bond_heads_split: null weights");
+  dim3 grid((unsigned)((a.nb + HB - 1) / HB), (unsigned)a.B);
+  hipStream_t st = (hipStream_t)stream;
+  const int nt = col_tiles(g->hid);
+  const bf16x8* wh = (const bf16x8*)w1h;
+  const bf16x8* wl = (const bf16x8*)w1l;
+  if (g->H == 16) {
+    if (nt == 1) bond_heads_split_kernel<16, 1><<<grid, 256, lds, st>>>(a, wh, wl);
+    else if (nt == 2) bond_heads_split_kernel<16, 2><<<grid, 256, lds, st>>>(a, wh, wl);
+    else bond_heads_split_kernel<16, 4><<<grid, 256, lds, st>>>(a, wh, wl);
+  } else {
+    if (nt == 1) bond_heads_split_kernel<8, 1><<<grid, 256, lds, st>>>(a, wh, wl);
+    else if (nt == 2) bond_heads_split_kernel<8, 2><<<grid, 256, lds, st>>>(a, wh, wl);
+    else bond_heads_split_kernel<8, 4><<<grid, 256, lds, st>>>(a, wh, wl);
+  }
+  LAUNCHCHK();
+  return LPM_OK;
 }
 
 }  // extern "C"

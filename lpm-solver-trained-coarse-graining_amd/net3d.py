@@ -37,6 +37,9 @@ _SIGS = {
     "lpm_bond_w1_size": (ctypes.c_size_t, [ctypes.POINTER(BondGroup)]),
     "lpm_bond_w1_layout": (I32, [ctypes.POINTER(BondGroup), P, P, P]),
     "lpm_bond_heads": (I32, [ctypes.POINTER(BondGroup), I32, I32, I32, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "lpm_bond_w1_split_size": (ctypes.c_size_t, [ctypes.POINTER(BondGroup)]),
+    "lpm_bond_w1_split_layout": (I32, [ctypes.POINTER(BondGroup), P, P, P, P]),
+    "lpm_bond_heads_split": (I32, [ctypes.POINTER(BondGroup), I32, I32, I32] + [P] * 14),
 }
 L.SIGS.update(_SIGS)
 
@@ -191,11 +194,19 @@ def iso_conv(x, w, b, gelu=False, proj=None, dstride=0):
 
 
 class BondNet(nn.Module):
-    """Exactly cubic-symmetric bond-parameter network (net3d.py:121-166); same parameters and names."""
+    """Exactly cubic-symmetric bond-parameter network (net3d.py:121-166); same parameters and names.
 
-    def __init__(self, s, cin=3, ch=24, D=12, hid=64):
+    precision: "fp32" (default; the first head layer on the FP32 matrix cores, as the reference computes it) or
+    "bf16x3" (opt-in, D = 12 only: that layer as three BF16 products of the split operands hi + lo, accumulated in
+    FP32, about twice as fast; the raw outputs stay within a few 1e-6 of the FP32 ones, relative to their largest)."""
+
+    def __init__(self, s, cin=3, ch=24, D=12, hid=64, precision="fp32"):
         super().__init__()
-        self.s, self.D = s, D
+        if precision not in ("fp32", "bf16x3"):
+            raise ValueError("precision must be 'fp32' or 'bf16x3'")
+        if precision == "bf16x3" and D != 12:
+            raise NotImplementedError("the split BF16 heads take D = 12 feature channels")
+        self.s, self.D, self.precision = s, D, precision
         self.trunk = nn.Sequential(IsoConv3d(cin, ch), nn.GELU(), IsoConv3d(ch, ch), nn.GELU(), IsoConv3d(ch, ch),
                                    nn.GELU(), nn.Conv3d(ch, D, 1))
         self.heads = nn.ModuleList()
@@ -285,19 +296,32 @@ class BondNet(nn.Module):
         outs = []
         lib = _lib()
         st = L.stream_of(Fv)
+        split = self.precision == "bf16x3"
         for (l1, l2, l3), t in zip(self.heads, self._device_tables(nc, Fv.device)):
             grp = t["grp"]
             w1 = l1.weight.detach().to(Fv.device, torch.float32).contiguous()
-            w1f = torch.empty(lib.lpm_bond_w1_size(ctypes.byref(grp)) // 4, device=Fv.device, dtype=torch.float32)
-            L.check(lib.lpm_bond_w1_layout(ctypes.byref(grp), _ptr(w1), _ptr(w1f), st))
+            if split:
+                nbytes = lib.lpm_bond_w1_split_size(ctypes.byref(grp))
+                w1h = torch.empty(nbytes, device=Fv.device, dtype=torch.uint8)
+                w1l = torch.empty(nbytes, device=Fv.device, dtype=torch.uint8)
+                L.check(lib.lpm_bond_w1_split_layout(ctypes.byref(grp), _ptr(w1), _ptr(w1h), _ptr(w1l), st))
+            else:
+                w1f = torch.empty(lib.lpm_bond_w1_size(ctypes.byref(grp)) // 4, device=Fv.device,
+                                  dtype=torch.float32)
+                L.check(lib.lpm_bond_w1_layout(ctypes.byref(grp), _ptr(w1), _ptr(w1f), st))
             par = [p.detach().to(Fv.device, torch.float32).contiguous() for p in
                    (l1.bias, l2.weight, l2.bias, l3.weight, l3.bias)]
             out = torch.empty((B, t["nb"], 2), device=Fv.device, dtype=torch.float32)
             if events is not None:
                 events.append((torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)))
                 events[-1][0].record()
-            L.check(lib.lpm_bond_heads(ctypes.byref(grp), B, nc, self.s, _ptr(Fv), _ptr(Fc), _ptr(w1f),
-                                       *[_ptr(p) for p in par], _ptr(out), _ptr(cp), _ptr(Kn), _ptr(Tv), st))
+            if split:
+                L.check(lib.lpm_bond_heads_split(ctypes.byref(grp), B, nc, self.s, _ptr(Fv), _ptr(Fc), _ptr(w1h),
+                                                 _ptr(w1l), *[_ptr(p) for p in par], _ptr(out), _ptr(cp), _ptr(Kn),
+                                                 _ptr(Tv), st))
+            else:
+                L.check(lib.lpm_bond_heads(ctypes.byref(grp), B, nc, self.s, _ptr(Fv), _ptr(Fc), _ptr(w1f),
+                                           *[_ptr(p) for p in par], _ptr(out), _ptr(cp), _ptr(Kn), _ptr(Tv), st))
             if events is not None:
                 events[-1][1].record()
             outs.append(out)

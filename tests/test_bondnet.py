@@ -392,3 +392,54 @@ def test_gpu_mask_cells_matches_oracle(dev):
     KR, TR = BR.mask_cells(Kn, Tv, cell, nc)
     assert np.abs(K.cpu().numpy() - KR).max() <= 1e-15 * np.abs(KR).max()
     assert np.abs(T.cpu().numpy() - TR).max() <= 1e-15 * np.abs(TR).max()
+
+
+def test_split_precision_needs_twelve_channels():
+    with pytest.raises(NotImplementedError):
+        net3d.BondNet(2, D=4, precision="bf16x3")
+    with pytest.raises(ValueError):
+        net3d.BondNet(2, precision="fp16")
+
+
+@pytest.mark.gpu
+def test_gpu_split_bf16_forward_matches_reference_fixture_and_fp32(dev):
+    """The opt-in split-BF16 first layer against the reference's outputs (same bar as the FP32 path) and against the
+    FP32 kernel (within 1e-5 of the largest output)."""
+    f, P = _fixture("bondnet_s2_full")
+    net = _net_from(P, 2, 24, 12, 64, dev)
+    split = net3d.BondNet(2, precision="bf16x3")
+    split.load_state_dict(net.state_dict())
+    split = split.to(dev)
+    X = torch.as_tensor(f["X"], device=dev)
+    o32 = net(X, int(f["nc"]))
+    osp = split(X, int(f["nc"]))
+    for gi in range(2):
+        ref = f["g%d/out" % gi]
+        got = osp[gi].cpu().numpy()
+        assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
+        assert np.abs(got - o32[gi].cpu().numpy()).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hid", [32, 64])
+def test_gpu_split_bf16_bond_params_match_oracle(dev, hid):
+    nc, s, B = 7, 2, 2
+    net, P = _random_net(s, 8, 12, hid, seed=hid)
+    split = net3d.BondNet(s, ch=8, D=12, hid=hid, precision="bf16x3")
+    split.load_state_dict(net.state_dict())
+    split = split.to(dev)
+    import geom3d as G
+    n = nc * s
+    Xs, cps = [], []
+    for b in range(B):
+        phase, X = _micro(n, s, 40 + b, contrast=2.0 + b)
+        Xs.append(X)
+        Cv = G.iso_C(np.where(phase == 1, 2.0 + b, 1.0), np.where(phase == 1, 0.3, 0.2))
+        cps.append(net3d.voigt_cell_params(Cv, n, s).numpy())
+    X = np.stack(Xs)
+    Kn, Tv = split.bond_params(torch.as_tensor(X, device=dev), nc, torch.as_tensor(np.stack(cps), device=dev))
+    outs, groups = BR.forward(X, nc, s, P)
+    for b in range(B):
+        KnR, TvR = BR.post([o[b] for o in outs], groups, cps[b], nc)
+        assert np.abs(Kn[b].cpu().numpy() - KnR).max() <= TOL32 * 10 * np.abs(KnR).max()
+        assert np.abs(Tv[b].cpu().numpy() - TvR).max() <= TOL32 * 10 * np.abs(KnR).max()
