@@ -583,6 +583,35 @@ def bondnet_cpu_baseline(s, seed, n_sample=48):
     return dt, sum(len(g["bonds"]) for g in groups)
 
 
+def _infer_split(net, Xd, cp, nc, s, Kn32, steps):
+    """The same inference with the opt-in split-BF16 first layer (BondNet(precision="bf16x3"), same weights):
+    time per inference, per-group head launches and the largest deviation of its Kn from the FP32 path's."""
+    import torch
+    import net3d
+    sp = net3d.BondNet(s, precision="bf16x3")
+    sp.load_state_dict(net.state_dict())
+    sp = sp.to(Xd.device)
+    Kn, _ = sp.bond_params(Xd, nc, cp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        Kn, _ = sp.bond_params(Xd, nc, cp)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    Fv, Fc = sp.features(Xd)
+    evs = []
+    sp.heads_run(Fv, Fc, nc, events=evs)
+    torch.cuda.synchronize()
+    dev_rel = float((Kn - Kn32).abs().max() / Kn32.abs().max())
+    print("[bench] split-BF16 heads: %.1f ms per inference, Kn within %.1e of FP32" % (dt * 1e3, dev_rel),
+          file=sys.stderr, flush=True)
+    return {"value": 1.0 / dt, "unit": "inferences/s", "ms_per_step": dt * 1e3,
+            "heads_axial_ms": evs[0][0].elapsed_time(evs[0][1]), "heads_diagonal_ms": evs[1][0].elapsed_time(evs[1][1]),
+            "max_rel_dev_kn_vs_fp32": dev_rel,
+            "note": "opt-in: first head layer as three BF16 matrix-core products of the hi/lo split operands, FP32 "
+                    "accumulation; not the headline value"}
+
+
 def _infer_then_solve(net, Xd, cp, nc, s, tol, reps=2):
     """Inference fused in front of the solve: Kn / Tv of the nc^3 coarse lattice (spacing h = s) from the network,
     packed, then tension z (planes z = 0 and z = nc - 1 fixed, top displaced by 1e-3 of the height) solved by
@@ -690,6 +719,7 @@ def bench_infer(a):
                         "alg_flops_per_launch": flops[1], "avg_us": head_ms[1] * 1e3,
                         "axial": {"alg_flops_per_launch": flops[0], "avg_us": head_ms[0] * 1e3,
                                   "achieved": flops[0] / (head_ms[0] * 1e-3) / 1e12}}}
+    out["split_bf16"] = _infer_split(net, Xd, cp, nc, s, Kn, a.steps)
     out["pipeline"] = _infer_then_solve(net, Xd, cp, nc, s, a.tol, reps=2)
     if not a.no_cpu:
         print("[bench] cpu baseline (oracle forward on a 48^3 field) ...", file=sys.stderr, flush=True)
