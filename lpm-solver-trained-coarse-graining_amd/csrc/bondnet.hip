@@ -191,6 +191,12 @@ constexpr int HMAX = 64;
 
 // window slots rounded up to even (the first-layer loop runs two slots per iteration)
 __host__ __device__ inline int kpad_of(int nslot) { return nslot + (nslot & 1); }
+// Slot pairs of the split-BF16 W1 fragments held in memory: the pair count rounded up to even (the heads loop reads
+// pairs two at a time) plus the two zero pairs its two-deep prefetch reads past the end.
+__host__ __device__ inline int split_pairs_of(int nslot) {
+  const int npair = (nslot + 1) / 2;
+  return npair + (npair & 1) + 2;
+}
 
 struct HeadArgs {
   int ndir, nv, ncell, nslot, H, D, ds, hid, Q, xs_floats;
@@ -504,7 +510,9 @@ __global__ __launch_bounds__(256) void bond_heads_split_kernel(HeadArgs a, const
   }
   const int npair = (nslot + 1) / 2;
   int lp = 0;
-  // W1 fragments two slot pairs ahead (W1 holds two zero pairs past the end); window fragments at their step
+  // W1 fragments two slot pairs ahead; window fragments at their step.  The loop makes 2 + 2 ceil(npair / 2)
+  // fragment loads, so the last one reads pair npair + (npair & 1) + 1 < split_pairs_of(nslot): W1 holds the pairs
+  // rounded up to even plus two zero pairs past the end.
   auto loadb = [&](bf16x8 (&bh)[NT], bf16x8 (&bl)[NT]) {
     const int64_t wo = (int64_t)lp * NT * 64 + lane;
 #pragma unroll
@@ -566,12 +574,12 @@ __global__ __launch_bounds__(256) void bond_heads_split_kernel(HeadArgs a, const
   heads_epilogue<H, NT>(a, sm, acc, tid, bi, b0);
 }
 
-// W1 split into hi / lo BF16 fragments: [slot pair][column tile][lane] x 8, two zero pairs past the end
+// W1 split into hi / lo BF16 fragments: [slot pair][column tile][lane] x 8, split_pairs_of(nslot) pairs (zero past
+// the last slot)
 __global__ void w1_split_kernel(int nslot, int NT, int hid, const float* __restrict__ w1, bf16x8* __restrict__ wh,
                                 bf16x8* __restrict__ wl) {
-  const int npair = (nslot + 1) / 2;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)(npair + 2) * NT * 64) return;
+  if (idx >= (int64_t)split_pairs_of(nslot) * NT * 64) return;
   const int lane = (int)(idx & 63);
   const int64_t r = idx >> 6;
   const int n = (int)(r % NT), pr = (int)(r / NT);
@@ -783,7 +791,7 @@ int lpm_bond_heads(const lpm_bond_group* g, int32_t B, int32_t nc, int32_t s, co
 
 size_t lpm_bond_w1_split_size(const lpm_bond_group* g) {
   if (!g || g->hid < 1) return 0;
-  return (size_t)((g->nv + g->ncell + 1) / 2 + 2) * col_tiles(g->hid) * 64 * sizeof(bf16x8);
+  return (size_t)split_pairs_of(g->nv + g->ncell) * col_tiles(g->hid) * 64 * sizeof(bf16x8);
 }
 
 int lpm_bond_w1_split_layout(const lpm_bond_group* g, const float* w1, void* w1h, void* w1l, void* stream) {
@@ -792,7 +800,7 @@ int lpm_bond_w1_split_layout(const lpm_bond_group* g, const float* w1, void* w1h
     return fail(LPM_E_UNSUPPORTED, "split BF16 heads: %d channels (12 supported)", g->D);
   if (!w1 || !w1h || !w1l) return fail(LPM_E_BADARG, "lpm_bond_w1_split_layout: null weights");
   const int nslot = g->nv + g->ncell, NT = col_tiles(g->hid);
-  const int64_t n = (int64_t)((nslot + 1) / 2 + 2) * NT * 64;
+  const int64_t n = (int64_t)split_pairs_of(nslot) * NT * 64;
   w1_split_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(nslot, NT, g->hid, w1, (bf16x8*)w1h,
                                                                                 (bf16x8*)w1l);
   LAUNCHCHK();

@@ -3,6 +3,7 @@
 torch's conv3d with the expanded kernel, and the cube-symmetry equivariance that the reference guarantees; the
 product's window tables against the reference's BondWindows; the HIP path (net3d.py of the package) against the oracle
 and the fixtures.  GPU results are FP32 against FP64 references: tolerances are relative to the largest output."""
+import ctypes
 import os
 
 import numpy as np
@@ -420,10 +421,37 @@ def test_gpu_split_bf16_forward_matches_reference_fixture_and_fp32(dev):
         assert np.abs(got - o32[gi].cpu().numpy()).max() <= 1e-5 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("nslot", [50, 51, 62, 88, 100, 162, 174])
+@pytest.mark.parametrize("hid", [16, 32, 64])
+def test_split_w1_buffer_covers_every_fragment_load(nslot, hid):
+    """The split-BF16 heads loop (bond_heads_split_kernel) loads W1 fragments two slot pairs ahead: two loads before
+    the loop and two per pass over pairs (pr, pr + 1).  Replay that schedule and check that the largest pair index
+    it reads lies inside the buffer that lpm_bond_w1_split_size reports for the group (no GPU needed)."""
+    lib = net3d._lib()
+    grp = net3d.BondGroup()
+    grp.ndir, grp.nv, grp.ncell, grp.H, grp.D, grp.dstride, grp.hid = 3, nslot - 8, 8, 16, 12, 12, hid
+    nbytes = lib.lpm_bond_w1_split_size(ctypes.byref(grp))
+    nt = 1 if hid <= 16 else (2 if hid <= 32 else 4)
+    pairs_held = nbytes // (nt * 64 * 16)                      # one bf16x8 (16 bytes) per lane, column tile and pair
+    assert pairs_held * nt * 64 * 16 == nbytes
+    npair = (nslot + 1) // 2
+    lp, reads = 0, []
+    for _ in range(2):
+        reads.append(lp)
+        lp += 1
+    for pr in range(0, npair, 2):
+        for _ in range(2):
+            reads.append(lp)
+            lp += 1
+    assert max(reads) < pairs_held
+    # every pair a step consumes holds real slots; the pairs read past npair are the zero padding
+    assert pairs_held >= npair
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("hid", [32, 64])
-def test_gpu_split_bf16_bond_params_match_oracle(dev, hid):
-    nc, s, B = 7, 2, 2
+@pytest.mark.parametrize("s,hid", [(2, 16), (2, 32), (2, 64), (3, 16), (3, 64)])
+def test_gpu_split_bf16_bond_params_match_oracle(dev, s, hid):
+    nc, B = (7, 2) if s == 2 else (5, 1)
     net, P = _random_net(s, 8, 12, hid, seed=hid)
     split = net3d.BondNet(s, ch=8, D=12, hid=hid, precision="bf16x3")
     split.load_state_dict(net.state_dict())
