@@ -32,6 +32,31 @@ def test_abi_exports_every_declared_symbol():
         assert n in names
 
 
+def test_integration_binding_matches_the_library_structs(monkeypatch):
+    """The ctypes binding INTEGRATION.md gives maintainers, executed as written: its Grid / Bcast have the size and
+    field offsets of the library's own binding (lpm_grid is 48 bytes with the bond-word pointer and its strides),
+    and its lpm_apply signature is the one the library is driven through."""
+    import _lpmlib as L
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```python\n(.*?)```", txt, re.S)
+    snippet = next(b for b in blocks if "class Grid" in b)
+    monkeypatch.setenv("LPM_LIB", L.LIB_PATH)
+    ns = {}
+    exec(compile(snippet, "INTEGRATION.md", "exec"), ns)
+    G, B = ns["Grid"], ns["Bcast"]
+    assert ctypes.sizeof(G) == ctypes.sizeof(L.Grid) == 48
+    assert ctypes.sizeof(B) == ctypes.sizeof(L.Bcast) == 16
+    for (name, _), (name2, _) in zip(G._fields_, L.Grid._fields_):
+        assert name == name2 and getattr(G, name).offset == getattr(L.Grid, name2).offset
+    assert len(G._fields_) == len(L.Grid._fields_)
+    assert [ctypes.sizeof(t) for t in ns["lib"].lpm_apply.argtypes] == \
+        [ctypes.sizeof(t) for t in L.SIGS["lpm_apply"][1]]
+    # a bad dimension is refused through the snippet's binding (no GPU call is made)
+    g = G(4, 2, 2, 2, 1.0, None, B(0, 0))
+    rc = ns["lib"].lpm_apply(ctypes.byref(g), 1, 1, 0, 0, None, B(0, 0), None, None, B(0, 0), None, None)
+    assert rc == L.LPM_E_BADARG
+
+
 def test_abi_rejects_bad_arguments_without_a_gpu():
     import _lpmlib as L
     g = L.Grid(4, 2, 2, 2, 1.0)
