@@ -7,11 +7,23 @@
 
 namespace lpm {
 
+template <typename T>
+__device__ __forceinline__ bool is_finite(T v) {
+  return __builtin_isfinite(v);
+}
+template <typename T>
+__device__ __forceinline__ T nan_of() {
+  return (T)__builtin_nan("");
+}
+
 // per-system scalars (type T) and integer state
 enum CgScal : int { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_BN, S_TOL, S_REL, NSCAL };
 // I_XUPD: the x += alpha p of the current iteration is still to be applied (set by cg_fin_alpha, consumed by
-// cg_pupdate, which folds the x update into its pass over p)
-enum CgInt : int { I_IT = 0, I_ACTIVE, I_MAXIT, I_CONV, I_XUPD, NINT };
+// cg_pupdate, which folds the x update into its pass over p).
+// I_NOTSPD: the system stopped because K (or M) is not positive definite on it: p.Ap <= 0 for a nonzero search
+// direction, or r.z < 0.  The reference's dense path raises FloatingPointError there (coarse2d/torch_lpm.py:20-25);
+// the host reads the flag at its convergence poll and returns LPM_E_NOT_SPD.
+enum CgInt : int { I_IT = 0, I_ACTIVE, I_MAXIT, I_CONV, I_XUPD, I_NOTSPD, NINT };
 
 template <typename T>
 struct CgVec {
@@ -113,10 +125,16 @@ __global__ __launch_bounds__(256) void cg_fin_init_kernel(T* partial, int nblk, 
     sc[S_ALPHA] = T(0);
     sc[S_BETA] = T(0);
     is[I_IT] = 0;
-    const bool done = !(bn > T(0)) || sqrt(rr) <= sc[S_TOL] * bn;
-    sc[S_REL] = bn > T(0) ? sqrt(rr) / bn : T(0);
-    is[I_ACTIVE] = done ? 0 : 1;
+    // a NaN / Inf in b, M or the mask stops the system with a NaN relative residual (reported as non-finite by the
+    // host) instead of passing for a zero right-hand side
+    const bool finite = is_finite(bb) && is_finite(rr) && is_finite(rz);
+    const bool done = finite && (!(bn > T(0)) || sqrt(rr) <= sc[S_TOL] * bn);
+    sc[S_REL] = !finite ? nan_of<T>() : (bn > T(0) ? sqrt(rr) / bn : T(0));
+    // r.M r < 0 for the first residual: an indefinite diagonal block of K
+    const bool indef = finite && !done && rz < T(0);
+    is[I_ACTIVE] = (done || indef || !finite) ? 0 : 1;
     is[I_CONV] = done ? 1 : 0;
+    is[I_NOTSPD] = indef ? 1 : 0;
   }
 }
 
@@ -134,7 +152,17 @@ __global__ __launch_bounds__(256) void cg_fin_alpha_kernel(const T* partial, int
   if (threadIdx.x == 0) {
     T* sc = scal + (int64_t)s * NSCAL;
     sc[S_PAP] = pap;
-    sc[S_ALPHA] = pap != T(0) ? sc[S_RZ] / pap : T(0);
+    // an active system has r.z > 0, so p = z + beta p != 0 and p.Ap > 0 for SPD K; p.Ap <= 0 means K is not
+    // positive definite on the free DOFs: stop the system without an update (a NaN p.Ap is left to the
+    // non-finite check of the residual)
+    if (pap <= T(0)) {
+      sc[S_ALPHA] = T(0);
+      is[I_NOTSPD] = 1;
+      is[I_ACTIVE] = 0;
+      is[I_XUPD] = 0;
+    } else {
+      sc[S_ALPHA] = sc[S_RZ] / pap;
+    }
   }
 }
 
@@ -237,6 +265,14 @@ __global__ __launch_bounds__(256) void cg_fin_beta_kernel(const T* partial, int 
     if (rn <= sc[S_TOL] * sc[S_BN]) {
       is[I_ACTIVE] = 0;
       is[I_CONV] = 1;
+    } else if (!is_finite(rn) || !is_finite(rzn)) {
+      // NaN / Inf from the coefficients or the preconditioner: stop; the host reports the non-finite residual
+      sc[S_REL] = nan_of<T>();
+      is[I_ACTIVE] = 0;
+    } else if (rzn < T(0)) {
+      // r.M r < 0 for r != 0: the preconditioner (the inverse diagonal blocks of K) is indefinite
+      is[I_ACTIVE] = 0;
+      is[I_NOTSPD] = 1;
     } else if (it >= is[I_MAXIT]) {
       is[I_ACTIVE] = 0;
     } else {

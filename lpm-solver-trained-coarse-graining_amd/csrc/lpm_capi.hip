@@ -253,6 +253,8 @@ __global__ void cg_setup_kernel(T* scal, int* ist, int nsys, T tol, int maxit) {
   ist[(int64_t)s * NINT + I_ACTIVE] = 0;
   ist[(int64_t)s * NINT + I_IT] = 0;
   ist[(int64_t)s * NINT + I_XUPD] = 0;
+  ist[(int64_t)s * NINT + I_NOTSPD] = 0;
+  ist[(int64_t)s * NINT + I_CONV] = 0;
 }
 // per-system tolerance from a device array (refinement inner solves)
 template <typename T>
@@ -402,7 +404,19 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
     HIPCHK(hipStreamSynchronize(st));
     prof_flush();
     bool any = false;
-    for (int s = 0; s < nsys; ++s) any |= host_ist[(size_t)s * NINT + I_ACTIVE] != 0;
+    int notspd = -1;
+    for (int s = 0; s < nsys; ++s) {
+      any |= host_ist[(size_t)s * NINT + I_ACTIVE] != 0;
+      if (notspd < 0 && host_ist[(size_t)s * NINT + I_NOTSPD]) notspd = s;
+    }
+    if (notspd >= 0) {
+      if (gexec) {
+        (void)hipGraphExecDestroy(gexec);
+        (void)hipGraphDestroy(graph);
+      }
+      return fail(LPM_E_NOT_SPD, "(P)CG: system %d is not positive definite (p.Ap <= 0 or r.z < 0 at iteration %d)",
+                  notspd, host_ist[(size_t)notspd * NINT + I_IT]);
+    }
     if (!any) break;
   }
   if (gexec) {
@@ -415,6 +429,8 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
   HIPCHK(hipStreamSynchronize(st));
   bool bad = false;
   for (int s = 0; s < nsys; ++s) {
+    if (host_ist[(size_t)s * NINT + I_NOTSPD])
+      return fail(LPM_E_NOT_SPD, "(P)CG: system %d is not positive definite", s);
     if (iters) iters[s] = host_ist[(size_t)s * NINT + I_IT];
     if (relres) relres[s] = (double)host_scal[(size_t)s * NSCAL + S_REL];
     if (conv) conv[s] = host_ist[(size_t)s * NINT + I_CONV];

@@ -206,3 +206,18 @@ def test_stability_penalty(dev):
     sc = np.abs(Kn).mean()
     assert rel(lam, lmin / sc) < 1e-12
     assert float(pen[0]) > 0 and lmin[0, 3, 0] < 0
+    # H of a group is the Hessian of the group's bond energy 1/2 sum Kn e^2 + 1/2 (sum Tv e)(sum e) over its live
+    # bonds (the particle energy of coarse3d/lin3d.py:213-236 split by shell): compare the quadratic forms
+    rng = np.random.default_rng(11)
+    shell = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+    for p in (0, 3, 7, 12):
+        for c in (0, 1):
+            ks = np.nonzero(shell == c)[0]
+            live = ref.valid[p, ks]
+            e = rng.standard_normal(4) * live
+            kn, tv = Kn[0, p, ks] * live, Tv[0, p, ks] * live
+            U = 0.5 * (kn * e * e).sum() + 0.5 * (tv * e).sum() * e.sum()
+            assert abs(0.5 * e @ H[0, p, c] @ e - U) <= 1e-12 * max(1.0, abs(U))
+    # the penalty is the sum of squared shortfalls of lambda_min / mean|Kn| below frac
+    frac = 0.05
+    assert abs(float(pen[0]) - (np.maximum(frac - lmin[0] / sc, 0.0) ** 2).sum()) <= 1e-12 * float(pen[0])
