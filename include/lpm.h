@@ -10,7 +10,8 @@
  * directed coefficients (batch, N, NN) with NN = 18 (3D) or 8 (2D) in the reference slot order; packed
  * coefficients are (batch, NN/2, P) forward-slot planes (valid for symmetric parameters), each plane holding
  * G zero guards, the N particles and G zero guards (P = N + 2G = lpm_packed_len, G = nx ny + nx + 1 in 3D,
- * nx + 1 in 2D), so the kernels read a neighbour's entry without a bounds test.  A batch of systems is
+ * nx + 1 in 2D), so the kernels read a neighbour's entry without a bounds test; the entry of a forward slot whose
+ * bond leaves the box is zero (lpm_pack_coef writes it so), like the guards.  A batch of systems is
  * (nb, nl) = (specimens, load cases); every operand carries element strides (sb, sl) for the two batch indices,
  * 0 meaning broadcast, which is the reference's (B,1,N,18) against (1|B,L,N,3) convention
  * (coarse3d/torch_lpm3d.py:212-227).

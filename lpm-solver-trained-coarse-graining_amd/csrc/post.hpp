@@ -371,7 +371,8 @@ __global__ __launch_bounds__(256) void pack_coef_kernel(Box box, const CI* __res
 #pragma unroll
   for (int k = 0; k < NN; k += 2) {
     const int64_t q = box.nbr<DIM>(x, y, z, k);
-    const CI a1 = kk[p * NN + k], t1 = tt[p * NN + k];
+    // a slot that leaves the box is no bond: its plane entry is zero (the fused K u kernels rely on it)
+    const CI a1 = q >= 0 ? kk[p * NN + k] : CI(0), t1 = q >= 0 ? tt[p * NN + k] : CI(0);
     kp[((int64_t)b * (NN / 2) + k / 2) * box.ps + e] = (CO)a1;
     tp[((int64_t)b * (NN / 2) + k / 2) * box.ps + e] = (CO)t1;
     if (q >= 0) {

@@ -449,9 +449,22 @@ def test_split_w1_buffer_covers_every_fragment_load(nslot, hid):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("s,hid", [(2, 16), (2, 32), (2, 64), (3, 16), (3, 64)])
-def test_gpu_split_bf16_bond_params_match_oracle(dev, s, hid):
-    nc, B = (7, 2) if s == 2 else (5, 1)
+def test_gpu_split_bf16_refuses_windows_beyond_its_lds(dev):
+    """At s = 3 a bond's window (88 / 100 slots of 12 channels, hi and lo planes) does not fit the split kernel's LDS
+    budget: the call is refused before any launch (the FP32 heads serve s = 3)."""
+    net, P = _random_net(3, 8, 12, 16, seed=5)
+    split = net3d.BondNet(3, ch=8, D=12, hid=16, precision="bf16x3")
+    split.load_state_dict(net.state_dict())
+    split = split.to(dev)
+    phase, X = _micro(15, 3, 41)
+    with pytest.raises(ValueError, match="LDS"):
+        split(torch.as_tensor(X[None], device=dev), 5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hid", [16, 32, 64])
+def test_gpu_split_bf16_bond_params_match_oracle(dev, hid):
+    nc, s, B = 7, 2, 2
     net, P = _random_net(s, 8, 12, hid, seed=hid)
     split = net3d.BondNet(s, ch=8, D=12, hid=hid, precision="bf16x3")
     split.load_state_dict(net.state_dict())

@@ -62,6 +62,28 @@ def test_apply_packed_matches_directed(dev, dims, dt):
     assert rel(out, R.apply(lat, u, Kn, Tv)) < TOL[dt]
 
 
+@pytest.mark.parametrize("dims", [(9, 7, 6), (70, 9, 5), (11, 8)])
+def test_slots_leaving_the_box_are_ignored_and_packed_as_zero(dev, dims):
+    """Entries of slots whose bond leaves the box are no bonds: garbage there changes neither K u (directed layout)
+    nor the packed planes, which hold zeros on such slots (the fused kernels rely on it)."""
+    lat, Kn, Tv, u = case(dims, seed=5, two_phase=True)
+    rng = np.random.default_rng(8)
+    out_of_box = ~lat.valid
+    Kg, Tg = Kn.copy(), Tv.copy()
+    Kg[out_of_box] = rng.uniform(5.0, 9.0, out_of_box.sum())
+    Tg[out_of_box] = rng.uniform(-3.0, 3.0, out_of_box.sum())
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    ref = R.apply(lat, u, Kn, Tv)
+    assert rel(ops.apply(dims, 1.0, t(u), t(Kg), t(Tg)), ref) < 1e-12
+    kp, tp, asym = ops.pack(dims, t(Kg), t(Tg))
+    kp0, tp0, _ = ops.pack(dims, t(Kn), t(Tv))
+    assert int(asym.sum()) == 0
+    assert torch.equal(kp, kp0) and torch.equal(tp, tp0)
+    for f32 in (False, True):
+        k, tt_, uu = (kp.float(), tp.float(), t(u).float()) if f32 else (kp, tp, t(u))
+        assert rel(ops.apply(dims, 1.0, uu, k, tt_, layout=L.PACKED), ref) < (2e-6 if f32 else 1e-12)
+
+
 def test_pack_flags_asymmetry(dev):
     lat, Kn, Tv, u = case((6, 5, 4), seed=1)
     Kn = Kn.copy()
