@@ -9,9 +9,10 @@
  * Lattices are full boxes, particle id p = (z * ny + y) * nx + x; vectors are (systems, N, dim) interleaved,
  * directed coefficients (batch, N, NN) with NN = 18 (3D) or 8 (2D) in the reference slot order; packed
  * coefficients are (batch, NN/2, P) forward-slot planes (valid for symmetric parameters), each plane holding
- * G zero guards, the N particles and G zero guards (P = N + 2G = lpm_packed_len, G = nx ny + nx + 1 in 3D,
- * nx + 1 in 2D), so the kernels read a neighbour's entry without a bounds test; the entry of a forward slot whose
- * bond leaves the box is zero (lpm_pack_coef writes it so), like the guards.  A batch of systems is
+ * G zero guards, the N particles and at least G trailing zero guards (G = nx ny + nx + 1 in 3D, nx + 1 in 2D,
+ * rounded up to 64; P = N + 2G rounded up to 64 = lpm_packed_len), so the kernels read a neighbour's entry without a
+ * bounds test and FP32 planes start 256-byte aligned; the entry of a forward slot whose bond leaves the box is zero
+ * (lpm_pack_coef writes it so), like the guards.  A batch of systems is
  * (nb, nl) = (specimens, load cases); every operand carries element strides (sb, sl) for the two batch indices,
  * 0 meaning broadcast, which is the reference's (B,1,N,18) against (1|B,L,N,3) convention
  * (coarse3d/torch_lpm3d.py:212-227).
@@ -75,6 +76,10 @@ void lpm_profile_ku(int32_t enable);
 /* Tiling of the 3D K u kernels: wide > 0 selects the 64-wide x tile, < 0 the narrow one, 0 automatic; zc planes
  * per block (0 automatic).  Process-wide; no counterpart in the reference (tuning knob). */
 void lpm_set_ku_tiling(int32_t wide, int32_t zc);
+/* lean = 1: 3D FP32 K u on packed coefficients without bond words runs the lean kernel (three blocks per CU,
+ * own-bond terms summed as the shell sums form, 64-wide tiles); 0 (default): the general kernel.  Process-wide; no
+ * counterpart in the reference (tuning / comparison knob). */
+void lpm_set_ku_kernel(int32_t lean);
 void lpm_profile_ku_read(double* total_ms, int64_t* launches);
 
 /* Bond words from solid flags (nbatch, N) bytes, nonzero = solid: bit k of p set iff slot k stays in the box and
@@ -95,7 +100,7 @@ int lpm_residual(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32
                  lpm_bcast bu, const void* kn, const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm,
                  const void* rhs, lpm_bcast brhs, void* out, void* work, void* stream);
 
-/* Plane length P of the packed layout (N plus the two guard bands). */
+/* Plane length P of the packed layout (N plus the guard bands, rounded up to 64 values). */
 int64_t lpm_packed_len(const lpm_grid* g);
 
 /* Directed (nbatch, N, NN) -> packed (nbatch, NN/2, P) forward planes, converting dtype_in -> dtype_out.

@@ -40,6 +40,7 @@ SIGS = {
     "lpm_version": (I32, []),
     "lpm_profile_ku": (None, [I32]),
     "lpm_set_ku_tiling": (None, [I32, I32]),
+    "lpm_set_ku_kernel": (None, [I32]),
     "lpm_packed_len": (ctypes.c_int64, [GP]),
     "lpm_profile_ku_read": (None, [P, P]),
     "lpm_bond_words": (I32, [GP, I32, P, P, P]),

@@ -32,6 +32,12 @@ struct KuGeom {
 
 template <typename T, int MODE, int EPI, bool BONDS>
 int launch_ku3(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const lpm::Ku3Args<T, T>& a0, hipStream_t st);
+// lean 3D kernel (ku3l.hpp): FP32, packed coefficients, no bond words, 64 x 4 tiles, epilogues PLAIN / MASKDOT;
+// blocks resident per CU (the waves per SIMD it is compiled for)
+constexpr int kLeanBlocksPerCU = 3;
+template <int EPI>
+int launch_ku3l(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const lpm::Ku3Args<float, float>& a0,
+                hipStream_t st);
 template <typename T, int MODE, int EPI, bool BONDS>
 int launch_ku2(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const lpm::Ku2Args<T, T>& a0, hipStream_t st);
 

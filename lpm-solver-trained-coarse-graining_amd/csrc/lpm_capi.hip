@@ -62,7 +62,7 @@ int prof_flush() {
 inline Bcast bc_of(lpm_bcast b) { return Bcast{b.sb, b.sl}; }
 inline int64_t nsites(const lpm_grid* g) { return (int64_t)g->nx * g->ny * (g->dim == 3 ? g->nz : 1); }
 inline int64_t guard_of(const lpm_grid* g) { return packed_guard(g->dim, g->nx, g->ny); }
-inline int64_t pstride_of(const lpm_grid* g) { return nsites(g) + 2 * guard_of(g); }
+inline int64_t pstride_of(const lpm_grid* g) { return packed_stride(g->dim, g->nx, g->ny, nsites(g)); }
 inline Box box_of(const lpm_grid* g) {
   Box b{g->nx, g->ny, g->dim == 3 ? g->nz : 1, pstride_of(g)};
   b.bonds = g->bonds;
@@ -96,8 +96,10 @@ hipStream_t capture_stream() {
 // blocks per system of the grid-stride vector kernels: at most 1024, so the per-system partial sums stay short
 inline int vec_blocks(int64_t N) { return (int)std::min<int64_t>(1024, std::max<int64_t>(1, (N + 255) / 256)); }
 
-// tiling overrides (0 = automatic): x tile width of the 3D kernels and planes per block
-int g_tile_wide = 0, g_zc = 0;
+// tiling overrides (0 = automatic): x tile width of the 3D kernels and planes per block; g_lean = 1 sends FP32 packed
+// K u (plain and masked-dot forms) to the lean kernel.  It is opt-in: at 256^3 it is faster alone (443 vs 477 us)
+// but slower inside the PCG iteration (471 vs 444 us), where the vector kernels between launches evict the caches.
+int g_tile_wide = 0, g_zc = 0, g_lean = 0;
 
 // compute units of the current device (cached per device; 256 when the runtime cannot tell)
 int cu_count() {
@@ -114,7 +116,8 @@ int cu_count() {
 
 // tile shapes: 3D float 64x4 or 32x8, double 32x4 or 16x8; 2D float 32x8, double 32x4
 template <int DIM, typename T>
-KuGeom ku_geom(const lpm_grid* g, int nb, int nl, bool bonds) {
+KuGeom ku_geom(const lpm_grid* g, int nb, int nl, bool bonds, int lean_switch = -1) {
+  const int use_lean = lean_switch < 0 ? g_lean : lean_switch;
   KuGeom k{};
   if constexpr (DIM == 3) {
     const bool wide = g_tile_wide ? g_tile_wide > 0 : g->nx >= 48;
@@ -129,15 +132,17 @@ KuGeom ku_geom(const lpm_grid* g, int nb, int nl, bool bonds) {
     k.ntx = (g->nx + k.tx - 1) / k.tx;
     k.nty = (g->ny + k.ty - 1) / k.ty;
     const int64_t xy = (int64_t)k.ntx * k.nty * nb * nl;
-    // Every 3D tile kernel keeps two blocks resident per CU (`slots` blocks on the chip).  The z range of a column
-    // tile is split into chunks of at least 8 planes so that the launch has about `slots` blocks.  Rounding the
-    // chunk count up can leave a short tail round (slots < blocks < 1.5 slots) that costs nearly a whole round;
-    // then the count is rounded down to one round.  Longer overshoots (1.5 slots and more) are kept: the block
-    // scheduler refills freed slots, and they measured faster than one partly filled round.
+    // The general 3D tile kernels keep two blocks resident per CU, the lean FP32 kernel (64 x 4 tiles without bond
+    // words) kLeanBlocksPerCU (`slots` blocks on the chip).  The z range of a column tile is split into chunks of at
+    // least 8 planes so that the launch has about `slots` blocks.  Rounding the chunk count up can leave a short tail round
+    // (slots < blocks < 1.5 slots) that costs nearly a whole round; then the count is rounded down to one round.
+    // Longer overshoots (1.5 slots and more) are kept: the block scheduler refills freed slots, and they measured
+    // faster than one partly filled round.
+    const bool lean = sizeof(T) == 4 && use_lean && k.tx == 64 && k.ty == 4 && !bonds;
     if (g_zc > 0) {
       k.zc = g_zc;
     } else {
-      const int64_t slots = 2 * (int64_t)cu_count();
+      const int64_t slots = (lean ? kLeanBlocksPerCU : 2) * (int64_t)cu_count();
       auto zc_for = [&](int64_t chunks) { return std::max<int>(8, (int)((g->nz + chunks - 1) / chunks)); };
       k.zc = zc_for(std::max<int64_t>(1, (slots + xy - 1) / xy));
       const int64_t blocks = xy * ((g->nz + k.zc - 1) / k.zc);
@@ -161,10 +166,14 @@ template <int DIM, typename T>
 KuGeom ku_geom(const lpm_grid* g, int nb, int nl) {
   return ku_geom<DIM, T>(g, nb, nl, g->bonds != nullptr);
 }
-// blocks per system of the K u launch with or without bond words, whichever is larger (workspace sizing)
+// blocks per system of the K u launch with or without bond words, general or lean kernel, whichever is larger
+// (workspace sizing, independent of the kernel switch at the time of the query)
 template <int DIM, typename T>
 int ku_blocks_max(const lpm_grid* g, int nsys) {
-  return (int)std::max(ku_geom<DIM, T>(g, 1, nsys, false).grid.x, ku_geom<DIM, T>(g, 1, nsys, true).grid.x);
+  unsigned m = 0;
+  for (int lean = 0; lean < 2; ++lean)
+    for (int bonds = 0; bonds < 2; ++bonds) m = std::max(m, ku_geom<DIM, T>(g, 1, nsys, bonds != 0, lean).grid.x);
+  return (int)m;
 }
 
 // one K u launch with epilogue EPI for either dimension / layout
@@ -214,6 +223,12 @@ int ku_call(const lpm_grid* g, int nb, int nl, int layout, const KuGeom& kg, con
     if (g->bonds)
       return layout == LPM_COEF_PACKED ? launch_ku3<T, COEF_PACKED, EPI, true>(g, nb, nl, kg, a, st)
                                        : launch_ku3<T, COEF_DIRECTED, EPI, true>(g, nb, nl, kg, a, st);
+    if constexpr (sizeof(T) == 4 && (EPI == EPI_PLAIN || EPI == EPI_MASKDOT)) {
+      // the PCG hot path: lean kernel when the whole packed set and the vectors have 32-bit byte offsets
+      if (layout == LPM_COEF_PACKED && g_lean && kg.tx == 64 && kg.ty == 4 &&
+          (uint64_t)pstride_of(g) * 9u * 4u < (1ull << 32) && (uint64_t)N * 12u < (1ull << 32))
+        return launch_ku3l<EPI>(g, nb, nl, kg, a, st);
+    }
     return layout == LPM_COEF_PACKED ? launch_ku3<T, COEF_PACKED, EPI, false>(g, nb, nl, kg, a, st)
                                      : launch_ku3<T, COEF_DIRECTED, EPI, false>(g, nb, nl, kg, a, st);
   } else {
@@ -454,6 +469,8 @@ void lpm_set_ku_tiling(int32_t wide, int32_t zc) {
   g_tile_wide = wide;
   g_zc = zc;
 }
+
+void lpm_set_ku_kernel(int32_t lean) { g_lean = lean != 0; }
 
 void lpm_profile_ku(int32_t enable) {
   g_prof.on = enable != 0;

@@ -43,13 +43,19 @@ __host__ __device__ constexpr T nd2(int k, int a) {
 // DIRECTED: (..., N, NN) per directed slot, as the reference tensors Kn/Tv (torch_lpm3d.py:212-227).
 // PACKED:   (..., NN/2, P) forward-slot planes (slots 0, 2, 4, ...), valid for symmetric parameters
 //           (Kn[i,k] == Kn[q,k^1]): slot k odd at particle i reads forward plane (k-1)/2 at its neighbour q.
-//           Each plane holds G zero guard values, the N particles, and G more guards (P = N + 2G,
-//           G = nx ny + nx + 1 in 3D, nx + 1 in 2D), so p + (neighbour offset) is inside the plane for every
-//           particle p and slot, bond or not: the kernels read it without a bounds select.
+//           Each plane holds G zero guard values, the N particles, and at least G more guards: G is one z-plane +
+//           one row + one site (2D: one row + one site) rounded up to 64 values and P = N + 2G rounded up to 64,
+//           so p + (neighbour offset) is inside the plane for every particle p and slot, bond or not (the kernels
+//           read it without a bounds select), and each plane's first particle is 256-byte aligned in FP32.
 enum CoefMode : int { COEF_DIRECTED = 0, COEF_PACKED = 1 };
 
 __host__ __device__ inline int64_t packed_guard(int dim, int nx, int ny) {
-  return dim == 3 ? (int64_t)nx * ny + nx + 1 : (int64_t)nx + 1;
+  const int64_t g = dim == 3 ? (int64_t)nx * ny + nx + 1 : (int64_t)nx + 1;
+  return (g + 63) & ~(int64_t)63;
+}
+// plane stride P of the packed layout for N particles
+__host__ __device__ inline int64_t packed_stride(int dim, int nx, int ny, int64_t N) {
+  return (N + 2 * packed_guard(dim, nx, ny) + 63) & ~(int64_t)63;
 }
 
 // base: first particle of plane 0 (array + G) in PACKED mode; pstride: plane stride P

@@ -27,7 +27,7 @@ struct Lat<2> {
 
 struct Box {
   int nx, ny, nz;   // nz = 1 in 2D
-  int64_t ps;       // plane stride of packed coefficients (N + 2 guards); unused for directed ones
+  int64_t ps;       // plane stride of packed coefficients (packed_stride); unused for directed ones
   // optional bond words (one uint32 per particle and system, bit k = slot k is a bond; removed particles): null =
   // every slot that stays in the box is a bond
   const uint32_t* bonds = nullptr;
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256) void pack_coef_kernel(Box box, const CI* __res
   using L = Lat<DIM>;
   constexpr int NN = L::NN;
   const int64_t N = (int64_t)box.nx * box.ny * box.nz;
-  const int64_t G = (box.ps - N) / 2;
+  const int64_t G = packed_guard(DIM, box.nx, box.ny);
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // position in the padded plane
   if (e >= box.ps) return;
   const int b = blockIdx.y;

@@ -22,13 +22,17 @@ def _dims(dims):
 
 
 def packed_guard(dims):
-    """Zero guard values before and after each packed plane: one z-plane plus one row plus one site."""
+    """Zero guard values before each packed plane: one z-plane plus one row plus one site (2D: one row plus one
+    site), rounded up to 64 values (256-byte aligned planes in FP32)."""
     dims = tuple(int(d) for d in dims)
-    return dims[0] * dims[1] + dims[0] + 1 if len(dims) == 3 else dims[0] + 1
+    g = dims[0] * dims[1] + dims[0] + 1 if len(dims) == 3 else dims[0] + 1
+    return (g + 63) // 64 * 64
 
 
 def packed_len(dims):
-    return int(np.prod([int(d) for d in dims])) + 2 * packed_guard(dims)
+    """Plane stride P: the leading guard, the N particles and at least as many trailing guards, rounded up to 64."""
+    n = int(np.prod([int(d) for d in dims])) + 2 * packed_guard(dims)
+    return (n + 63) // 64 * 64
 
 
 def _bshape(bonds):

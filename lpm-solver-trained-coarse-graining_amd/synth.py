@@ -160,8 +160,16 @@ def clean_solid(solid, kmin=4, max_it=50):
 
 # ------------------------------------------------------------------------------------------------ coefficients
 def guard_len(dims):
-    """Zero guard values on both sides of a packed plane (one z-plane + one row + one site; 2D: one row + one)."""
-    return dims[0] * dims[1] + dims[0] + 1 if len(dims) == 3 else dims[0] + 1
+    """Leading zero guard values of a packed plane (one z-plane + one row + one site; 2D: one row + one), rounded
+    up to 64 values: the layout of lpm_ops.packed_guard / the C ABI."""
+    g = dims[0] * dims[1] + dims[0] + 1 if len(dims) == 3 else dims[0] + 1
+    return (g + 63) // 64 * 64
+
+
+def plane_len(dims):
+    """Plane stride of the packed layout (lpm_ops.packed_len)."""
+    n = int(np.prod(dims)) + 2 * guard_len(dims)
+    return (n + 63) // 64 * 64
 
 
 def packed_from_particles(dims, ppar, device, dtype):
@@ -173,7 +181,7 @@ def packed_from_particles(dims, ppar, device, dtype):
     fwd = SC_FWD if d == 3 else SQ_FWD
     N = int(np.prod(dims))
     G = guard_len(dims)
-    kn = torch.zeros((len(fwd), N + 2 * G), device=device, dtype=dtype)
+    kn = torch.zeros((len(fwd), plane_len(dims)), device=device, dtype=dtype)
     tv = torch.zeros_like(kn)
     cols = [pp[:, c].reshape(shape) for c in range(3)]
     for k, off in enumerate(fwd):

@@ -307,3 +307,46 @@ def test_apply_tilings_and_z_chunks(dev, wide, zc, dt):
             assert info.conv.all() and rel(x, c["xr"]) < 1e-8
     finally:
         L.lib().lpm_set_ku_tiling(0, 0)
+
+
+_LEAN_DIMS = [(64, 4, 1), (64, 1, 3), (128, 5, 7), (130, 9, 13), (64, 8, 20), (96, 6, 15)]
+_LEAN_REF = {}
+
+
+@pytest.mark.parametrize("dims", _LEAN_DIMS)
+@pytest.mark.parametrize("zc", [0, 7, 9])
+def test_lean_kernel_matches_oracle_and_general_kernel(dev, dims, zc):
+    """The lean FP32 packed K u (64-wide tiles, used when nx >= 48 without bond words): single planes, partial x and
+    y tiles, z chunks that end inside the unrolled group; plain K u against the oracle (FP32 bar) and against the
+    general kernel, and its masked-dot form inside the refined solver against the oracle's FP64 PCG solution."""
+    print("lean", dims, zc, flush=True)
+    lat, Kn, Tv, u = case(dims, seed=21, two_phase=True)
+    t64 = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    kp, tp, _ = ops.pack(dims, t64(Kn), t64(Tv), out_dtype=torch.float32)
+    u32 = t64(u).float()
+    ref = R.apply(lat, u, Kn, Tv)
+    try:
+        L.lib().lpm_set_ku_tiling(1, zc)
+        L.lib().lpm_set_ku_kernel(1)
+        lean = ops.apply(dims, 1.0, u32, kp, tp, layout=L.PACKED)
+        assert rel(lean, ref) < TOL[torch.float32]
+        L.lib().lpm_set_ku_kernel(0)
+        general = ops.apply(dims, 1.0, u32, kp, tp, layout=L.PACKED)
+        L.lib().lpm_set_ku_kernel(1)
+        assert rel(lean, general.double().cpu().numpy()) < 1e-6
+        if dims[2] >= 3 and dims[1] >= 3:
+            if dims not in _LEAN_REF:
+                fixed, up, _ = R.prescribed3("tb_tz", lat.x, lat.ijk, dims, 1)
+                b = -R.apply(lat, up * fixed[:, None], Kn, Tv) * (~fixed)[:, None]
+                xr, _, _ = R.pcg(lat, b, Kn, Tv, R.precond(lat, Kn, Tv, ~fixed), ~fixed, tol=1e-12, maxit=20000)
+                _LEAN_REF[dims] = (fixed, b, xr)
+            fixed, b, xr = _LEAN_REF[dims]
+            free = torch.as_tensor(~fixed, device=dev)
+            k64, tp64, _ = ops.pack(dims, t64(Kn), t64(Tv))
+            minv = ops.block_jacobi(dims, 1.0, k64, tp64, free=free, layout=L.PACKED, out_dtype=torch.float32)
+            x, info = ops.pcg_refined(dims, 1.0, t64(b), k64, tp64, kp, tp, minv32=minv, free=free, tol=1e-10,
+                                      maxit=20000, inner_tol=1e-4, layout64=L.PACKED, layout32=L.PACKED)
+            assert info.conv.all() and rel(x, xr) < 1e-8
+    finally:
+        L.lib().lpm_set_ku_kernel(0)
+        L.lib().lpm_set_ku_tiling(0, 0)

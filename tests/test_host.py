@@ -57,6 +57,23 @@ def test_integration_binding_matches_the_library_structs(monkeypatch):
     assert rc == L.LPM_E_BADARG
 
 
+@pytest.mark.parametrize("dims", [(256, 256, 256), (70, 11, 40), (7, 6, 5), (1, 1, 1), (48, 48, 48), (20, 14),
+                                  (128, 128), (3, 2)])
+def test_packed_layout_is_one_definition_and_aligned(dims):
+    """The packed plane stride of the C ABI, lpm_ops and synth agree; guard and stride are multiples of 64 values
+    (FP32 planes start 256-byte aligned) and the trailing guard is at least the leading one, which covers every
+    neighbour offset (one z-plane + one row + one site)."""
+    import _lpmlib as L
+    import synth
+    g = L.grid_of(dims, 1.0)
+    P = L.lib().lpm_packed_len(ctypes.byref(g))
+    G = ops.packed_guard(dims)
+    N = int(np.prod(dims))
+    reach = (dims[0] * dims[1] + dims[0] + 1) if len(dims) == 3 else dims[0] + 1
+    assert P == ops.packed_len(dims) == synth.plane_len(dims) and G == synth.guard_len(dims)
+    assert P % 64 == 0 and G % 64 == 0 and G >= reach and P - N - G >= G
+
+
 def test_abi_rejects_bad_arguments_without_a_gpu():
     import _lpmlib as L
     g = L.Grid(4, 2, 2, 2, 1.0)
