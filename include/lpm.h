@@ -154,6 +154,18 @@ int lpm_pcg_solve(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int3
                   const void* b, void* x, int32_t use_x0, const lpm_cg_opts* opts, void* work, int32_t* iters,
                   double* relres, int32_t* conv, void* stream);
 
+/* lpm_pcg_solve without a workspace argument and without any device allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a file-scope device array of lpm_noalloc_scratch_bytes() (64 MiB) that each GPU the
+ * library runs on holds until the process ends; a solve whose lpm_pcg_workspace exceeds it is refused with
+ * LPM_E_BADARG.  Every call on a GPU shares that GPU's array: calls that overlap on one GPU (other threads or
+ * streams) race.  opts->graph is ignored (no graph is captured) and no profiling events are created. */
+size_t lpm_noalloc_scratch_bytes(void);
+int lpm_pcg_solve_noalloc(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* kn,
+                          const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm, const void* minv,
+                          lpm_bcast bminv, const void* b, void* x, int32_t use_x0, const lpm_cg_opts* opts,
+                          int32_t* iters, double* relres, int32_t* conv, void* stream);
+
 /* Mixed precision: FP32 (P)CG inner solves inside an FP64 iterative-refinement loop; the true residual
  * r = mask * (b - K x) and the update x += d are FP64.  kn64/tv64 (layout64) define the FP64 operator, kn32/tv32
  * (layout32) the FP32 one; minv32 is the FP32 preconditioner.  relres is the true FP64 relative residual, iters the

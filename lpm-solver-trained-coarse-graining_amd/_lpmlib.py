@@ -53,6 +53,9 @@ SIGS = {
     "lpm_pcg_workspace": (ctypes.c_size_t, [GP, I32, I32]),
     "lpm_pcg_solve": (I32, [GP, I32, I32, I32, I32, P, P, Bcast, P, Bcast, P, Bcast, P, P, I32,
                             ctypes.POINTER(CgOpts), P, P, P, P, P]),
+    "lpm_noalloc_scratch_bytes": (ctypes.c_size_t, []),
+    "lpm_pcg_solve_noalloc": (I32, [GP, I32, I32, I32, I32, P, P, Bcast, P, Bcast, P, Bcast, P, P, I32,
+                                    ctypes.POINTER(CgOpts), P, P, P, P]),
     "lpm_serve_footprint": (ctypes.c_size_t, [I32, I32]),
     "lpm_serve_create": (I32, [I32, ctypes.c_size_t, I32, ctypes.POINTER(ctypes.c_void_p)]),
     "lpm_serve_destroy": (None, [P]),

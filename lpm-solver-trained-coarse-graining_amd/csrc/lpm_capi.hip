@@ -1,5 +1,6 @@
 // C ABI of liblpm.so (declarations and reference citations in include/lpm.h).
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cmath>
 #include <cstdio>
@@ -315,7 +316,7 @@ template <int DIM, typename T>
 int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, const void* tv, Bcast bc,
              const uint8_t* mask, Bcast bm, const T* minv, Bcast bminv, const T* b, T* x, int use_x0,
              const lpm_cg_opts* o, char* work, const double* tol_dev, int32_t* iters, double* relres, int32_t* conv,
-             hipStream_t st) {
+             hipStream_t st, bool no_alloc = false) {
   constexpr int D = DIM;
   const int nsys = nb * nl;
   const int64_t N = nsites(g);
@@ -368,7 +369,8 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
   bool capturing = false;
   auto body = [&](hipStream_t sq) -> int {
     KuCall c{p, bvec, kn, tv, bc, mask, bm, nullptr, bvec, ap, bvec, partial, ist + I_ACTIVE};
-    const bool prof = g_prof.on && !capturing;
+    // the allocation-free entry point creates no events (profiling) and no graph
+    const bool prof = g_prof.on && !capturing && !no_alloc;
     if (prof) {
       while (g_prof.ev.size() < g_prof.used + 2) {
         hipEvent_t e;
@@ -394,7 +396,7 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
   int rc = LPM_OK;
   while (done_it < o->maxit) {
     const int nrep = std::min(check, o->maxit - done_it);
-    if (o->graph && nrep == check) {
+    if (o->graph && !no_alloc && nrep == check) {
       if (!gexec) {
         // capture on a private stream (the caller's may be the null stream, which cannot capture); the graph is
         // then launched on the caller's stream
@@ -799,6 +801,77 @@ int lpm_pcg_solve(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int3
                                  relres, conv, st);
     default:
       return fail(LPM_E_BADARG, "lpm_pcg_solve: bad dtype %d", dtype);
+  }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------ no allocation
+// Workspace of lpm_pcg_solve_noalloc: a file-scope device array, one copy per GPU for the life of the process.  In
+// host code its name is only a stand-in; the per-device address comes from hipGetSymbolAddress and is the only
+// thing host code passes on.
+namespace {
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::atomic<void*> g_scratch_ptr[kMaxDevices];
+
+// address of the current device's scratch array (looked up once per device, with that device current)
+int scratch_of_current_device(void** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(LPM_E_UNSUPPORTED, "device %d: no scratch slot", dev);
+  void* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (!p) {
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(LPM_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev].store(p, std::memory_order_release);
+  }
+  *out = p;
+  return LPM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+size_t lpm_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+int lpm_pcg_solve_noalloc(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* kn,
+                          const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm, const void* minv,
+                          lpm_bcast bminv, const void* b, void* x, int32_t use_x0, const lpm_cg_opts* opts,
+                          int32_t* iters, double* relres, int32_t* conv, void* stream) {
+  if (int rc = check_solve_args(g, nb, nl, opts, minv)) return rc;
+  if (!kn || !tv || !b || !x) return fail(LPM_E_BADARG, "lpm_pcg_solve_noalloc: null pointer");
+  if (dtype != LPM_F32 && dtype != LPM_F64) return fail(LPM_E_BADARG, "lpm_pcg_solve_noalloc: bad dtype %d", dtype);
+  const size_t need = lpm_pcg_workspace(g, nb * nl, dtype);
+  if (need == 0 || need > kScratchBytes)
+    return fail(LPM_E_BADARG, "lpm_pcg_solve_noalloc: the solve needs %zu bytes of workspace, more than the %zu-byte "
+                "static scratch (use lpm_pcg_solve with a caller workspace)", need, kScratchBytes);
+  void* work = nullptr;
+  if (int rc = scratch_of_current_device(&work)) return rc;
+  // every workspace value the solver reads is written by it first (scalars and flags by the setup kernel,
+  // vectors by the init kernel, partial sums by the kernels that produce them): nothing to clear
+  lpm_cg_opts o = *opts;
+  o.graph = 0;
+  hipStream_t st = (hipStream_t)stream;
+  const int key = (g->dim == 3 ? 10 : 0) + dtype;
+  const Bcast bcc = bc_of(bc), bmm = bc_of(bm), bmi = bc_of(bminv);
+  switch (key) {
+    case 10:
+      return pcg_core<3, float>(g, nb, nl, layout, kn, tv, bcc, mask, bmm, (const float*)minv, bmi, (const float*)b,
+                                (float*)x, use_x0, &o, (char*)work, nullptr, iters, relres, conv, st, true);
+    case 11:
+      return pcg_core<3, double>(g, nb, nl, layout, kn, tv, bcc, mask, bmm, (const double*)minv, bmi,
+                                 (const double*)b, (double*)x, use_x0, &o, (char*)work, nullptr, iters, relres, conv,
+                                 st, true);
+    case 0:
+      return pcg_core<2, float>(g, nb, nl, layout, kn, tv, bcc, mask, bmm, (const float*)minv, bmi, (const float*)b,
+                                (float*)x, use_x0, &o, (char*)work, nullptr, iters, relres, conv, st, true);
+    default:
+      return pcg_core<2, double>(g, nb, nl, layout, kn, tv, bcc, mask, bmm, (const double*)minv, bmi,
+                                 (const double*)b, (double*)x, use_x0, &o, (char*)work, nullptr, iters, relres, conv,
+                                 st, true);
   }
 }
 

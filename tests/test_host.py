@@ -74,6 +74,22 @@ def test_packed_layout_is_one_definition_and_aligned(dims):
     assert P % 64 == 0 and G % 64 == 0 and G >= reach and P - N - G >= G
 
 
+def test_noalloc_solve_refuses_oversize_before_any_gpu_call():
+    """The allocation-free PCG entry point checks its workspace against the static scratch before touching the GPU."""
+    import _lpmlib as L
+    lib = L.lib()
+    cap = int(lib.lpm_noalloc_scratch_bytes())
+    assert cap == 64 << 20
+    g = L.grid_of((200, 200, 200), 1.0)
+    assert int(lib.lpm_pcg_workspace(ctypes.byref(g), 1, L.F32)) > cap
+    o = ops._opts(False, 1e-8, 10, 10, False)
+    z = L.Bcast(0, 0)
+    dummy = ctypes.c_void_p(16)
+    rc = lib.lpm_pcg_solve_noalloc(ctypes.byref(g), 1, 1, L.F32, L.PACKED, dummy, dummy, z, None, z, None, z, dummy,
+                                   dummy, 0, ctypes.byref(o), None, None, None, None)
+    assert rc == L.LPM_E_BADARG and b"static scratch" in lib.lpm_last_error()
+
+
 def test_abi_rejects_bad_arguments_without_a_gpu():
     import _lpmlib as L
     g = L.Grid(4, 2, 2, 2, 1.0)
