@@ -154,6 +154,18 @@ int lpm_pcg_solve(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int3
                   const void* b, void* x, int32_t use_x0, const lpm_cg_opts* opts, void* work, int32_t* iters,
                   double* relres, int32_t* conv, void* stream);
 
+/* Multi-tolerance (P)CG: the solve of lpm_pcg_solve, stopped per system once its recursive relative residual meets
+ * every tolerance of the descending list tols[0 .. ntol) (ntol <= 64; opts->tol is not used).  snaps (device,
+ * (nb*nl, ntol, N, dim), dtype) receives the iterate of the first iteration at which each level is met (iteration 0
+ * included) and snap_iters (host, nb*nl*ntol) that iteration; levels never met (maxit) get the final iterate and -1.
+ * Replaces pcg_mf / solve_mf with TOLS (timing/t2d/solver2d.py:27, 349-421, 577-613) and the snapshots of pcg_bj
+ * (:64-142).  Workspace: lpm_pcg_workspace. */
+int lpm_pcg_snapshots(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* kn,
+                      const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm, const void* minv,
+                      lpm_bcast bminv, const void* b, void* x, int32_t use_x0, const lpm_cg_opts* opts, int32_t ntol,
+                      const double* tols, void* snaps, int32_t* snap_iters, void* work, int32_t* iters,
+                      double* relres, int32_t* conv, void* stream);
+
 /* lpm_pcg_solve without a workspace argument and without any device allocation: no counterpart in the reference;
  * for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The workspace is a file-scope device array of lpm_noalloc_scratch_bytes() (64 MiB) that each GPU the

@@ -23,7 +23,29 @@ enum CgScal : int { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_BN, S_TOL, S_REL, 
 // I_NOTSPD: the system stopped because K (or M) is not positive definite on it: p.Ap <= 0 for a nonzero search
 // direction, or r.z < 0.  The reference's dense path raises FloatingPointError there (coarse2d/torch_lpm.py:20-25);
 // the host reads the flag at its convergence poll and returns LPM_E_NOT_SPD.
-enum CgInt : int { I_IT = 0, I_ACTIVE, I_MAXIT, I_CONV, I_XUPD, I_NOTSPD, NINT };
+// Snapshots (multi-tolerance solves, timing/t2d/solver2d.py:27, 349-421): I_SNAPK = next tolerance of the descending
+// list not yet met; [I_SNAP0, I_SNAP1) = levels first met in the current iteration, whose snapshot is the iterate
+// after this iteration's update.
+enum CgInt : int { I_IT = 0, I_ACTIVE, I_MAXIT, I_CONV, I_XUPD, I_NOTSPD, I_SNAPK, I_SNAP0, I_SNAP1, NINT };
+
+// Snapshot mode of a solve (nt = 0: off).  tols: descending tolerances (device); snap_it (systems, nt): iteration
+// at which each level was first met (-1 = never); snaps (systems, nt, N, D): the iterates.
+template <typename T>
+struct SnapDev {
+  int nt;
+  const double* tols;
+  int* snap_it;
+  T* snaps;
+};
+
+// levels k >= k0 met by relative residual rn / bn (bn = 0: all met), recorded with iteration `it`; returns the next
+// unmet level
+__device__ __forceinline__ int snap_levels(const double* tols, int nt, int k0, double rn, double bn, int* snap_it,
+                                           int it) {
+  int k = k0;
+  while (k < nt && (bn == 0.0 || rn <= tols[k] * bn)) snap_it[k++] = it;
+  return k;
+}
 
 template <typename T>
 struct CgVec {
@@ -106,9 +128,11 @@ __device__ __forceinline__ T sum_partials(const T* __restrict__ pp, int nblk, in
   return block_sum(acc, red);
 }
 
-// after cg_init: rz, rr, bn; a system with b = 0 is done at iteration 0 (coarse3d/lin3d.py:479-480)
+// after cg_init: rz, rr, bn; a system with b = 0 is done at iteration 0 (coarse3d/lin3d.py:479-480).  In snapshot
+// mode the levels the initial residual meets are recorded with iteration 0 and the system is done once every level
+// is met.
 template <typename T>
-__global__ __launch_bounds__(256) void cg_fin_init_kernel(T* partial, int nblk, T* scal, int* ist) {
+__global__ __launch_bounds__(256) void cg_fin_init_kernel(T* partial, int nblk, T* scal, int* ist, SnapDev<T> sd) {
   __shared__ T red[4];
   const int s = blockIdx.x;
   const T* pp = partial + (int64_t)s * nblk * 3;
@@ -131,9 +155,16 @@ __global__ __launch_bounds__(256) void cg_fin_init_kernel(T* partial, int nblk, 
     const bool done = finite && (!(bn > T(0)) || sqrt(rr) <= sc[S_TOL] * bn);
     sc[S_REL] = !finite ? nan_of<T>() : (bn > T(0) ? sqrt(rr) / bn : T(0));
     // r.M r < 0 for the first residual: an indefinite diagonal block of K
-    const bool indef = finite && !done && rz < T(0);
-    is[I_ACTIVE] = (done || indef || !finite) ? 0 : 1;
-    is[I_CONV] = done ? 1 : 0;
+    bool done_ = done;
+    is[I_SNAPK] = is[I_SNAP0] = is[I_SNAP1] = 0;
+    if (sd.nt > 0 && finite) {
+      const int k = snap_levels(sd.tols, sd.nt, 0, (double)sqrt(rr), (double)bn, sd.snap_it + (int64_t)s * sd.nt, 0);
+      is[I_SNAPK] = is[I_SNAP1] = k;
+      done_ = k == sd.nt;
+    }
+    const bool indef = finite && !done_ && rz < T(0);
+    is[I_ACTIVE] = (done_ || indef || !finite) ? 0 : 1;
+    is[I_CONV] = done_ ? 1 : 0;
     is[I_NOTSPD] = indef ? 1 : 0;
   }
 }
@@ -245,9 +276,10 @@ __global__ __launch_bounds__(256) void cg_update_kernel(CgVec<T> v) {
   }
 }
 
-// convergence test and beta
+// convergence test and beta (snapshot mode: convergence = every level met)
 template <typename T>
-__global__ __launch_bounds__(256) void cg_fin_beta_kernel(const T* partial, int nblk, T* scal, int* ist) {
+__global__ __launch_bounds__(256) void cg_fin_beta_kernel(const T* partial, int nblk, T* scal, int* ist,
+                                                          SnapDev<T> sd) {
   __shared__ T red[4];
   const int s = blockIdx.x;
   int* is = ist + (int64_t)s * NINT;
@@ -262,7 +294,17 @@ __global__ __launch_bounds__(256) void cg_fin_beta_kernel(const T* partial, int 
     sc[S_RR] = rr;
     const T rn = sqrt(rr);
     sc[S_REL] = rn / sc[S_BN];
-    if (rn <= sc[S_TOL] * sc[S_BN]) {
+    bool met = rn <= sc[S_TOL] * sc[S_BN];
+    if (sd.nt > 0) {
+      const int k0 = is[I_SNAPK];
+      const int k = is_finite(rn) ? snap_levels(sd.tols, sd.nt, k0, (double)rn, (double)sc[S_BN],
+                                                sd.snap_it + (int64_t)s * sd.nt, it)
+                                  : k0;
+      is[I_SNAP0] = k0;
+      is[I_SNAP1] = is[I_SNAPK] = k;
+      met = k == sd.nt;
+    }
+    if (met) {
       is[I_ACTIVE] = 0;
       is[I_CONV] = 1;
     } else if (!is_finite(rn) || !is_finite(rzn)) {
@@ -285,11 +327,13 @@ __global__ __launch_bounds__(256) void cg_fin_beta_kernel(const T* partial, int 
 // x += alpha p (the iteration's update, also in the iteration that converged), then p = z + beta p while the
 // system is still active
 template <int D, typename T>
-__global__ __launch_bounds__(256) void cg_pupdate_kernel(CgVec<T> v) {
+__global__ __launch_bounds__(256) void cg_pupdate_kernel(CgVec<T> v, SnapDev<T> sd) {
   const int l = blockIdx.y, b = blockIdx.z;
   const int64_t s = (int64_t)b * v.nl + l;
   if (!v.ist[s * NINT + I_XUPD]) return;
   const bool active = v.ist[s * NINT + I_ACTIVE] != 0;
+  // snapshot levels first met in this iteration take the updated iterate
+  const int k0 = sd.nt > 0 ? v.ist[s * NINT + I_SNAP0] : 0, k1 = sd.nt > 0 ? v.ist[s * NINT + I_SNAP1] : 0;
   const T alpha = v.scal[s * NSCAL + S_ALPHA];
   const T beta = v.scal[s * NSCAL + S_BETA];
   T* __restrict__ x_ = v.x + s * v.N * D;
@@ -313,6 +357,7 @@ __global__ __launch_bounds__(256) void cg_pupdate_kernel(CgVec<T> v) {
       if (q < v.N) {
         for (int a = 0; a < D; ++a) xv[u].c[a] += alpha * pv[u].c[a];
         st_site<D>(x_, q * D, xv[u]);
+        for (int k = k0; k < k1; ++k) st_site<D>(sd.snaps + (s * sd.nt + k) * v.N * D, q * D, xv[u]);
         if (active) {
           for (int a = 0; a < D; ++a) pv[u].c[a] = zv[u].c[a] + beta * pv[u].c[a];
           st_site<D>(p_, q * D, pv[u]);
@@ -320,6 +365,32 @@ __global__ __launch_bounds__(256) void cg_pupdate_kernel(CgVec<T> v) {
       }
     }
   }
+}
+
+// x into the snapshot levels [I_SNAP0, I_SNAP1) of each system (grid (blocks, nl, nb)): the levels met by the
+// initial residual, and at the end the levels never met
+template <int D, typename T>
+__global__ __launch_bounds__(256) void cg_snap_copy_kernel(CgVec<T> v, SnapDev<T> sd) {
+  const int l = blockIdx.y, b = blockIdx.z;
+  const int64_t s = (int64_t)b * v.nl + l;
+  const int k0 = v.ist[s * NINT + I_SNAP0], k1 = v.ist[s * NINT + I_SNAP1];
+  if (k1 <= k0) return;
+  const T* __restrict__ x_ = v.x + s * v.N * D;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.N * D; i += (int64_t)gridDim.x * blockDim.x) {
+    const T xi = x_[i];
+    for (int k = k0; k < k1; ++k) sd.snaps[(s * sd.nt + k) * v.N * D + i] = xi;
+  }
+}
+// levels never met: iteration -1, snapshot = the final iterate (timing/t2d/solver2d.py:417-420)
+template <typename T>
+__global__ void cg_snap_rest_kernel(int* ist, int nsys, SnapDev<T> sd) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  int* is = ist + (int64_t)s * NINT;
+  const int k0 = is[I_SNAPK];
+  for (int k = k0; k < sd.nt; ++k) sd.snap_it[(int64_t)s * sd.nt + k] = -1;
+  is[I_SNAP0] = k0;
+  is[I_SNAP1] = sd.nt;
 }
 
 // ------------------------------------------------------------------------------------------ refinement helpers

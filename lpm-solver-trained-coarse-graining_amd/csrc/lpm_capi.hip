@@ -280,9 +280,10 @@ __global__ void cg_settol_kernel(T* scal, const double* tol, int nsys) {
 }
 
 struct PcgLayout {
-  size_t r, z, p, ap, partial, scal, ist, total;
+  size_t r, z, p, ap, partial, scal, ist, tols, snapit, total;
   int maxblk;
 };
+constexpr int kMaxSnapLevels = 64;  // tolerances of one snapshot solve
 
 template <int DIM, typename T>
 PcgLayout pcg_layout(const lpm_grid* g, int nsys) {
@@ -307,6 +308,10 @@ PcgLayout pcg_layout(const lpm_grid* g, int nsys) {
   o += align256(sizeof(T) * (size_t)nsys * NSCAL);
   L.ist = o;
   o += align256(sizeof(int) * (size_t)nsys * NINT);
+  L.tols = o;
+  o += align256(sizeof(double) * kMaxSnapLevels);
+  L.snapit = o;
+  o += align256(sizeof(int) * (size_t)nsys * kMaxSnapLevels);
   L.total = o;
   return L;
 }
@@ -316,7 +321,7 @@ template <int DIM, typename T>
 int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, const void* tv, Bcast bc,
              const uint8_t* mask, Bcast bm, const T* minv, Bcast bminv, const T* b, T* x, int use_x0,
              const lpm_cg_opts* o, char* work, const double* tol_dev, int32_t* iters, double* relres, int32_t* conv,
-             hipStream_t st, bool no_alloc = false) {
+             hipStream_t st, bool no_alloc = false, SnapDev<T> sd = SnapDev<T>{0, nullptr, nullptr, nullptr}) {
   constexpr int D = DIM;
   const int nsys = nb * nl;
   const int64_t N = nsites(g);
@@ -359,7 +364,9 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
     if (rc) return rc;
   }
   hipLaunchKernelGGL((cg_init_kernel<D, T>), vgrid, dim3(256), 0, st, v, b, use_x0);
-  hipLaunchKernelGGL((cg_fin_init_kernel<T>), dim3(nsys), dim3(256), 0, st, partial, vblk, scal, ist);
+  hipLaunchKernelGGL((cg_fin_init_kernel<T>), dim3(nsys), dim3(256), 0, st, partial, vblk, scal, ist, sd);
+  // levels met by the initial residual take the initial iterate
+  if (sd.nt > 0) hipLaunchKernelGGL((cg_snap_copy_kernel<D, T>), vgrid, dim3(256), 0, st, v, sd);
   LAUNCHCHK();
 
   const int check = std::max(1, o->check);
@@ -387,8 +394,8 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
     }
     hipLaunchKernelGGL((cg_fin_alpha_kernel<T>), dim3(nsys), dim3(256), 0, sq, partial, (int)kg.grid.x, scal, ist);
     hipLaunchKernelGGL((cg_update_kernel<D, T>), vgrid, dim3(256), 0, sq, v);
-    hipLaunchKernelGGL((cg_fin_beta_kernel<T>), dim3(nsys), dim3(256), 0, sq, partial, vblk, scal, ist);
-    hipLaunchKernelGGL((cg_pupdate_kernel<D, T>), vgrid, dim3(256), 0, sq, v);
+    hipLaunchKernelGGL((cg_fin_beta_kernel<T>), dim3(nsys), dim3(256), 0, sq, partial, vblk, scal, ist, sd);
+    hipLaunchKernelGGL((cg_pupdate_kernel<D, T>), vgrid, dim3(256), 0, sq, v, sd);
     LAUNCHCHK();
     return LPM_OK;
   };
@@ -439,6 +446,12 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
   if (gexec) {
     (void)hipGraphExecDestroy(gexec);
     (void)hipGraphDestroy(graph);
+  }
+  if (sd.nt > 0) {
+    // levels never met: the final iterate, iteration -1
+    hipLaunchKernelGGL((cg_snap_rest_kernel<T>), dim3((nsys + 255) / 256), dim3(256), 0, st, ist, nsys, sd);
+    hipLaunchKernelGGL((cg_snap_copy_kernel<D, T>), vgrid, dim3(256), 0, st, v, sd);
+    LAUNCHCHK();
   }
   std::vector<T> host_scal((size_t)nsys * NSCAL);
   HIPCHK(hipMemcpyAsync(host_scal.data(), scal, sizeof(T) * nsys * NSCAL, hipMemcpyDeviceToHost, st));
@@ -802,6 +815,43 @@ int lpm_pcg_solve(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int3
     default:
       return fail(LPM_E_BADARG, "lpm_pcg_solve: bad dtype %d", dtype);
   }
+}
+
+int lpm_pcg_snapshots(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, int32_t layout, const void* kn,
+                      const void* tv, lpm_bcast bc, const uint8_t* mask, lpm_bcast bm, const void* minv,
+                      lpm_bcast bminv, const void* b, void* x, int32_t use_x0, const lpm_cg_opts* opts, int32_t ntol,
+                      const double* tols, void* snaps, int32_t* snap_iters, void* work, int32_t* iters,
+                      double* relres, int32_t* conv, void* stream) {
+  if (int rc = check_solve_args(g, nb, nl, opts, minv)) return rc;
+  if (!kn || !tv || !b || !x || !work || !tols || !snaps || !snap_iters)
+    return fail(LPM_E_BADARG, "lpm_pcg_snapshots: null pointer");
+  if (ntol < 1 || ntol > kMaxSnapLevels) return fail(LPM_E_BADARG, "lpm_pcg_snapshots: 1 .. %d tolerances", kMaxSnapLevels);
+  for (int k = 0; k < ntol; ++k)
+    if (!(tols[k] >= 0.0) || (k > 0 && !(tols[k] <= tols[k - 1])))
+      return fail(LPM_E_BADARG, "lpm_pcg_snapshots: tolerances must be non-negative and descending");
+  hipStream_t st = (hipStream_t)stream;
+  const int nsys = nb * nl;
+  int rc = LPM_OK;
+  auto run = [&](auto dim_tag, auto t_tag) -> int {
+    constexpr int DIM = decltype(dim_tag)::value;
+    using T = decltype(t_tag);
+    const PcgLayout Lw = pcg_layout<DIM, T>(g, nsys);
+    char* w = (char*)work;
+    HIPCHK(hipMemcpyAsync(w + Lw.tols, tols, sizeof(double) * ntol, hipMemcpyHostToDevice, st));
+    SnapDev<T> sd{ntol, (const double*)(w + Lw.tols), (int*)(w + Lw.snapit), (T*)snaps};
+    if (int r = pcg_core<DIM, T>(g, nb, nl, layout, kn, tv, bc_of(bc), mask, bc_of(bm), (const T*)minv, bc_of(bminv),
+                                 (const T*)b, (T*)x, use_x0, opts, w, nullptr, iters, relres, conv, st, false, sd))
+      return r;
+    HIPCHK(hipMemcpyAsync(snap_iters, w + Lw.snapit, sizeof(int32_t) * nsys * ntol, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return LPM_OK;
+  };
+  if (dtype != LPM_F32 && dtype != LPM_F64) return fail(LPM_E_BADARG, "lpm_pcg_snapshots: bad dtype %d", dtype);
+  if (g->dim == 3)
+    rc = dtype == LPM_F32 ? run(std::integral_constant<int, 3>{}, float{}) : run(std::integral_constant<int, 3>{}, double{});
+  else
+    rc = dtype == LPM_F32 ? run(std::integral_constant<int, 2>{}, float{}) : run(std::integral_constant<int, 2>{}, double{});
+  return rc;
 }
 
 }  // extern "C"

@@ -18,6 +18,10 @@ from geom2d import (SQ_OFFS, SQ_SHELL, SquareGeom, iso_C, square_params_from_C, 
                     affine_disp)
 
 
+# descending tolerance list of the multi-tolerance solves (timing/t2d/solver2d.py:27)
+TOLS = np.array(sorted({m * 10.0 ** k for k in range(-6, 0) for m in (1, 1.5, 2, 3, 4, 5, 7)}, reverse=True))
+
+
 class Lattice2D(SquareGeom):
     """Square lattice of nx x ny sites with spacing h (coarse2d/lin2d.py:29-74)."""
 
@@ -97,6 +101,40 @@ class Lattice2D(SquareGeom):
         """K u as a flat (2N,) vector."""
         out = ops.apply(self.dims, self.h, self._t(np.reshape(u, (self.N, 2))), self._t(Kn), self._t(Tv))
         return out.cpu().numpy().reshape(-1)
+
+    def solve_mf(self, Kn, Tv, bcs, starts=None, tols=TOLS, maxit=20000, fp32=False, timing=None):
+        """Matrix-free 2x2 block-Jacobi PCG with snapshots at every tolerance of `tols` (timing/t2d/solver2d.py:
+        577-613).  bcs {load: (fixed (N,) bool, ub (2N,) prescribed displacement)}; starts {load: u0 (2N,)}
+        optional initial guesses.  All loads run as one batched launch, each with its own mask and preconditioner.
+        fp32=True stores Kn, Tv rounded to FP32 as the reference's mf_arrays(fp32=True) does; the vector arithmetic
+        stays FP64, as in the reference.  Returns {load: (snaps (ntol, 2N) with ub added, iters (ntol,))}."""
+        import time
+        t0 = time.perf_counter()
+        loads = list(bcs)
+        N = self.N
+        kn, tv = np.asarray(Kn, np.float64), np.asarray(Tv, np.float64)
+        if fp32:
+            kn, tv = kn.astype(np.float32).astype(np.float64), tv.astype(np.float32).astype(np.float64)
+        K, T = self._t(kn), self._t(tv)
+        free = np.stack([~np.asarray(bcs[ld][0], bool) for ld in loads])                   # (L, N)
+        ub = np.stack([np.reshape(bcs[ld][1], (N, 2)) for ld in loads])                   # (L, N, 2)
+        fr = torch.as_tensor(free, device=self.device)
+        ubt = self._t(ub)
+        b = -ops.apply(self.dims, self.h, ubt, K, T) * fr[..., None]
+        x0 = None
+        if starts:
+            x0 = torch.zeros_like(b)
+            for l, ld in enumerate(loads):
+                if ld in starts:
+                    x0[l] = (self._t(np.reshape(starts[ld], (N, 2))) - ubt[l]) * fr[l][..., None]
+        minv = ops.block_jacobi(self.dims, self.h, K, T, free=fr)
+        t1 = time.perf_counter()
+        x, info = ops.pcg(self.dims, self.h, b, K, T, minv=minv, free=fr, maxit=maxit, x0=x0, tols=tols)
+        snaps = (info.snaps + ubt[:, None]).cpu().numpy()
+        if timing is not None:
+            timing.update(setup=t1 - t0, iter=time.perf_counter() - t1)
+        return {ld: (snaps[l].reshape(len(tols), 2 * N), info.snap_iters[0, l].astype(np.int64))
+                for l, ld in enumerate(loads)}
 
 
 def solve_disp(K, fixed_dofs, u_fixed, f_ext=None, lu_cache=None):

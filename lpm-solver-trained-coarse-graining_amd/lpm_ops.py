@@ -278,8 +278,9 @@ def post_vjp(dims, h, u, Kn, Tv, g_ku=None, g_stress=None, g_energy=None, layout
 
 
 class SolveInfo:
-    def __init__(self, iters, relres, conv, outer=None):
+    def __init__(self, iters, relres, conv, outer=None, snaps=None, snap_iters=None):
         self.iters, self.relres, self.conv, self.outer = iters, relres, conv, outer
+        self.snaps, self.snap_iters = snaps, snap_iters
 
     @property
     def it(self):
@@ -303,9 +304,12 @@ def _opts(precond, tol, maxit, check, graph, inner_tol=1e-5, max_outer=20):
 
 
 def pcg(dims, h, b, Kn, Tv, minv=None, free=None, tol=1e-10, maxit=20000, x0=None, check=10, graph=False,
-        layout=L.DIRECTED, bonds=None):
+        layout=L.DIRECTED, bonds=None, tols=None):
     """Batched (P)CG on K_ff x = b_f.  b (B, L, N, d) or broadcastable; minv symmetric blocks (…, N, 6|3) or None
-    (plain CG).  Returns x (batch, N, d) and SolveInfo (per-system iterations, relative residual, convergence)."""
+    (plain CG).  Returns x (batch, N, d) and SolveInfo (per-system iterations, relative residual, convergence).
+    tols: a descending tolerance list (multi-tolerance solve, timing/t2d/solver2d.py:349-421): the solve runs until
+    every level is met (tol is not used) and info.snaps (batch, ntol, N, d) / info.snap_iters (nb, nl, ntol) hold the
+    iterate of the first iteration meeting each level (-1: never met, final iterate)."""
     dims, N, d, NN = _dims(dims)
     L.require_gpu(b, Kn, Tv)
     dt = b.dtype
@@ -340,12 +344,21 @@ def pcg(dims, h, b, Kn, Tv, minv=None, free=None, tol=1e-10, maxit=20000, x0=Non
     rel = np.zeros(nsys, np.float64)
     conv = np.zeros(nsys, np.int32)
     o = _opts(minv is not None, tol, maxit, check, graph)
-    L.check(L.lib().lpm_pcg_solve(ctypes.byref(g), nb, nl, L.dtype_code(dt), layout, _ptr(ke), _ptr(te), bc, _ptr(me),
-                                  bm, _ptr(mi), bmi, _ptr(be), _ptr(x), 1 if x0 is not None else 0, ctypes.byref(o),
-                                  _ptr(work), iters.ctypes.data_as(ctypes.c_void_p),
-                                  rel.ctypes.data_as(ctypes.c_void_p), conv.ctypes.data_as(ctypes.c_void_p),
-                                  L.stream_of(b)))
-    return x, SolveInfo(iters.reshape(nb, nl), rel.reshape(nb, nl), conv.reshape(nb, nl))
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    if tols is None:
+        L.check(L.lib().lpm_pcg_solve(ctypes.byref(g), nb, nl, L.dtype_code(dt), layout, _ptr(ke), _ptr(te), bc,
+                                      _ptr(me), bm, _ptr(mi), bmi, _ptr(be), _ptr(x), 1 if x0 is not None else 0,
+                                      ctypes.byref(o), _ptr(work), vp(iters), vp(rel), vp(conv), L.stream_of(b)))
+        return x, SolveInfo(iters.reshape(nb, nl), rel.reshape(nb, nl), conv.reshape(nb, nl))
+    tl = np.ascontiguousarray(np.asarray(tols, np.float64).ravel())
+    snaps = torch.empty(tuple(batch) + (len(tl), N, d), device=b.device, dtype=dt)
+    sit = np.zeros(nsys * len(tl), np.int32)
+    L.check(L.lib().lpm_pcg_snapshots(ctypes.byref(g), nb, nl, L.dtype_code(dt), layout, _ptr(ke), _ptr(te), bc,
+                                      _ptr(me), bm, _ptr(mi), bmi, _ptr(be), _ptr(x), 1 if x0 is not None else 0,
+                                      ctypes.byref(o), len(tl), vp(tl), _ptr(snaps), vp(sit), _ptr(work), vp(iters),
+                                      vp(rel), vp(conv), L.stream_of(b)))
+    return x, SolveInfo(iters.reshape(nb, nl), rel.reshape(nb, nl), conv.reshape(nb, nl), snaps=snaps,
+                        snap_iters=sit.reshape(nb, nl, len(tl)))
 
 
 def pcg_refined(dims, h, b, Kn64, Tv64, Kn32, Tv32, minv32=None, free=None, tol=1e-8, maxit=20000, inner_tol=1e-5,

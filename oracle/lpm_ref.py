@@ -18,7 +18,8 @@ Reference conventions restated here (citations relative to the reference tree):
   * computeStress with the opp-flag surface rule (coarse3d/lin3d.py:170-210, coarse2d/lin2d.py:143-154), particle
     energy (coarse3d/lin3d.py:213-236), block diagonal (coarse3d/lin3d.py:239-269), inverse by cofactors
     (coarse3d/lin3d.py:272-291), K = B^T H B (coarse2d/lin2d.py:77-118, coarse3d/lin3d.py:62-92).
-  * block-Jacobi PCG from x0 = 0 with ||r|| <= tol ||b|| tested every iteration (coarse3d/lin3d.py:464-532),
+  * block-Jacobi PCG from x0 = 0 with ||r|| <= tol ||b|| tested every iteration (coarse3d/lin3d.py:464-532), and
+    its multi-tolerance form with snapshots (timing/t2d/solver2d.py:349-421),
     plain CG (timing/t3d/cgplain.py:11-55, timing/t2d/cg2plain.py:11-49), SuperLU direct solve
     (coarse3d/lin3d.py:535-546, coarse2d/lin2d.py:166-180).
   * boundary conditions of the load cases (coarse3d/geom3d.py:71-106, coarse2d/torch_lpm.py:92-111).
@@ -434,6 +435,56 @@ def pcg(lat, b, Kn, Tv, M, free, tol=1e-10, maxit=20000):
         p = z + (rz2 / rz) * p
         rz = rz2
     return x, it, rn / bn
+
+
+def pcg_snapshots(lat, b, Kn, Tv, M, mask, tols, maxit=20000, x0=None):
+    """Multi-tolerance (P)CG of timing/t2d/solver2d.py:349-421 (pcg_mf): x0 (zero on fixed DOFs) or 0, per-DOF mask
+    (N, dim) float, block-Jacobi M (N, dim, dim) or None (plain CG); tols descending.  The snapshot of level k is the
+    iterate of the first iteration whose recursive residual meets ||r|| <= tols[k] ||b|| (iteration 0 included);
+    the loop stops once every level is met; levels never met get the final iterate and -1.  Returns (snaps
+    (ntol, N, dim), iters (ntol,), iterations run)."""
+    d = lat.dim
+    m = np.asarray(mask, float).reshape(lat.N, d)
+    b = np.asarray(b, float).reshape(lat.N, d)
+    x = np.zeros_like(b) if x0 is None else np.asarray(x0, float).reshape(lat.N, d).copy()
+    prec = (lambda r: np.einsum("nij,nj->ni", M, r)) if M is not None else (lambda r: r.copy())
+    r = (b - apply(lat, x, Kn, Tv)) * m
+    bn = math.sqrt(float((b * b).sum()))
+    tols = np.asarray(tols, float)
+    nt = len(tols)
+    snaps = np.empty((nt, lat.N, d))
+    its = np.zeros(nt, np.int64)
+    k = 0
+    rn = math.sqrt(float((r * r).sum()))
+    while k < nt and rn <= tols[k] * bn:
+        snaps[k], its[k] = x, 0
+        k += 1
+    if k == nt:
+        return snaps, its, 0
+    z = prec(r) * m
+    p = z.copy()
+    rz = float((r * z).sum())
+    it = 0
+    while it < maxit:
+        Ap = apply(lat, p, Kn, Tv) * m
+        alpha = rz / float((p * Ap).sum())
+        x = x + alpha * p
+        r = r - alpha * Ap
+        it += 1
+        rn = math.sqrt(float((r * r).sum()))
+        while k < nt and rn <= tols[k] * bn:
+            snaps[k], its[k] = x, it
+            k += 1
+        if k == nt:
+            break
+        z = prec(r) * m
+        rzn = float((r * z).sum())
+        p = z + (rzn / rz) * p
+        rz = rzn
+    while k < nt:
+        snaps[k], its[k] = x, -1
+        k += 1
+    return snaps, its, it
 
 
 def cg_plain(lat, b, Kn, Tv, mask, tol=1e-8, maxit=200000):

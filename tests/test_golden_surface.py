@@ -298,7 +298,7 @@ def test_gpu_coarse3d_run_and_gradient_match_reference_autograd(dev):
         assert rel(o[k][0], g[k]) < 1e-9, k
     loss = ((o["sig"] * t(g["Wsig"])).sum() + 1e3 * (o["u"] * t(g["Wu"])).sum() + (o["w"] * t(g["Ww"])).sum()
             + (o["F"] * t(g["WF"])).sum())
-    assert abs(float(loss) - float(g["loss"])) <= 1e-9 * abs(float(g["loss"]))
+    assert abs(float(loss.detach()) - float(g["loss"])) <= 1e-9 * abs(float(g["loss"]))
     loss.backward()
     gmax = max(np.abs(g["gKn"]).max(), np.abs(g["gTv"]).max())
     valid = R.Box((nc,) * 3).valid

@@ -221,3 +221,61 @@ def test_stability_penalty(dev):
     # the penalty is the sum of squared shortfalls of lambda_min / mean|Kn| below frac
     frac = 0.05
     assert abs(float(pen[0]) - (np.maximum(frac - lmin[0] / sc, 0.0) ** 2).sum()) <= 1e-12 * float(pen[0])
+
+
+@pytest.mark.parametrize("dims,precond", [((24, 20), True), ((24, 20), False), ((9, 8, 7), True)])
+def test_pcg_snapshots_match_oracle(dev, dims, precond):
+    """Multi-tolerance solve (timing/t2d/solver2d.py:349-421): three systems with different masks in one launch;
+    per level the first iteration meeting it and that iterate equal the oracle's restatement; a small maxit leaves
+    the last levels at -1 with the final iterate."""
+    print("snapshots", dims, precond, flush=True)
+    lat = R.Box(dims)
+    d = lat.dim
+    Kn, Tv = R.random_symmetric_params(lat, np.random.default_rng(6))
+    rng = np.random.default_rng(7)
+    fixeds = [lat.ijk[:, 0] == 0, lat.ijk[:, 1] == 0, (lat.ijk[:, 0] == 0) | (lat.ijk[:, 1] == lat.dims[1] - 1)]
+    bs = [rng.standard_normal((lat.N, d)) * (~f)[:, None] for f in fixeds]
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    free = torch.as_tensor(np.stack([~f for f in fixeds]), device=dev)
+    minv = ops.block_jacobi(dims, 1.0, t(Kn), t(Tv), free=free) if precond else None
+    tols = lin2d.TOLS
+    for maxit in (20000, 25):
+        x, info = ops.pcg(dims, 1.0, t(np.stack(bs)), t(Kn), t(Tv), minv=minv, free=free, maxit=maxit, tols=tols)
+        for s, (f, b) in enumerate(zip(fixeds, bs)):
+            M = R.precond(lat, Kn, Tv, ~f) if precond else None
+            snaps, its, _ = R.pcg_snapshots(lat, b, Kn, Tv, M, np.repeat((~f)[:, None], d, 1), tols, maxit=maxit)
+            assert (info.snap_iters[0, s] == its).all(), (s, info.snap_iters[0, s], its)
+            assert rel(info.snaps[s], snaps) < 1e-9
+            if maxit == 25:
+                assert (its == -1).any() and not info.conv[0, s]
+            else:
+                assert info.conv[0, s] and info.iters[0, s] == its[-1]
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_lin2d_solve_mf_matches_oracle(dev, fp32):
+    """Lattice2D.solve_mf keeps solve_mf's shape (snapshots with ub added, per-level iterations per load);
+    fp32=True rounds the stored coefficients to FP32 (the reference's mf_arrays(fp32=True)), arithmetic FP64."""
+    nx, ny = 16, 12
+    lat = lin2d.Lattice2D("square", nx, ny, 1.0, device=dev)
+    ref = R.Box((nx, ny))
+    Kn, Tv = params2(ref, 8)
+    bcs = {}
+    for ld in ("tb_ty", "ring_xy"):
+        fixed, up, _ = R.prescribed2(ld, ref.x, ref.ijk, nx, ny, 1, 1.0)
+        bcs[ld] = (fixed, up.ravel())
+    out = lat.solve_mf(Kn, Tv, bcs, tols=lin2d.TOLS[::4], fp32=fp32)
+    kn, tv = (Kn.astype(np.float32).astype(np.float64), Tv.astype(np.float32).astype(np.float64)) if fp32 else (Kn, Tv)
+    for ld, (fixed, ub) in bcs.items():
+        ubm = ub.reshape(-1, 2)
+        b = -R.apply(ref, ubm, kn, tv) * (~fixed)[:, None]
+        snaps, its, _ = R.pcg_snapshots(ref, b, kn, tv, R.precond(ref, kn, tv, ~fixed),
+                                        np.repeat((~fixed)[:, None], 2, 1), lin2d.TOLS[::4])
+        S, I = out[ld]
+        assert S.shape == (len(its), 2 * lat.N) and (I == its).all()
+        assert rel(S, (snaps + ubm[None]).reshape(len(its), -1)) < 1e-9
+    # the FP32-rounded set is a different operator: its final snapshot differs from the FP64 one by ~1e-7
+    if fp32:
+        o64 = lat.solve_mf(Kn, Tv, bcs, tols=lin2d.TOLS[::4])
+        d = rel(out["tb_ty"][0][-1], o64["tb_ty"][0][-1])
+        assert 1e-10 < d < 1e-5

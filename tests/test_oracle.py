@@ -270,3 +270,31 @@ def test_c_oracle_solvers_match_numpy():
     xr, itr, rr = R.cg_plain(lat, b.reshape(-1, 2), Kn, Tv, mask.reshape(-1, 2), tol=1e-10)
     assert abs(it - itr) <= 1 and np.abs(x.reshape(-1, 2) - xr).max() < 1e-8 * np.abs(xr).max()
     assert math.isfinite(rel[0])
+
+
+def test_pcg_snapshots_are_the_first_iterates_meeting_each_tolerance():
+    """The multi-tolerance restatement (timing/t2d/solver2d.py:349-421): snapshot k is the plain PCG iterate after
+    its[k] iterations, the recursive residual meets tols[k] there and not one iteration earlier; with a small maxit
+    the levels never met hold the final iterate and -1."""
+    lat = R.Box((11, 9))
+    Kn, Tv = R.random_symmetric_params(lat, np.random.default_rng(3))
+    fixed = lat.ijk[:, 1] == 0
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal((lat.N, 2)) * (~fixed)[:, None]
+    M = R.precond(lat, Kn, Tv, ~fixed)
+    mask = np.repeat((~fixed)[:, None], 2, 1).astype(float)
+    tols = np.array([0.5, 0.3, 0.1, 1e-2, 5e-3, 1e-4, 1e-7, 1e-9])
+    snaps, its, n = R.pcg_snapshots(lat, b, Kn, Tv, M, mask, tols, maxit=5000)
+    assert (its >= 0).all() and np.all(np.diff(its) >= 0) and n == its[-1]
+    for k, t in enumerate(tols):
+        xk, itk, rk = R.pcg(lat, b, Kn, Tv, M, ~fixed, tol=0.0, maxit=int(its[k])) if its[k] > 0 else (
+            np.zeros_like(b), 0, 1.0)
+        assert np.abs(snaps[k] - xk).max() <= 1e-12 * max(np.abs(xk).max(), 1e-300)
+        assert its[k] == 0 or rk <= t
+        if its[k] > 1:
+            _, _, rprev = R.pcg(lat, b, Kn, Tv, M, ~fixed, tol=0.0, maxit=int(its[k]) - 1)
+            assert rprev > t
+    s2, i2, n2 = R.pcg_snapshots(lat, b, Kn, Tv, M, mask, tols, maxit=int(its[3]) + 1)
+    assert (i2[:4] == its[:4]).all() and (i2[5:] == -1).all() and n2 == its[3] + 1
+    xf, _, _ = R.pcg(lat, b, Kn, Tv, M, ~fixed, tol=0.0, maxit=n2)
+    assert np.abs(s2[-1] - xf).max() <= 1e-12 * np.abs(xf).max()
