@@ -153,9 +153,43 @@ def amg_cpu_baseline(P, gpu_iters, sample_its=3):
             "setup_s": t_setup, "sec_per_iter": t_it}
 
 
-def cpu_baseline(P, gpu_iters, budget_s=20.0):
-    """The C oracle's block-Jacobi PCG (FP64, OpenMP) on the same problem for a bounded number of iterations; the
-    solve time is the measured time per iteration times the GPU solve's iteration count."""
+def host_cpu():
+    """(model name, logical CPUs this process may use, threads for 'all cores': the affinity set, capped by
+    OMP_NUM_THREADS when the host sets it)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return model, avail, min(avail, cap) if cap > 0 else avail
+
+
+def fp64_pcg_iterations(P, tol):
+    """Iterations of the FP64 block-Jacobi PCG (the reference's algorithm and arithmetic, coarse3d/lin3d.py:464-518)
+    on the same problem, run once on the GPU outside the timed region: the count the CPU baseline is scaled by."""
+    import torch
+    ops, L = P["ops"], P["L"]
+    dims = P["dims"]
+    b = -ops.apply(dims, 1.0, P["up"], P["kp64"], P["tp64"], layout=L.PACKED) * P["free"][:, None]
+    M = ops.block_jacobi(dims, 1.0, P["kp64"], P["tp64"], free=P["free"], layout=L.PACKED)
+    _, info = ops.pcg(dims, 1.0, b, P["kp64"], P["tp64"], minv=M, free=P["free"], tol=tol, maxit=100000,
+                      layout=L.PACKED)
+    torch.cuda.synchronize()
+    return info.it, info.relmax
+
+
+def cpu_baseline(P, gpu_iters, budget_s=20.0, tol=1e-8, rounds=3):
+    """The C oracle's block-Jacobi PCG (FP64, OpenMP: the reference's CPU solver, coarse3d/lin3d.py:464-518) on the
+    same problem: time per iteration as the median of `rounds` rounds of a bounded number of iterations (warm-up
+    excluded, timing/summ_timing.py:19-46), at 6 threads (the reference's LPMCG_THREADS, timing/t2d/a2common.py:10)
+    and at every core this process may use; the solve time is that times the iteration count of an FP64 PCG solve
+    of the same problem (run on the GPU; the CPU solver takes the same iterations in the same arithmetic)."""
     import torch
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import build_oracle
@@ -167,9 +201,11 @@ def cpu_baseline(P, gpu_iters, budget_s=20.0):
     lib.orc_pcg3.argtypes = [ctypes.c_int64, Pp, Pp, Pp, Pp, Pp, Pp, Pp, ctypes.c_double, ctypes.c_int, Pp]
     lib.orc_pcg3.restype = ctypes.c_int
     lib.orc_threads.restype = ctypes.c_int
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
     dims = P["dims"]
     n = dims[0]
     N = n ** 3
+    it64, rel64 = fp64_pcg_iterations(P, tol)
     Kn, Tv = synth.directed_from_packed(dims, P["kp64"], P["tp64"])
     Kn = np.ascontiguousarray(Kn.cpu().numpy())
     Tv = np.ascontiguousarray(Tv.cpu().numpy())
@@ -195,20 +231,34 @@ def cpu_baseline(P, gpu_iters, budget_s=20.0):
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     rel = np.zeros(1)
     x = np.zeros_like(b)
-    t0 = time.perf_counter()
-    lib.orc_pcg3(N, ptr(nbr), ptr(Kn), ptr(Tv), ptr(M), ptr(fr), ptr(b), ptr(x), 0.0, 2, ptr(rel))
-    t_probe = (time.perf_counter() - t0) / 2
-    its = int(max(3, min(200, budget_s / max(t_probe, 1e-6))))
-    x[:] = 0
-    t0 = time.perf_counter()
-    done = lib.orc_pcg3(N, ptr(nbr), ptr(Kn), ptr(Tv), ptr(M), ptr(fr), ptr(b), ptr(x), 0.0, its, ptr(rel))
-    t_it = (time.perf_counter() - t0) / max(done, 1)
-    t_solve = t_it * gpu_iters
-    out = {"value": 1.0 / t_solve, "unit": "solves/s", "cores": int(lib.orc_threads()), "kind": "port",
-            "sample": "%d block-Jacobi PCG iterations (FP64, C/OpenMP restatement of coarse3d/lin3d.py:464-518) on "
-                      "the same 256^3 problem: %.3f s per iteration, scaled by the GPU solve's %d PCG iterations"
-                      % (done, t_it, gpu_iters),
-            "sec_per_iter": t_it}
+    model, avail, all_threads = host_cpu()
+    per = {}
+    for nthr in sorted({min(6, all_threads), all_threads}):
+        lib.orc_set_threads(nthr)
+        x[:] = 0
+        t0 = time.perf_counter()
+        lib.orc_pcg3(N, ptr(nbr), ptr(Kn), ptr(Tv), ptr(M), ptr(fr), ptr(b), ptr(x), 0.0, 2, ptr(rel))   # warm-up
+        t_probe = (time.perf_counter() - t0) / 2
+        # about budget_s / 2 of CPU work per thread count, split over the rounds
+        its = int(max(3, min(200, budget_s / 2 / rounds / max(t_probe, 1e-6))))
+        times = []
+        for _ in range(rounds):
+            x[:] = 0
+            t0 = time.perf_counter()
+            done = lib.orc_pcg3(N, ptr(nbr), ptr(Kn), ptr(Tv), ptr(M), ptr(fr), ptr(b), ptr(x), 0.0, its, ptr(rel))
+            times.append((time.perf_counter() - t0) / max(done, 1))
+        per[nthr] = dict(sec_per_iter=float(np.median(times)), rounds=[float(v) for v in times], iters_per_round=its,
+                         solves_per_s=1.0 / (float(np.median(times)) * it64))
+    t_it = per[all_threads]["sec_per_iter"]
+    out = {"value": 1.0 / (t_it * it64), "unit": "solves/s", "cores": all_threads, "kind": "port",
+           "sample": "block-Jacobi PCG (FP64, C/OpenMP restatement of coarse3d/lin3d.py:464-518) on the same %d^3 "
+                     "problem: %d iterations per round, median of %d rounds, %.3f s per iteration at %d threads "
+                     "(%s); solve time = that x %d iterations of an FP64 block-Jacobi PCG solve of the problem to "
+                     "rel-res %g (the GPU's mixed-precision solve ran %d inner iterations)"
+                     % (n, per[all_threads]["iters_per_round"], rounds, t_it, all_threads, model, it64, tol,
+                        gpu_iters),
+           "cpu_model": model, "threads_available": avail, "fp64_pcg_iters": it64, "fp64_pcg_relres": rel64,
+           "sec_per_iter": t_it, "per_threads": {str(k): v for k, v in per.items()}}
     out["fp64_oracle_64cube"] = oracle_subvolume_check(lib, P["lab"], n)
     return out
 
@@ -1064,7 +1114,7 @@ def main():
     }
     if not a.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(P, int(np.median(its)), a.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(P, int(np.median(its)), a.cpu_budget, a.tol)
     print(json.dumps(out), flush=True)
 
 

@@ -39,6 +39,17 @@ int orc_threads(void) {
 #endif
 }
 
+/* thread count of the following parallel regions (the reference times its CPU solvers at LPMCG_THREADS = 6,
+ * timing/t2d/a2common.py:10) */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  extern void omp_set_num_threads(int);
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* out = K u (3D).  S: scratch (N, 4) = S0, S1, TS0, TS1. */
 void orc_apply3(int64_t N, const int64_t* nbr, const double* u, const double* Kn, const double* Tv, double* out,
                 double* S) {
