@@ -252,7 +252,15 @@ int lpm_amg_level0(const lpm_grid* g, const lpm_amg* a, int32_t dtype, int32_t o
 int lpm_amg_levelq(const lpm_amg* a, int32_t k, int32_t dtype, int32_t op, int32_t col, const void* in, void* out,
                    void* stream);
 /* y = A x (b null) or y = b - A x on a stencil of grid (gx, gy, gz); nsys systems of x and y (n, 6) share A and b
- * (amg3d.py:230-253). */
+ * (amg3d.py:230-253).  Summation order depends on the stencil kernel (lpm_amg_set_stencil_kernel). */
+/* Kernel of the one-system stencil products (process-wide; AUTO by default): ROWS = one thread per (row, node), the
+ * small-level form; NODE = one thread per node; TWO_PASS = forward products and stored transposed products, then a
+ * gather, reading every stored block once (inside the AMG cycle, whose workspace holds 84 values per node of the
+ * first stencil level; lpm_amg_st_mv has no workspace and takes AUTO's other choices).  AUTO: ROWS below 2^17 nodes,
+ * TWO_PASS from there (NODE outside the cycle).  A
+ * performance switch of this library with no counterpart in the reference (amg3d.py:230-253 is one product). */
+enum { LPM_AMG_ST_AUTO = 0, LPM_AMG_ST_ROWS = 1, LPM_AMG_ST_NODE = 2, LPM_AMG_ST_TWO_PASS = 3 };
+int lpm_amg_set_stencil_kernel(int32_t mode);
 int lpm_amg_st_mv(int32_t gx, int32_t gy, int32_t gz, int32_t nsys, int32_t dtype, const void* A, const void* x,
                   const void* b, void* y, void* stream);
 /* Minv (SoA (36, n)) = inverses of the diagonal blocks of the stencil A (np.linalg.inv(A[:, 13]), amg3d.py:536);

@@ -49,6 +49,7 @@ _SIGS = {
     "lpm_amg_level0": (L.I32, [L.GP, ctypes.POINTER(AmgStruct), L.I32, L.I32, L.I32, L.P, L.P, L.P]),
     "lpm_amg_levelq": (L.I32, [ctypes.POINTER(AmgStruct), L.I32, L.I32, L.I32, L.I32, L.P, L.P, L.P]),
     "lpm_amg_st_mv": (L.I32, [L.I32, L.I32, L.I32, L.I32, L.I32, L.P, L.P, L.P, L.P, L.P]),
+    "lpm_amg_set_stencil_kernel": (L.I32, [L.I32]),
     "lpm_amg_inv_diag": (L.I32, [L.I32, L.I32, L.P, L.P, L.P, L.P]),
     "lpm_amg_bj": (L.I32, [ctypes.c_int64, L.I32, L.I32, L.P, L.P, ctypes.c_double, L.I32, L.P, L.P]),
     "lpm_amg_coarsest": (L.I32, [L.I32, L.I32, L.I32, L.I32, L.P, L.P, L.P, L.P]),
@@ -76,6 +77,15 @@ def _lib():
 
 def _p(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+ST_AUTO, ST_ROWS, ST_NODE, ST_TWO_PASS = 0, 1, 2, 3
+
+
+def set_stencil_kernel(mode):
+    """Kernel of the stencil products y = A x (process-wide): ST_AUTO (by level size), ST_ROWS (one thread per row
+    and node), ST_NODE (one thread per node) or ST_TWO_PASS (every stored block read once; inside the cycle)."""
+    L.check(_lib().lpm_amg_set_stencil_kernel(int(mode)))
 
 
 def chunk_index(L_, a, rule="merge"):

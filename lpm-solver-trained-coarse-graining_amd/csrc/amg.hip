@@ -4,6 +4,8 @@
 // one elementwise pass that also applies the smoother; the FP64 residual of the refinement loop is lpm_residual.
 #include <algorithm>
 #include <cmath>
+#include <atomic>
+#include <cstddef>
 #include <vector>
 
 #include "amg.hpp"
@@ -14,8 +16,15 @@ using lpmc::fail;
 
 namespace {
 
-constexpr int VEC_BLOCKS = 512;  // partial sums of the dot products (LPM_AMG_SCRATCH_BYTES = 8 VEC_BLOCKS)
+constexpr int VEC_BLOCKS = 512;  // partial sums of lpm_amg_dot (LPM_AMG_SCRATCH_BYTES = 8 VEC_BLOCKS)
 static_assert(LPM_AMG_SCRATCH_BYTES >= 8 * VEC_BLOCKS, "scratch holds the partial sums");
+constexpr int PCG_BLOCKS = 2048;
+// stencil products y = A x of one system: one thread per (row, node) below kStRowsBelow nodes, the two-pass form
+// (every stored block read once, 84 values of workspace per node) from kStTwoPassFrom nodes where the caller has
+// that workspace, one thread per node otherwise; lpm_amg_set_stencil_kernel fixes one of them
+constexpr int64_t kStRowsBelow = 1 << 17;
+constexpr int64_t kStTwoPassFrom = 1 << 17;
+std::atomic<int> g_st_mode{LPM_AMG_ST_AUTO};  // blocks of the PCG vector kernels (8 per CU), one partial sum each
 
 inline unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
 inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -135,15 +144,35 @@ int levelq(const lpm_amg* a, int k, int op, int col, const void* in, void* out, 
 }
 
 template <typename T>
-int st_mv(const int g[3], int nsys, const void* A, const void* x, const void* b, void* y, hipStream_t st) {
+int st_mv(const int g[3], int nsys, const void* A, const void* x, const void* b, void* y, hipStream_t st,
+          void* two_pass_work = nullptr) {
   const int64_t n = nodes(g);
   if (nsys == 6 && !b) {
     hipLaunchKernelGGL((amg_st_mv_kernel<T, 6>), dim3(blocks_for(n)), dim3(256), 0, st, g[0], g[1], g[2], (const T*)A,
                        (const T*)x, (const T*)nullptr, (T*)y);
   } else {
-    for (int s = 0; s < nsys; ++s)
-      hipLaunchKernelGGL((amg_st_mv_kernel<T, 1>), dim3(blocks_for(n)), dim3(256), 0, st, g[0], g[1], g[2],
-                         (const T*)A, (const T*)x + s * n * 6, (const T*)b, (T*)y + s * n * 6);
+    const int mode = g_st_mode.load(std::memory_order_relaxed);
+    const bool two = two_pass_work && (mode == LPM_AMG_ST_TWO_PASS || (mode == LPM_AMG_ST_AUTO && n >= kStTwoPassFrom));
+    const bool rows = !two && (mode == LPM_AMG_ST_ROWS || ((mode == LPM_AMG_ST_AUTO || mode == LPM_AMG_ST_TWO_PASS) &&
+                                                           n < kStRowsBelow));
+    T* yp = (T*)two_pass_work;
+    T* tt = yp + 6 * n;
+    for (int s = 0; s < nsys; ++s) {
+      const T* xs = (const T*)x + s * n * 6;
+      T* ys = (T*)y + s * n * 6;
+      if (two) {
+        hipLaunchKernelGGL((amg_st_fwd_kernel<T>), dim3(blocks_for(n)), dim3(256), 0, st, g[0], g[1], g[2],
+                           (const T*)A, xs, yp, tt);
+        hipLaunchKernelGGL((amg_st_bwd_kernel<T>), dim3(blocks_for(n)), dim3(256), 0, st, g[0], g[1], g[2],
+                           (const T*)yp, (const T*)tt, (const T*)b, ys);
+      } else if (rows) {
+        hipLaunchKernelGGL((amg_st_mv_rows_kernel<T>), dim3(blocks_for(6 * n)), dim3(256), 0, st, g[0], g[1], g[2],
+                           (const T*)A, xs, (const T*)b, ys);
+      } else {
+        hipLaunchKernelGGL((amg_st_mv_kernel<T, 1>), dim3(blocks_for(n)), dim3(256), 0, st, g[0], g[1], g[2],
+                           (const T*)A, xs, (const T*)b, ys);
+      }
+    }
   }
   LAUNCHCHK();
   return LPM_OK;
@@ -180,6 +209,7 @@ struct Cycle {
   T* x[LPM_AMG_MAXLEV];
   T* r[LPM_AMG_MAXLEV];
   T* q[LPM_AMG_MAXLEV];  // restriction stage buffer (n_k, 6)
+  void* stw;             // two-pass stencil product: yp (6, n_k) and the transposed products (13, 6, n_k)
 
   int64_t N() const { return (int64_t)g->nx * g->ny * g->nz; }
 
@@ -215,7 +245,7 @@ struct Cycle {
     const int64_t n = nodes(a->g[k]);
     if (k == a->nlev - 1) {
       const int m = (int)(6 * n);
-      hipLaunchKernelGGL((amg_dense_mv_kernel<T>), dim3(blocks_for(m)), dim3(256), 0, st, m, (const T*)a->dinv, rhs, out);
+      hipLaunchKernelGGL((amg_dense_mv_kernel<T>), dim3((m + 3) / 4), dim3(256), 0, st, m, (const T*)a->dinv, rhs, out);
       LAUNCHCHK();
       return LPM_OK;
     }
@@ -223,15 +253,15 @@ struct Cycle {
     int rc;
     if ((rc = bj<T>(n, 6, a->Minv[k], rhs, w, 0, out, st))) return rc;
     for (int it = 0; it < a->nu_c - 1; ++it) {
-      if ((rc = st_mv<T>(a->g[k], 1, a->A[k], out, rhs, r[k], st))) return rc;
+      if ((rc = st_mv<T>(a->g[k], 1, a->A[k], out, rhs, r[k], st, stw))) return rc;
       if ((rc = bj<T>(n, 6, a->Minv[k], r[k], w, 1, out, st))) return rc;
     }
-    if ((rc = st_mv<T>(a->g[k], 1, a->A[k], out, rhs, r[k], st))) return rc;
+    if ((rc = st_mv<T>(a->g[k], 1, a->A[k], out, rhs, r[k], st, stw))) return rc;
     if ((rc = levelq<T>(a, k, LPM_AMG_RESTRICT, 0, r[k], b[k + 1], st, q[k]))) return rc;
     if ((rc = coarse(k + 1, b[k + 1], e[k + 1]))) return rc;
     if ((rc = levelq<T>(a, k, LPM_AMG_PROLONG_ADD, 0, e[k + 1], out, st))) return rc;
     for (int it = 0; it < a->nu_c; ++it) {
-      if ((rc = st_mv<T>(a->g[k], 1, a->A[k], out, rhs, r[k], st))) return rc;
+      if ((rc = st_mv<T>(a->g[k], 1, a->A[k], out, rhs, r[k], st, stw))) return rc;
       if ((rc = bj<T>(n, 6, a->Minv[k], r[k], w, 1, out, st))) return rc;
     }
     return LPM_OK;
@@ -246,7 +276,7 @@ struct Cycle {
     const int extra = lvl >= a->w_from ? a->gamma - 1 : 0;
     const int64_t n6 = nodes(a->g[k]) * 6;
     for (int c = 0; c < extra; ++c) {
-      if ((rc = st_mv<T>(a->g[k], 1, a->A[k], ek, rhs, t[k], st))) return rc;
+      if ((rc = st_mv<T>(a->g[k], 1, a->A[k], ek, rhs, t[k], st, stw))) return rc;
       if ((rc = vc(k, t[k], x[k]))) return rc;
       hipLaunchKernelGGL((amg_add_kernel<T>), dim3(std::min<unsigned>(blocks_for(n6), 4096)), dim3(256), 0, st, n6,
                          (const T*)x[k], ek);
@@ -274,7 +304,7 @@ struct Cycle {
 };
 
 struct WorkLayout {
-  size_t r0, rwork, lev[LPM_AMG_MAXLEV][6], pcg[4], part, total;
+  size_t r0, rwork, lev[LPM_AMG_MAXLEV][6], stw, pcg[4], part, scal, total;
 };
 
 template <typename T>
@@ -294,12 +324,16 @@ WorkLayout work_layout(const lpm_grid* g, const lpm_amg* a) {
       o += v;
     }
   }
+  L.stw = o;  // the first stencil level is the largest
+  o += align256(sizeof(T) * nodes(a->g[0]) * 6 * 14);
   for (int j = 0; j < 4; ++j) {  // PCG: r, z, p, Ap
     L.pcg[j] = o;
     o += vec;
   }
   L.part = o;
-  o += align256(sizeof(double) * VEC_BLOCKS);
+  o += align256(sizeof(double) * PCG_BLOCKS);
+  L.scal = o;
+  o += align256(sizeof(AmgScal));
   L.total = o;
   return L;
 }
@@ -327,6 +361,7 @@ Cycle<T> make_cycle(const lpm_grid* g, const lpm_amg* a, int layout, const void*
     c.r[k] = (T*)(w + L.lev[k][4]);
     c.q[k] = (T*)(w + L.lev[k][5]);
   }
+  c.stw = w + L.stw;
   return c;
 }
 
@@ -343,6 +378,64 @@ int dot(int64_t n, const T* x, const T* y, double* part, double* out, hipStream_
   return LPM_OK;
 }
 
+// the PCG vector kernels: W values per load where every vector and the mask allow 16-byte loads, one otherwise
+template <typename T>
+struct PcgVec {
+  int64_t n;
+  unsigned nblk;
+  bool wide;
+  double* part;
+  AmgScal* sc;
+  hipStream_t st;
+
+  template <int W>
+  unsigned blocks() const {
+    return (unsigned)std::min<int64_t>(PCG_BLOCKS, std::max<int64_t>(1, (n / W + 255) / 256));
+  }
+  int dot(const T* x, const T* y, int op, double tol = 0.0, int maxit = 0) {
+    if (wide)
+      hipLaunchKernelGGL((amg_dotv_kernel<T, 16 / sizeof(T)>), dim3(nblk), dim3(256), 0, st, n, x, y, part);
+    else
+      hipLaunchKernelGGL((amg_dotv_kernel<T, 1>), dim3(nblk), dim3(256), 0, st, n, x, y, part);
+    LAUNCHCHK();
+    return fin(op, tol, maxit);
+  }
+  int fin(int op, double tol, int maxit) {
+    hipLaunchKernelGGL(amg_fin_kernel, dim3(1), dim3(256), 0, st, (const double*)part, (int)nblk, sc, op, tol, maxit);
+    LAUNCHCHK();
+    return LPM_OK;
+  }
+  int step(const T* p, const T* ap, const uint8_t* mask, T* x, T* r, double tol, int maxit) {
+    if (wide)
+      hipLaunchKernelGGL((amg_cg_stepv_kernel<T, 16 / sizeof(T)>), dim3(nblk), dim3(256), 0, st, n,
+                         (const AmgScal*)sc, p, ap, mask, x, r, part);
+    else
+      hipLaunchKernelGGL((amg_cg_stepv_kernel<T, 1>), dim3(nblk), dim3(256), 0, st, n, (const AmgScal*)sc, p, ap,
+                         mask, x, r, part);
+    LAUNCHCHK();
+    return fin(AF_RN, tol, maxit);
+  }
+  int xpby(const T* z, T* p) {
+    if (wide)
+      hipLaunchKernelGGL((amg_xpbyv_kernel<T, 16 / sizeof(T)>), dim3(nblk), dim3(256), 0, st, n, z,
+                         (const AmgScal*)sc, p);
+    else
+      hipLaunchKernelGGL((amg_xpbyv_kernel<T, 1>), dim3(nblk), dim3(256), 0, st, n, z, (const AmgScal*)sc, p);
+    LAUNCHCHK();
+    return LPM_OK;
+  }
+};
+
+// the solve's scalars, read back into page-locked memory allocated once per thread
+AmgScal* host_scal() {
+  thread_local AmgScal* h = nullptr;
+  if (!h && hipHostMalloc((void**)&h, sizeof(AmgScal), hipHostMallocDefault) != hipSuccess) h = nullptr;
+  return h;
+}
+
+// AMG-PCG from x = 0 (amg3d.py:649-672).  The scalars stay on the device: p.Ap, alpha, |r|, beta and the stopping
+// test are one-block kernels over fixed-order partial sums, and the vector kernels read them there.  The host waits
+// once per iteration, for the 4-byte active flag after the residual update, before it queues the cycle.
 template <typename T>
 int pcg(const lpm_grid* g, const lpm_amg* a, int layout, const void* kn, const void* tv, const uint8_t* mask,
         const T* b, T* x, double tol, int maxit, void* work, int32_t* iters, double* relres, hipStream_t st) {
@@ -353,54 +446,49 @@ int pcg(const lpm_grid* g, const lpm_amg* a, int layout, const void* kn, const v
   T* z = (T*)(w + L.pcg[1]);
   T* p = (T*)(w + L.pcg[2]);
   T* ap = (T*)(w + L.pcg[3]);
-  double* part = (double*)(w + L.part);
   const int64_t N = cy.N(), n3 = N * 3;
+  AmgScal* hs = host_scal();
+  if (!hs) return fail(LPM_E_HIP, "AMG-PCG: page-locked host memory unavailable");
+  constexpr int WV = 16 / sizeof(T);
+  PcgVec<T> v{n3, 0, false, (double*)(w + L.part), (AmgScal*)(w + L.scal), st};
+  v.wide = ((uintptr_t)x % 16 == 0) && ((uintptr_t)b % 16 == 0) && ((uintptr_t)mask % WV == 0);
+  v.nblk = v.wide ? v.template blocks<WV>() : v.template blocks<1>();
   const unsigned eg = std::min<unsigned>(blocks_for(n3), 4096);
   int rc;
+  auto poll = [&](size_t bytes) -> int {
+    HIPCHK(hipMemcpyAsync(hs, v.sc, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return LPM_OK;
+  };
   hipLaunchKernelGGL((amg_fill_kernel<T>), dim3(eg), dim3(256), 0, st, n3, T(0), x);
   HIPCHK(hipMemcpyAsync(r, b, sizeof(T) * n3, hipMemcpyDeviceToDevice, st));
-  double bn2;
-  if ((rc = dot<T>(n3, b, b, part, &bn2, st))) return rc;
-  const double bn = std::sqrt(bn2);
+  if ((rc = v.dot(b, b, AF_BN))) return rc;
+  if ((rc = poll(sizeof(AmgScal)))) return rc;
   *iters = 0;
   *relres = 0.0;
-  if (bn == 0.0) return LPM_OK;
+  if (hs->nonfinite) return fail(LPM_E_NONFINITE, "AMG-PCG: non-finite right-hand side");
+  if (hs->bn == 0.0) return LPM_OK;
   if ((rc = cy.fine(r, z))) return rc;
   HIPCHK(hipMemcpyAsync(p, z, sizeof(T) * n3, hipMemcpyDeviceToDevice, st));
-  double rz;
-  if ((rc = dot<T>(n3, r, z, part, &rz, st))) return rc;
-  int it = 0;
-  double rn = bn;
+  if ((rc = v.dot(r, z, AF_RZ0))) return rc;
   const lpm_bcast full{n3, n3}, none{0, 0};
-  for (it = 1; it <= maxit; ++it) {
+  static_assert(offsetof(AmgScal, active) + sizeof(int) <= sizeof(AmgScal), "flag inside the scalars");
+  for (int it = 1; it <= maxit; ++it) {
     if ((rc = lpm_apply(g, 1, 1, sizeof(T) == 4 ? LPM_F32 : LPM_F64, layout, p, full, kn, tv, none, ap, st)))
       return rc;
-    double pap;
-    if ((rc = dot<T>(n3, p, ap, part, &pap, st))) return rc;
-    if (!(pap > 0.0)) return fail(LPM_E_NOT_SPD, "AMG-PCG: p.Ap = %g", pap);
-    const double alpha = rz / pap;
-    hipLaunchKernelGGL((amg_cg_step_kernel<T>), dim3(VEC_BLOCKS), dim3(256), 0, st, n3, (T)alpha, (const T*)p,
-                       (const T*)ap, mask, x, r, part);
-    LAUNCHCHK();
-    {
-      std::vector<double> h(VEC_BLOCKS);
-      HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(double) * VEC_BLOCKS, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      double s = 0.0;
-      for (int i = 0; i < VEC_BLOCKS; ++i) s += h[i];
-      rn = std::sqrt(s);
-    }
-    if (!std::isfinite(rn)) return fail(LPM_E_NONFINITE, "AMG-PCG: non-finite residual");
-    if (rn <= tol * bn) break;
+    if ((rc = v.dot(p, ap, AF_PAP))) return rc;
+    if ((rc = v.step(p, ap, mask, x, r, tol, maxit))) return rc;
+    if ((rc = poll(offsetof(AmgScal, active) + sizeof(int)))) return rc;
+    if (!hs->active) break;
     if ((rc = cy.fine(r, z))) return rc;
-    double rz2;
-    if ((rc = dot<T>(n3, r, z, part, &rz2, st))) return rc;
-    hipLaunchKernelGGL((amg_xpby_kernel<T>), dim3(eg), dim3(256), 0, st, n3, (const T*)z, (T)(rz2 / rz), p);
-    LAUNCHCHK();
-    rz = rz2;
+    if ((rc = v.dot(r, z, AF_RZ))) return rc;
+    if ((rc = v.xpby(z, p))) return rc;
   }
-  *iters = std::min(it, maxit);
-  *relres = rn / bn;
+  if ((rc = poll(sizeof(AmgScal)))) return rc;
+  if (hs->notspd) return fail(LPM_E_NOT_SPD, "AMG-PCG: p.Ap = %g at iteration %d", hs->pap, hs->it + 1);
+  if (hs->nonfinite) return fail(LPM_E_NONFINITE, "AMG-PCG: non-finite residual at iteration %d", hs->it);
+  *iters = hs->it;
+  *relres = hs->rn / hs->bn;
   return LPM_OK;
 }
 
@@ -462,6 +550,13 @@ int lpm_amg_levelq(const lpm_amg* a, int32_t k, int32_t dtype, int32_t op, int32
   if (dtype == LPM_F64) return levelq<double>(a, k, op, col, in, out, st);
   if (dtype == LPM_F32) return levelq<float>(a, k, op, col, in, out, st);
   return fail(LPM_E_BADARG, "lpm_amg_levelq: bad dtype %d", dtype);
+}
+
+int lpm_amg_set_stencil_kernel(int32_t mode) {
+  if (mode < LPM_AMG_ST_AUTO || mode > LPM_AMG_ST_TWO_PASS)
+    return fail(LPM_E_BADARG, "lpm_amg_set_stencil_kernel: mode %d", mode);
+  g_st_mode.store(mode);
+  return LPM_OK;
 }
 
 int lpm_amg_st_mv(int32_t gx, int32_t gy, int32_t gz, int32_t nsys, int32_t dtype, const void* A, const void* x,

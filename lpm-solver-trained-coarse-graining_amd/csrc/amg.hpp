@@ -254,6 +254,111 @@ __global__ __launch_bounds__(256) void amg_st_mv_kernel(int gx, int gy, int gz, 
   }
 }
 
+// y = A x or y = b - A x as amg_st_mv_kernel with NS = 1, one thread per (row r, node): six times the threads, for
+// the small levels where one thread per node leaves the chip mostly idle.  Thread t: r = t / n, node t % n.
+template <typename T>
+__global__ __launch_bounds__(256) void amg_st_mv_rows_kernel(int gx, int gy, int gz, const T* __restrict__ A,
+                                                             const T* __restrict__ x, const T* __restrict__ b,
+                                                             T* __restrict__ y) {
+  const int n = gx * gy * gz;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * n) return;
+  const int r = t / n, I = t - r * n;
+  const int g[3] = {gx, gy, gz};
+  int c[3];
+  node_xyz(I, g, c);
+  T acc = T(0);
+  for (int q = 0; q < 27; ++q) {
+    const int j0 = c[0] + q % 3 - 1, j1 = c[1] + (q / 3) % 3 - 1, j2 = c[2] + q / 9 - 1;
+    if (j0 < 0 || j0 >= gx || j1 < 0 || j1 >= gy || j2 < 0 || j2 >= gz) continue;
+    const int J = (j2 * gy + j1) * gx + j0;
+    const bool fwd = q >= 13;
+    const int e = fwd ? q - 13 : 13 - q;
+    // row r of block (I, J): entries (r, k) of the stored forward block, or (k, r) of J's stored block
+    const T* Ad = A + (int64_t)e * 36 * n + (fwd ? I : J);
+    const int o0 = fwd ? r * 6 : r, ok = fwd ? 1 : 6;
+    const T* xj = x + (int64_t)J * 6;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc += Ad[(int64_t)(o0 + k * ok) * n] * xj[k];
+  }
+  y[(int64_t)I * 6 + r] = b ? b[(int64_t)I * 6 + r] - acc : acc;
+}
+
+// y = A x or y = b - A x as amg_st_mv_kernel with NS = 1 in two passes that read every stored block once.  Pass 1:
+// the forward products of node I (blocks e = 0..13, stored at I) into yp (6, n), and the transposed products
+// tt[e - 1] = A_e(I)^T x_I (6, n) for e = 1..13, which belong to the neighbour I + off_e.  Pass 2: y_I = yp_I +
+// sum_e tt[e - 1](I - off_e) (or b minus that).
+template <typename T>
+__global__ __launch_bounds__(256) void amg_st_fwd_kernel(int gx, int gy, int gz, const T* __restrict__ A,
+                                                         const T* __restrict__ x, T* __restrict__ yp,
+                                                         T* __restrict__ tt) {
+  const int n = gx * gy * gz;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  const int g[3] = {gx, gy, gz};
+  int c[3];
+  node_xyz(I, g, c);
+  T xi[6], acc[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    xi[k] = x[(int64_t)I * 6 + k];
+    acc[k] = T(0);
+  }
+  for (int e = 0; e < 14; ++e) {
+    const int q = 13 + e;
+    const int j0 = c[0] + q % 3 - 1, j1 = c[1] + (q / 3) % 3 - 1, j2 = c[2] + q / 9 - 1;
+    if (j0 < 0 || j0 >= gx || j1 < 0 || j1 >= gy || j2 < 0 || j2 >= gz) continue;
+    const int J = (j2 * gy + j1) * gx + j0;
+    T ab[36], xv[6];
+    const T* Ad = A + (int64_t)e * 36 * n + I;
+#pragma unroll
+    for (int k = 0; k < 36; ++k) ab[k] = Ad[(int64_t)k * n];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) xv[k] = x[(int64_t)J * 6 + k];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[r] += ab[r * 6 + k] * xv[k];
+    if (e > 0) {
+      T* te = tt + (int64_t)(e - 1) * 6 * n + I;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        T t = T(0);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) t += ab[r * 6 + k] * xi[r];
+        te[(int64_t)k * n] = t;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 6; ++r) yp[(int64_t)r * n + I] = acc[r];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void amg_st_bwd_kernel(int gx, int gy, int gz, const T* __restrict__ yp,
+                                                         const T* __restrict__ tt, const T* __restrict__ b,
+                                                         T* __restrict__ y) {
+  const int n = gx * gy * gz;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  const int g[3] = {gx, gy, gz};
+  int c[3];
+  node_xyz(I, g, c);
+  T acc[6];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) acc[r] = yp[(int64_t)r * n + I];
+  for (int e = 1; e < 14; ++e) {
+    const int q = 13 + e;  // the neighbour I - off_e owns the block
+    const int j0 = c[0] - (q % 3 - 1), j1 = c[1] - ((q / 3) % 3 - 1), j2 = c[2] - (q / 9 - 1);
+    if (j0 < 0 || j0 >= gx || j1 < 0 || j1 >= gy || j2 < 0 || j2 >= gz) continue;
+    const T* te = tt + (int64_t)(e - 1) * 6 * n + (j2 * gy + j1) * gx + j0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) acc[r] += te[(int64_t)r * n];
+  }
+#pragma unroll
+  for (int r = 0; r < 6; ++r) y[(int64_t)I * 6 + r] = b ? b[(int64_t)I * 6 + r] - acc[r] : acc[r];
+}
+
 // members of aggregate K of a node level: first node and lengths per axis
 __device__ __forceinline__ void agg_box(const AmgLGeo& l, int K, int s0[3], int len[3], int kc[3]) {
   node_xyz(K, l.gc, kc);
@@ -575,15 +680,18 @@ __global__ __launch_bounds__(1024) void amg_spd_inverse_kernel(int m, T* __restr
   }
 }
 
-// x = Dinv b for the symmetric dense inverse (m x m); thread i reads column i (= row i), coalesced
+// x = Dinv b for the symmetric dense inverse (m x m, column i taken as row i): one wave per entry of x, the lanes
+// over the column (the matrix is small and stays in L2), a fixed order of summation
 template <typename T>
 __global__ __launch_bounds__(256) void amg_dense_mv_kernel(int m, const T* __restrict__ Dinv, const T* __restrict__ b,
                                                            T* __restrict__ xo) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (i >= m) return;
   T s = T(0);
-  for (int j = 0; j < m; ++j) s += Dinv[(int64_t)j * m + i] * b[j];
-  xo[i] = s;
+  for (int j = lane; j < m; j += 64) s += Dinv[(int64_t)j * m + i] * b[j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) xo[i] = s;
 }
 
 // ---------------------------------------------------------------------------------------------- vectors
@@ -601,31 +709,189 @@ __global__ __launch_bounds__(256) void amg_dot_kernel(int64_t n, const T* __rest
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// x += alpha p; r -= alpha mask Ap; per-block partial sums of r.r (amg3d.py:399-408 with the free mask of apply0)
+// ------------------------------------------------------------------------------------------ AMG-PCG on the device
+// Scalars of one AMG-PCG solve kept in device memory (amg3d.py:649-672): the host only polls `active`.
+struct AmgScal {
+  double bn, rz, pap, alpha, beta, rn;
+  int it, active, conv, notspd, nonfinite, pad_;
+};
+enum AmgFin : int { AF_BN = 0, AF_RZ0, AF_PAP, AF_RN, AF_RZ };
+
+// W consecutive values of T and of the mask bytes (W = 16 / sizeof(T) for 16-byte loads, 1 for unaligned buffers)
+template <typename T, int W>
+struct VecT {
+  T v[W];
+};
+template <int W>
+struct VecM {
+  uint8_t v[W];
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void amg_cg_step_kernel(int64_t n, T alpha, const T* __restrict__ p,
-                                                          const T* __restrict__ ap, const uint8_t* __restrict__ mask,
-                                                          T* __restrict__ x, T* __restrict__ r,
-                                                          double* __restrict__ partial) {
-  __shared__ double red[4];
-  double s = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
-    const T ri = r[i] - (mask[i] ? alpha * ap[i] : T(0));
-    r[i] = ri;
-    s += (double)ri * (double)ri;
-  }
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+__device__ inline double block_sum4(double t, double* red) {
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
   __syncthreads();
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// p = z + beta p (amg3d.py:411-415)
-template <typename T>
-__global__ __launch_bounds__(256) void amg_xpby_kernel(int64_t n, const T* __restrict__ z, T beta, T* __restrict__ p) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+// per-block partial sums of x.y, U vectors of W values in flight per thread; the tail past the last whole vector by
+// block 0
+template <typename T, int W>
+__global__ __launch_bounds__(256) void amg_dotv_kernel(int64_t n, const T* __restrict__ x, const T* __restrict__ y,
+                                                       double* __restrict__ partial) {
+  __shared__ double red[4];
+  constexpr int U = 4;
+  using V = VecT<T, W>;
+  const int64_t nv = n / W;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double s[U] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < nv; i0 += stride * U) {
+    V a[U], c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * stride < nv) {
+        a[u] = ((const V*)x)[i0 + u * stride];
+        c[u] = ((const V*)y)[i0 + u * stride];
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * stride < nv)
+#pragma unroll
+        for (int j = 0; j < W; ++j) s[u] += (double)a[u].v[j] * (double)c[u].v[j];
+  }
+  double t = (s[0] + s[1]) + (s[2] + s[3]);
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * W) t += (double)x[nv * W + threadIdx.x] * (double)y[nv * W + threadIdx.x];
+  t = block_sum4<T>(t, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// the scalar step of the PCG from the fixed-order sum of the partials (one block of 256)
+__global__ __launch_bounds__(256) void amg_fin_kernel(const double* __restrict__ partial, int nblk,
+                                                      AmgScal* __restrict__ sc, int op, double tol, int maxit) {
+  __shared__ double red[4];
+  double t = 0.0;
+  for (int j = threadIdx.x; j < nblk; j += blockDim.x) t += partial[j];
+  const double s = block_sum4<double>(t, red);
+  if (threadIdx.x != 0) return;
+  if (op == AF_BN) {
+    sc->bn = sqrt(s);
+    sc->rn = sc->bn;
+    sc->it = 0;
+    sc->conv = sc->notspd = sc->nonfinite = 0;
+    sc->active = sc->bn > 0.0 ? 1 : 0;
+    if (!(sc->bn <= 1e300)) {  // non-finite right-hand side
+      sc->nonfinite = 1;
+      sc->active = 0;
+    }
+    return;
+  }
+  if (!sc->active) return;
+  if (op == AF_RZ0) {
+    sc->rz = s;
+  } else if (op == AF_PAP) {
+    sc->pap = s;
+    if (!(s > 0.0)) {  // K not positive definite on the search direction
+      sc->notspd = 1;
+      sc->active = 0;
+    } else {
+      sc->alpha = sc->rz / s;
+    }
+  } else if (op == AF_RN) {
+    sc->it += 1;
+    sc->rn = sqrt(s);
+    if (!(sc->rn <= 1e300)) {
+      sc->nonfinite = 1;
+      sc->active = 0;
+    } else if (sc->rn <= tol * sc->bn) {
+      sc->conv = 1;
+      sc->active = 0;
+    } else if (sc->it >= maxit) {
+      sc->active = 0;
+    }
+  } else if (op == AF_RZ) {
+    sc->beta = s / sc->rz;
+    sc->rz = s;
+  }
+}
+
+// x += alpha p; r -= alpha mask Ap; partial sums of r.r (amg3d.py:399-408 with the free mask of apply0); alpha and
+// the active flag from the device scalars
+template <typename T, int W>
+__global__ __launch_bounds__(256) void amg_cg_stepv_kernel(int64_t n, const AmgScal* __restrict__ sc,
+                                                           const T* __restrict__ p, const T* __restrict__ ap,
+                                                           const uint8_t* __restrict__ mask, T* __restrict__ x,
+                                                           T* __restrict__ r, double* __restrict__ partial) {
+  __shared__ double red[4];
+  constexpr int U = 2;
+  using V = VecT<T, W>;
+  using M = VecM<W>;
+  const bool active = sc->active != 0;
+  const T alpha = (T)sc->alpha;
+  const int64_t nv = n / W;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double s[U] = {0.0, 0.0};
+  if (active) {
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < nv; i0 += stride * U) {
+      V pv[U], av[U], xv[U], rv[U];
+      M mv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = i0 + u * stride;
+        if (i < nv) {
+          pv[u] = ((const V*)p)[i];
+          av[u] = ((const V*)ap)[i];
+          xv[u] = ((const V*)x)[i];
+          rv[u] = ((const V*)r)[i];
+          mv[u] = ((const M*)mask)[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = i0 + u * stride;
+        if (i < nv) {
+#pragma unroll
+          for (int j = 0; j < W; ++j) {
+            xv[u].v[j] += alpha * pv[u].v[j];
+            rv[u].v[j] -= mv[u].v[j] ? alpha * av[u].v[j] : T(0);
+            s[u] += (double)rv[u].v[j] * (double)rv[u].v[j];
+          }
+          ((V*)x)[i] = xv[u];
+          ((V*)r)[i] = rv[u];
+        }
+      }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
+      const int64_t i = nv * W + threadIdx.x;
+      x[i] += alpha * p[i];
+      const T ri = r[i] - (mask[i] ? alpha * ap[i] : T(0));
+      r[i] = ri;
+      s[0] += (double)ri * (double)ri;
+    }
+  }
+  const double t = block_sum4<T>(s[0] + s[1], red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// p = z + beta p while the solve is active (amg3d.py:411-415)
+template <typename T, int W>
+__global__ __launch_bounds__(256) void amg_xpbyv_kernel(int64_t n, const T* __restrict__ z,
+                                                        const AmgScal* __restrict__ sc, T* __restrict__ p) {
+  if (!sc->active) return;
+  using V = VecT<T, W>;
+  const T beta = (T)sc->beta;
+  const int64_t nv = n / W;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+    const V zv = ((const V*)z)[i];
+    V pv = ((const V*)p)[i];
+#pragma unroll
+    for (int j = 0; j < W; ++j) pv.v[j] = zv.v[j] + beta * pv.v[j];
+    ((V*)p)[i] = pv;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {
+    const int64_t i = nv * W + threadIdx.x;
     p[i] = z[i] + beta * p[i];
+  }
 }
 
 // out = a - b (elementwise)

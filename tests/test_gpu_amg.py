@@ -84,9 +84,21 @@ def test_stencils_and_rho_match_oracle(dims, coarse_max):
         assert abs(a_ - b_) <= FP64_TOL * b_
 
 
+@pytest.fixture
+def stencil_kernel(request):
+    """Selects the stencil-product kernel for one test and restores the automatic choice after it."""
+    amg3d.set_stencil_kernel(request.param)
+    yield request.param
+    amg3d.set_stencil_kernel(amg3d.ST_AUTO)
+
+
+_ST_MODES = [amg3d.ST_AUTO, amg3d.ST_ROWS, amg3d.ST_NODE, amg3d.ST_TWO_PASS]
+
+
 @pytest.mark.gpu
+@pytest.mark.parametrize("stencil_kernel", _ST_MODES, indirect=True)
 @pytest.mark.parametrize("gamma", [1, 2])
-def test_cycle_matches_oracle(gamma):
+def test_cycle_matches_oracle(gamma, stencil_kernel):
     dims = (20, 19, 21)
     lat, Kn, Tv, free, b, ref, gpu = _amg_pair(dims, 100, gamma=gamma)
     rng = np.random.default_rng(7)
@@ -97,8 +109,9 @@ def test_cycle_matches_oracle(gamma):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("stencil_kernel", [amg3d.ST_AUTO, amg3d.ST_TWO_PASS], indirect=True)
 @pytest.mark.parametrize("load", ["k_xy", "tb_tz"])
-def test_amg_pcg_matches_superlu_and_oracle(load):
+def test_amg_pcg_matches_superlu_and_oracle(load, stencil_kernel):
     dims = (21, 20, 19)
     lat, Kn, Tv, free, b, ref, gpu = _amg_pair(dims, 100, load=load)
     x, it, rr = gpu.pcg(torch.as_tensor(b, device="cuda"), 1e-10, 500)
@@ -131,7 +144,8 @@ def test_amg_pcg_zero_rhs_and_single_level():
 
 
 @pytest.mark.gpu
-def test_refined_fp32_amg_reaches_fp64_accuracy():
+@pytest.mark.parametrize("stencil_kernel", [amg3d.ST_AUTO, amg3d.ST_TWO_PASS], indirect=True)
+def test_refined_fp32_amg_reaches_fp64_accuracy(stencil_kernel):
     """FP32 AMG-PCG inner solves inside the FP64 refinement reach rel-res 1e-8 on the true FP64 residual, and the
     displacement agrees with SuperLU within 1e-6."""
     dims = (21, 20, 19)
