@@ -351,7 +351,13 @@ def bench_plain2d(a):
     mfree = P["mask"]
     b = -ops.apply(dims, 1.0, P["up"], P["Kn"], P["Tv"]) * mfree.double()
 
+    # the single-launch persistent CG (one cooperative launch per solve) is the timed path
+    cgp = ops.Cg2Persistent(dims, P["Kn"], P["Tv"], mfree)
+
     def gpu_solve():
+        return cgp.solve(b, tol=a.tol, maxit=200000)
+
+    def batched_solve():
         return ops.pcg(dims, 1.0, b, kp, tp, minv=None, free=mfree, tol=a.tol, maxit=200000, check=50,
                        graph=True, layout=L.PACKED)
     for _ in range(a.warmup):
@@ -363,6 +369,14 @@ def bench_plain2d(a):
         x, info = gpu_solve()
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0)
+    x = x.reshape(-1, 2)
+    xb, info_b = batched_solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xb, info_b = batched_solve()
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    diff_b = float((xb.reshape(-1, 2) - x).abs().max() / x.abs().max())
     req = serve.Request(128, 128, 1.0, P["Kn"], P["Tv"], mfree, P["up"], "tension")
     t0 = time.perf_counter()
     up_plain, it_plain = serve.plain_cg(req, a.tol)
@@ -385,7 +399,10 @@ def bench_plain2d(a):
     _line(METRIC, a.steps / sum(lat), "solves/s", a.steps, a.warmup, float(lat_ms.mean()), "f64",
           "plain 2D LPM 128^2 (32768 DOF), disks phi 0.30 contrast 10, unpreconditioned CG to rel-res %g" % a.tol,
           {"latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
-           "solve": {"cg_iters": info.it, "relres": float(info.relmax), "phase_fraction": P["phase_fraction"]},
+           "solve": {"cg_iters": info.it, "relres": float(info.relmax), "phase_fraction": P["phase_fraction"],
+                     "path": "lpm_cg2_persistent (one cooperative launch, %d workgroups)" % 32},
+           "batched_graph_cg": {"solves_per_s": 1.0 / t_batched, "iters": info_b.it, "max_rel_diff": diff_b,
+                                "note": "lpm_pcg_solve, launch per kernel step replayed from a HIP graph"},
            "plain_gpu_cg": {"solves_per_s": 1.0 / t_plain, "iters": it_plain, "max_rel_diff": diff,
                             "note": "host-driven alpha/beta, one solve, FP64, same HIP K u"},
            "cpu_baseline": {"value": 1.0 / t_cpu, "unit": "solves/s", "cores": int(lib.orc_threads()), "kind": "port",

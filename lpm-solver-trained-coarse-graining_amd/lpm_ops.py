@@ -361,6 +361,35 @@ def pcg(dims, h, b, Kn, Tv, minv=None, free=None, tol=1e-10, maxit=20000, x0=Non
                         snap_iters=sit.reshape(nb, nl, len(tl)))
 
 
+class Cg2Persistent:
+    """Plain FP64 CG of one 2D square-lattice system in one cooperative launch (cg2plain.cg_mf,
+    timing/t2d/cg2plain.py:11-49).  Kn, Tv (N, 8) directed FP64, free (N, 2) per-DOF mask (nonzero = free);
+    solve(b) returns x (N, 2) from a zero start and SolveInfo.  The workspace is allocated once per instance;
+    workgroups = 0 picks the launch shape automatically."""
+
+    def __init__(self, dims, Kn, Tv, free, workgroups=0):
+        nx, ny = int(dims[0]), int(dims[1])
+        N = nx * ny
+        self.dims, self.workgroups = (nx, ny), int(workgroups)
+        dev = Kn.device
+        self.kn = Kn.to(torch.float64).reshape(N, 8).contiguous()
+        self.tv = Tv.to(torch.float64).reshape(N, 8).contiguous()
+        self.mask = (free.reshape(N, 2) != 0).to(torch.uint8).contiguous()
+        self.work = torch.empty(int(L.lib().lpm_cg2_persistent_workspace(nx, ny)), device=dev, dtype=torch.uint8)
+
+    def solve(self, b, tol=1e-8, maxit=200000, out=None):
+        nx, ny = self.dims
+        N = nx * ny
+        be = b.to(torch.float64).reshape(N, 2).contiguous()
+        x = out if out is not None else torch.empty((N, 2), device=be.device, dtype=torch.float64)
+        it, conv = ctypes.c_int32(0), ctypes.c_int32(0)
+        rel = ctypes.c_double(0.0)
+        L.check(L.lib().lpm_cg2_persistent(nx, ny, _ptr(self.kn), _ptr(self.tv), _ptr(self.mask), _ptr(be), _ptr(x),
+                                           float(tol), int(maxit), self.workgroups, _ptr(self.work), ctypes.byref(it),
+                                           ctypes.byref(rel), ctypes.byref(conv), L.stream_of(be)))
+        return x, SolveInfo(np.array([[it.value]]), np.array([[rel.value]]), np.array([[conv.value]]))
+
+
 def pcg_refined(dims, h, b, Kn64, Tv64, Kn32, Tv32, minv32=None, free=None, tol=1e-8, maxit=20000, inner_tol=1e-5,
                 max_outer=30, x0=None, check=10, graph=False, layout64=L.DIRECTED, layout32=L.DIRECTED, bonds=None):
     """FP32 (P)CG inner solves inside an FP64 iterative refinement; b FP64 (…, N, d).  Returns x FP64 and SolveInfo

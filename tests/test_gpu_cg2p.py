@@ -1,0 +1,101 @@
+"""Single-launch persistent FP64 CG of one 2D system (lpm_cg2_persistent) against the oracle's plain CG
+(timing/t2d/cg2plain.py:11-49 restated in oracle/lpm_ref.cg_plain): solutions and iteration counts over launch
+shapes (one row per workgroup, several rows, a ragged last workgroup, 1, 2 and 4 particles per thread, a single
+workgroup), the per-DOF roller + pinned-node mask of the plain 2D benchmark, a zero right-hand side, the maxit stop,
+a matrix that is not positive definite and a lattice too wide for one workgroup's LDS."""
+import numpy as np
+import pytest
+import torch
+
+import lpm_ops as ops
+import lpm_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(dims, seed):
+    """Random symmetric bonds, bottom row on rollers (y fixed), node 0 pinned, top row pulled in y."""
+    lat = R.Box(dims)
+    Kn, Tv = R.random_symmetric_params(lat, np.random.default_rng(seed), kn=(0.5, 5.0), tv=(-0.05, 0.3))
+    j = lat.ijk[:, 1]
+    m = np.ones((lat.N, 2))
+    m[j == 0, 1] = 0
+    m[j == dims[1] - 1, 1] = 0
+    m[0, 0] = 0
+    up = np.zeros((lat.N, 2))
+    up[j == dims[1] - 1, 1] = 1e-3 * (dims[1] - 1)
+    b = -R.apply(lat, up, Kn, Tv) * m
+    return lat, Kn, Tv, m, b
+
+
+def _solver(dims, Kn, Tv, m, dev, workgroups=0):
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    return ops.Cg2Persistent(dims, t(Kn), t(Tv), torch.as_tensor(m, device=dev), workgroups=workgroups)
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+# (dims, workgroups): rows per workgroup = ceil(ny / min(workgroups or 64, ny)); particles per thread from rows nx
+SHAPES = [((16, 12), 0), ((37, 23), 7), ((37, 23), 1), ((120, 9), 2), ((40, 30), 3), ((64, 64), 0), ((9, 70), 0),
+          ((33, 40), 40)]
+
+
+@pytest.mark.parametrize("dims,workgroups", SHAPES)
+def test_persistent_cg_matches_oracle(dev, dims, workgroups):
+    print("cg2p", dims, workgroups, flush=True)
+    lat, Kn, Tv, m, b = _problem(dims, 3 + dims[0])
+    x, info = _solver(dims, Kn, Tv, m, dev, workgroups).solve(torch.as_tensor(b, device=dev), tol=1e-12)
+    xr, it, rr = R.cg_plain(lat, b, Kn, Tv, m, tol=1e-12)
+    assert info.conv.all() and info.relmax <= 1e-12
+    assert abs(info.it - it) <= 2, (info.it, it)
+    assert rel(x.cpu().numpy(), xr) < 1e-9
+
+
+def test_persistent_cg_matches_batched_pcg(dev):
+    """Same iterations and solution as the launch-per-iteration plain CG of lpm_pcg_solve."""
+    dims = (48, 40)
+    lat, Kn, Tv, m, b = _problem(dims, 9)
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
+    free = torch.as_tensor(m.astype(np.uint8), device=dev)
+    x1, i1 = _solver(dims, Kn, Tv, m, dev).solve(t(b), tol=1e-10)
+    x2, i2 = ops.pcg(dims, 1.0, t(b), t(Kn), t(Tv), minv=None, free=free, tol=1e-10, maxit=100000)
+    assert abs(i1.it - i2.it) <= 1
+    assert rel(x1.cpu().numpy(), x2.reshape(-1, 2).cpu().numpy()) < 1e-9
+
+
+def test_persistent_cg_zero_rhs_and_maxit(dev):
+    dims = (24, 20)
+    lat, Kn, Tv, m, b = _problem(dims, 5)
+    s = _solver(dims, Kn, Tv, m, dev)
+    x, info = s.solve(torch.zeros((lat.N, 2), device=dev, dtype=torch.float64), tol=1e-10)
+    assert info.it == 0 and info.conv.all() and info.relmax == 0.0 and float(x.abs().max()) == 0.0
+    # stopped after 5 iterations: the oracle's iterate after 5 iterations, not converged
+    x, info = s.solve(torch.as_tensor(b, device=dev), tol=1e-12, maxit=5)
+    xr, it, rr = R.cg_plain(lat, b, Kn, Tv, m, tol=1e-12, maxit=5)
+    assert info.it == 5 and it == 5 and not info.conv.any()
+    assert rel(x.cpu().numpy(), xr) < 1e-12
+    assert abs(info.relmax - rr) <= 1e-9 * rr
+    # the same instance solves again (barrier counter and flags reset per launch)
+    x2, info2 = s.solve(torch.as_tensor(b, device=dev), tol=1e-12, maxit=5)
+    assert info2.it == 5 and torch.equal(x, x2)
+
+
+def test_persistent_cg_not_spd_raises(dev):
+    """K = -K_spd: p.Ap < 0 at the first iteration; the solve stops with FloatingPointError (LPM_E_NOT_SPD)."""
+    dims = (20, 16)
+    lat, Kn, Tv, m, b = _problem(dims, 7)
+    with pytest.raises(FloatingPointError):
+        _solver(dims, -Kn, -Tv, m, dev).solve(torch.as_tensor(b, device=dev), tol=1e-10)
+
+
+def test_persistent_cg_too_wide_is_refused(dev):
+    """One workgroup cannot hold the LDS rows of a 1000-wide lattice: NotImplementedError, nothing launched."""
+    dims = (1000, 4)
+    lat = R.Box(dims)
+    z = torch.zeros((lat.N, 8), device=dev, dtype=torch.float64)
+    s = ops.Cg2Persistent(dims, z + 1.0, z, torch.ones((lat.N, 2), device=dev, dtype=torch.uint8))
+    with pytest.raises(NotImplementedError):
+        s.solve(torch.ones((lat.N, 2), device=dev, dtype=torch.float64))

@@ -51,14 +51,20 @@ def rel(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
 
 
-def test_c1_plain2d_128_matches_superlu(dev):
-    print("C1 plain2d 128", flush=True)
+@pytest.mark.parametrize("path", ["batched", "persistent"])
+def test_c1_plain2d_128_matches_superlu(dev, path):
+    """path: the batched launch-per-iteration CG (lpm_pcg_solve) or the single-launch persistent CG
+    (lpm_cg2_persistent)."""
+    print("C1 plain2d 128", path, flush=True)
     W = workloads.plain2d(128, device=dev)
     dims = W["dims"]
     b = -ops.apply(dims, 1.0, W["up"], W["Kn"], W["Tv"]) * W["mask"]
-    x, info = ops.pcg(dims, 1.0, b, W["Kn"], W["Tv"], minv=None, free=W["mask"], tol=1e-8, maxit=200000)
+    if path == "batched":
+        x, info = ops.pcg(dims, 1.0, b, W["Kn"], W["Tv"], minv=None, free=W["mask"], tol=1e-8, maxit=200000)
+    else:
+        x, info = ops.Cg2Persistent(dims, W["Kn"], W["Tv"], W["mask"]).solve(b, tol=1e-8, maxit=200000)
     assert info.conv.all() and info.relmax <= 1e-8
-    ug = W["up_np"] + x.cpu().numpy()
+    ug = W["up_np"] + x.reshape(-1, 2).cpu().numpy()
     lat = R.Box(dims)
     Kn, Tv, mask, up = W["Kn_np"], W["Tv_np"], W["mask_np"], W["up_np"]
     uo = R.solve_direct(R.stiffness(lat, Kn, Tv), mask.ravel() == 0, up).reshape(-1, 2)
