@@ -28,7 +28,7 @@ PHASE_KN = np.array([0.1, 1.0, 2.0, 5.0])
 # HBM bytes per launch of the masked K p kernel at 256^3 (FP32, packed), from rocprofv3 --pmc FETCH_SIZE and
 # --pmc WRITE_SIZE runs of `bench.py --probe 256` (KiB per dispatch), scaled by the same counters of a 1.61 GB copy
 # (factors 2.0 and 1.0; profiles/ku3d_256cube_fp32_packed.md)
-KU_TRAFFIC_256 = 936727 * 1024 * 2.0 + 196612 * 1024 * 1.0
+KU_TRAFFIC_256 = 802852 * 1024 * 2.0 + 196611 * 1024 * 1.0
 
 
 def probe(n, reps):
