@@ -400,7 +400,7 @@ def bench_plain2d(a):
           "plain 2D LPM 128^2 (32768 DOF), disks phi 0.30 contrast 10, unpreconditioned CG to rel-res %g" % a.tol,
           {"latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
            "solve": {"cg_iters": info.it, "relres": float(info.relmax), "phase_fraction": P["phase_fraction"],
-                     "path": "lpm_cg2_persistent (one cooperative launch, %d workgroups)" % 32},
+                     "path": "lpm_cg2_persistent (one cooperative launch, %d workgroups of 3 rows)" % -(-128 // 3)},
            "batched_graph_cg": {"solves_per_s": 1.0 / t_batched, "iters": info_b.it, "max_rel_diff": diff_b,
                                 "note": "lpm_pcg_solve, launch per kernel step replayed from a HIP graph"},
            "plain_gpu_cg": {"solves_per_s": 1.0 / t_plain, "iters": it_plain, "max_rel_diff": diff,

@@ -7,11 +7,13 @@
 // (p.Ap, r.r).  An exchange is also the barrier: each workgroup publishes its partial sum as epoch-tagged words with
 // device-coherent stores once its earlier stores have completed, and every wave polls the words of all workgroups
 // and sums the payloads in one fixed order, so every workgroup takes the same decisions from bit-identical scalars.
-// No cache writeback or invalidation is needed: every array that crosses workgroups (r, p, the words) is read and
-// written only with device-coherent accesses.  The new p of the two halo rows each side is recomputed locally as
-// fma(beta, p_old, r) from the neighbours' published r and previous p (double-buffered by iteration parity), which
-// is the owner's own arithmetic, so no third exchange is needed before the next K p.  Every wait is bounded: on
-// time-out the solve stops with an error.
+// No cache writeback or invalidation is needed: every array that crosses workgroups (p_0, r, the words) is read and
+// written only with device-coherent accesses.  r is published as epoch-tagged words too (two per double, buffered
+// by iteration parity), and the neighbours' halo r is polled in the same loop as the partial sums of r.r, so it
+// arrives with that exchange instead of a load round trip after it.  The new p of the two halo rows each side is
+// recomputed locally as fma(beta, p_old, r) from that r and the previous p in LDS, which is the owner's own
+// arithmetic, so no third exchange is needed before the next K p.  Every wait is bounded: on time-out the solve
+// stops with an error.
 #include <algorithm>
 #include <cmath>
 
@@ -25,7 +27,7 @@ namespace {
 
 constexpr int kCgpMaxWaves = 16;  // waves of the largest workgroup (1024 threads)
 constexpr int kCgpMaxWg = 64;     // one lane of a wave per workgroup in the gather
-constexpr int kCgpAutoWg = 32;    // workgroups of the automatic launch shape
+constexpr int kCgpAutoWg = 48;    // workgroups requested by the automatic launch shape
 constexpr long kSpinLimit = 1L << 22;  // polls of one exchange (each with s_sleep, seconds in all) before giving up
 
 enum CgpStatus : int { CGP_CONV = 0, CGP_MAXIT = 1, CGP_NOTSPD = 2, CGP_NONFINITE = 3, CGP_TIMEOUT = 4 };
@@ -38,7 +40,7 @@ struct CgpArgs {
   const double* b;      // (N, 2)
   double* x;            // (N, 2) out
   double* pbuf;         // (N, 2) p_0 (exchanged once)
-  double* rbuf;         // (N, 2) r (exchanged)
+  uint64_t* rwords;     // (2, N, 2, 2) r by iteration parity, each double as two words (epoch << 32) | half
   uint64_t* slots;      // (2, kCgpMaxWg, 4) published partial sums: (epoch << 32) | half of a double (zero at launch)
   int* abort_flag;      // set by a workgroup whose exchange wait timed out (zero at launch)
   double tol;
@@ -74,9 +76,9 @@ __device__ __forceinline__ double xld(const double* p) {
 // words of all workgroups, lane j those of workgroup j, until every tag equals `epoch`, and sums the payloads over
 // the lanes in a fixed order, so every wave of every workgroup obtains the same bits.  Returns false (in every
 // thread of the workgroup) when a wait timed out or another workgroup gave up.
-template <int NT>
+template <int NT, bool WAIT, typename Extra>
 __device__ bool exchange(const CgpArgs& a, unsigned epoch, double v0, double v1, double* red, double& s0,
-                         double& s1) {
+                         double& s1, Extra&& extra_poll) {
   constexpr int NW = NT / 64;
   v0 = wave_sum(v0);
   v1 = wave_sum(v1);
@@ -85,7 +87,7 @@ __device__ bool exchange(const CgpArgs& a, unsigned epoch, double v0, double v1,
     red[2 * wv] = v0;
     red[2 * wv + 1] = v1;
   }
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores of exchanged data have completed
+  if constexpr (WAIT) __builtin_amdgcn_s_waitcnt(0);  // this wave's stores of untagged exchanged data completed
   __syncthreads();
   uint64_t* slot = a.slots + (size_t)(epoch & 1u) * kCgpMaxWg * 4;
   if (threadIdx.x == 0) {
@@ -106,7 +108,7 @@ __device__ bool exchange(const CgpArgs& a, unsigned epoch, double v0, double v1,
   const bool mine_lane = lane < a.nwg;
   bool ok = true;
   for (long spins = 0;; ++spins) {
-    bool have = true;
+    bool have = extra_poll();
     if (mine_lane) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -136,6 +138,18 @@ __device__ bool exchange(const CgpArgs& a, unsigned epoch, double v0, double v1,
   for (int i = 0; i < NW; ++i) all_ok = all_ok && red[2 * kCgpMaxWaves + i] != 0.0;
   __syncthreads();
   return all_ok;
+}
+
+__device__ __forceinline__ bool no_extra() { return true; }
+
+// a double as two tagged words and back
+__device__ __forceinline__ void put_tagged(uint64_t* w, unsigned epoch, double v) {
+  const uint64_t tag = (uint64_t)epoch << 32, b = (uint64_t)__double_as_longlong(v);
+  __hip_atomic_store(w, tag | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(w + 1, tag | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double from_tagged(uint64_t hi, uint64_t lo) {
+  return __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
 }
 
 // one site's slots that stay in the box, as bits
@@ -168,6 +182,7 @@ __device__ __forceinline__ void shell_site(const double* pL, int nx, int lp, int
 
 template <int NT, int PPT>
 __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
+  constexpr int HMAX = (8 * 128 + NT - 1) / NT;  // halo values of r per thread (lattices up to 128 wide)
   extern __shared__ double lds[];
   const int nx = a.nx, ny = a.ny, R = a.rows;
   const CgpLds L(nx, R);
@@ -210,7 +225,6 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
       bb += bv * bv;
       rr += rr_[j][c] * rr_[j][c];
       if (own) {
-        xst(a.rbuf + 2 * gp + c, rr_[j][c]);
         xst(a.pbuf + 2 * gp + c, pr[j][c]);
         pL[(2 * nx + o) * 2 + c] = pr[j][c];
       }
@@ -224,7 +238,7 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
 
   double s0, s1;
   ++epoch;
-  bool ok = exchange<NT>(a, epoch, bb, rr, red + CgpLds::kRed * (epoch & 1u), s0, s1);
+  bool ok = exchange<NT, true>(a, epoch, bb, rr, red + CgpLds::kRed * (epoch & 1u), s0, s1, [] { return no_extra(); });
   const double bn = sqrt(s0);
   rr = s1;
   int it = 0, status = ok ? CGP_CONV : CGP_TIMEOUT;
@@ -292,7 +306,8 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
       }
     }
     ++epoch;
-    if (!exchange<NT>(a, epoch, pap, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1)) {
+    if (!exchange<NT, false>(a, epoch, pap, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1,
+                             [] { return no_extra(); })) {
       status = CGP_TIMEOUT;
       break;
     }
@@ -302,6 +317,9 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
     }
     const double alpha = rr / s0;
     double rn = 0.0;
+    // r of this iteration, published as tagged words (buffer by iteration parity) with the next exchange's epoch
+    const unsigned e2 = epoch + 1;
+    uint64_t* rw = a.rwords + (size_t)(it & 1) * nx * ny * 4;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
       const int o = tid + j * NT;
@@ -312,12 +330,31 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
           xr[j][c] += alpha * pr[j][c];
           rr_[j][c] -= alpha * ap[j][c];
           rn += rr_[j][c] * rr_[j][c];
-          xst(a.rbuf + 2 * gp + c, rr_[j][c]);
+          put_tagged(rw + (gp * 2 + c) * 2, e2, rr_[j][c]);
         }
       }
     }
+    // the neighbours' r on the two halo rows each side, polled together with the partial sums of r.r: thread t
+    // takes halo values t + i NT (row h / (2 nx), site (h mod 2 nx) / 2, component h mod 2)
+    uint64_t hw[HMAX][2];
+    auto halo_poll = [&]() -> bool {
+      bool have = true;
+#pragma unroll
+      for (int i = 0; i < HMAX; ++i) {
+        const int h = tid + i * NT;
+        if (h >= 8 * nx) break;
+        const int hr = h / (2 * nx), rem = h - hr * 2 * nx;
+        const int gy = y0 - 2 + (hr < 2 ? hr : S + hr);
+        if (gy < 0 || gy >= ny) continue;
+        const uint64_t* src = rw + (((size_t)gy * nx + (rem >> 1)) * 2 + (rem & 1)) * 2;
+        hw[i][0] = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        hw[i][1] = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        have = have && (unsigned)(hw[i][0] >> 32) == e2 && (unsigned)(hw[i][1] >> 32) == e2;
+      }
+      return have;
+    };
     ++epoch;
-    if (!exchange<NT>(a, epoch, rn, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1)) {
+    if (!exchange<NT, false>(a, epoch, rn, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1, halo_poll)) {
       status = CGP_TIMEOUT;
       break;
     }
@@ -346,15 +383,16 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
           pL[(2 * nx + o) * 2 + c] = pr[j][c];
         }
     }
-    for (int t = tid; t < 4 * nx; t += NT) {
-      const int hr = t / nx, gx = t % nx;
+#pragma unroll
+    for (int i = 0; i < HMAX; ++i) {
+      const int h = tid + i * NT;
+      if (h >= 8 * nx) break;
+      const int hr = h / (2 * nx), rem = h - hr * 2 * nx;
       const int lr = hr < 2 ? hr : S + hr;
       const int gy = y0 - 2 + lr;
       if (gy < 0 || gy >= ny) continue;
-      const size_t gp = (size_t)gy * nx + gx;
-      double* ph = pL + (lr * nx + gx) * 2;
-      ph[0] = fma(beta, ph[0], xld(a.rbuf + 2 * gp));
-      ph[1] = fma(beta, ph[1], xld(a.rbuf + 2 * gp + 1));
+      double* ph = pL + (lr * nx + (rem >> 1)) * 2 + (rem & 1);
+      *ph = fma(beta, *ph, from_tagged(hw[i][0], hw[i][1]));
     }
     __syncthreads();
   }
@@ -380,16 +418,16 @@ __global__ void cgp_zero_kernel(uint64_t* slots, int* abort_flag) {
   if (threadIdx.x == 0) *abort_flag = 0;
 }
 
-// workspace: p_0 (N, 2), r (N, 2), published words, abort flag, result words
+// workspace: p_0 (N, 2), tagged r words (2, N, 2, 2), published words, abort flag, result words
 struct CgpWork {
-  size_t pbuf, rbuf, slots, flag, out, rel, total;
+  size_t pbuf, rwords, slots, flag, out, rel, total;
   CgpWork(int64_t N) {
     auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
     size_t o = 0;
     pbuf = o;
     o += al(sizeof(double) * 2 * N);
-    rbuf = o;
-    o += al(sizeof(double) * 2 * N);
+    rwords = o;
+    o += al(sizeof(uint64_t) * 8 * N);
     slots = o;
     o += al(sizeof(uint64_t) * 2 * kCgpMaxWg * 4);
     flag = o;
@@ -463,7 +501,7 @@ int lpm_cg2_persistent(int32_t nx, int32_t ny, const double* kn, const double* t
   a.b = b;
   a.x = x;
   a.pbuf = (double*)(w + W.pbuf);
-  a.rbuf = (double*)(w + W.rbuf);
+  a.rwords = (uint64_t*)(w + W.rwords);
   a.slots = (uint64_t*)(w + W.slots);
   a.abort_flag = (int*)(w + W.flag);
   a.tol = tol;
