@@ -123,19 +123,19 @@ def block_index(ijk, s, nc):
 
 
 def interface_cells(phase, n, s):
-    """Cells whose block holds a voxel with a face neighbour of the other phase."""
+    """Cells (nc^3, x fastest) whose s^3 voxel block holds a voxel with a face neighbour of the other phase
+    (coarse3d/geom3d.py:134-147).  The field is padded by repeating its faces, so a box face never counts as an
+    interface, and compared with its six unit shifts."""
     p = np.asarray(phase).reshape(n, n, n)
-    m = np.zeros(p.shape, bool)
+    q = np.pad(p, 1, mode="edge")
+    core = (slice(1, -1),) * 3
+    hit = np.zeros(p.shape, bool)
     for ax in range(3):
-        d = np.diff(p, axis=ax) != 0
-        lo = [slice(None)] * 3
-        hi = [slice(None)] * 3
-        lo[ax] = slice(None, -1)
-        hi[ax] = slice(1, None)
-        m[tuple(lo)] |= d
-        m[tuple(hi)] |= d
+        for sh in (-1, 1):
+            hit |= np.roll(q, sh, axis=ax)[core] != p
     nc = n // s
-    return m.reshape(nc, s, nc, s, nc, s).any(axis=(1, 3, 5)).ravel()
+    blocks = hit[:nc * s, :nc * s, :nc * s].reshape(nc, s, nc, s, nc, s)
+    return blocks.max(axis=(1, 3, 5)).ravel()
 
 
 # ---------------------------------------------------------------------------------------------- materials
@@ -167,6 +167,8 @@ def bond_params_two_end_mean(lat, ppar):
 
 
 def C21_to_mat(C):
+    """Cell stiffnesses are (..., 6, 6) Voigt matrices with engineering shear everywhere, so the reference's
+    conversion helper is the identity: the input comes back unchanged."""
     return C
 
 

@@ -183,3 +183,22 @@ def test_geom2d_matches_oracle():
     Kn, Tv = geom2d.bond_params_two_end_mean(lat, pp)
     Kr, Tr = R.two_end_mean(ref, pp)
     assert np.allclose(Kn, Kr) and np.allclose(Tv, Tr)
+
+
+def test_interface_cells_match_a_voxel_loop():
+    """geom3d.interface_cells against a plain loop over voxels and their face neighbours inside the box."""
+    import geom3d as G3
+    rng = np.random.default_rng(8)
+    for n, s in ((6, 2), (9, 3), (8, 4)):
+        phase = (rng.random(n ** 3) < 0.3).astype(np.int64)
+        p = phase.reshape(n, n, n)
+        nc = n // s
+        want = np.zeros(nc ** 3, bool)
+        for z in range(n):
+            for y in range(n):
+                for x in range(n):
+                    for dz, dy, dx in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+                        zz, yy, xx = z + dz, y + dy, x + dx
+                        if 0 <= zz < n and 0 <= yy < n and 0 <= xx < n and p[zz, yy, xx] != p[z, y, x]:
+                            want[((z // s) * nc + y // s) * nc + x // s] = True
+        assert np.array_equal(G3.interface_cells(phase, n, s), want)
