@@ -179,6 +179,15 @@ int lpm_cg2_persistent(int32_t nx, int32_t ny, const double* kn, const double* t
                        const double* b, double* x, double tol, int32_t maxit, int32_t workgroups, void* work,
                        int32_t* iters, double* relres, int32_t* conv, void* stream);
 
+/* The same solve (timing/t2d/cg2plain.py:11-49) in the pipelined form of CG (Ghysels and Vanroose 2014, Alg. 4
+ * without preconditioner): one grid-wide reduction per iteration, (r.r, w.r) with w = A r kept by recurrence, whose
+ * exchange overlaps the product A w.  Same Krylov iterates as lpm_cg2_persistent in exact arithmetic; rounding
+ * differs, so iteration counts may differ by a few.  Arguments, workspace (lpm_cg2_persistent_workspace), limits
+ * and errors as lpm_cg2_persistent; workgroups 0 = automatic (up to 64). */
+int lpm_cg2_pipelined(int32_t nx, int32_t ny, const double* kn, const double* tv, const uint8_t* mask,
+                      const double* b, double* x, double tol, int32_t maxit, int32_t workgroups, void* work,
+                      int32_t* iters, double* relres, int32_t* conv, void* stream);
+
 /* lpm_pcg_solve without a workspace argument and without any device allocation: no counterpart in the reference;
  * for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The workspace is a file-scope device array of lpm_noalloc_scratch_bytes() (64 MiB) that each GPU the

@@ -57,6 +57,7 @@ SIGS = {
                                 ctypes.POINTER(CgOpts), I32, P, P, P, P, P, P, P, P]),
     "lpm_cg2_persistent_workspace": (ctypes.c_size_t, [I32, I32]),
     "lpm_cg2_persistent": (I32, [I32, I32, P, P, P, P, P, ctypes.c_double, I32, I32, P, P, P, P, P]),
+    "lpm_cg2_pipelined": (I32, [I32, I32, P, P, P, P, P, ctypes.c_double, I32, I32, P, P, P, P, P]),
     "lpm_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "lpm_pcg_solve_noalloc": (I32, [GP, I32, I32, I32, I32, P, P, Bcast, P, Bcast, P, Bcast, P, P, I32,
                                     ctypes.POINTER(CgOpts), P, P, P, P]),

@@ -27,7 +27,7 @@ namespace {
 
 constexpr int kCgpMaxWaves = 16;  // waves of the largest workgroup (1024 threads)
 constexpr int kCgpMaxWg = 64;     // one lane of a wave per workgroup in the gather
-constexpr int kCgpAutoWg = 48;    // workgroups requested by the automatic launch shape
+constexpr int kCgpAutoWg = 64;    // workgroups requested by the automatic launch shape
 constexpr long kSpinLimit = 1L << 22;  // polls of one exchange (each with s_sleep, seconds in all) before giving up
 
 enum CgpStatus : int { CGP_CONV = 0, CGP_MAXIT = 1, CGP_NOTSPD = 2, CGP_NONFINITE = 3, CGP_TIMEOUT = 4 };
@@ -49,16 +49,18 @@ struct CgpArgs {
   double* relres;
 };
 
-// LDS layout (doubles): p on the rows + 2 each side, shell sums on the rows + 1 each side, Tv of the two rows next to
-// the block, two reduction buffers (per-wave partial pairs and per-wave flags)
+// LDS layout (doubles), structure of arrays so that lanes reading consecutive sites read consecutive words (no bank
+// conflicts): the multiplied vector on the rows + 2 each side as (2, R + 4, nx), shell sums on the rows + 1 each side
+// as (4, R + 2, nx), two reduction buffers (per-wave partial pairs, per-wave flags, a broadcast pair)
 struct CgpLds {
-  int p, w, tv, red, total;
-  static constexpr int kRed = 3 * kCgpMaxWaves;
+  int p, w, red, total, vs, ws;
+  static constexpr int kRed = 3 * kCgpMaxWaves + 2;
   __host__ __device__ CgpLds(int nx, int R) {
-    p = 0;                      // (R + 4, nx, 2)
-    w = p + (R + 4) * nx * 2;   // (R + 2, nx, 4)
-    tv = w + (R + 2) * nx * 4;  // (2, nx, 8): rows y0 - 1 and y1
-    red = tv + 2 * nx * 8;      // (2, kRed)
+    vs = (R + 4) * nx;  // component stride of the vector
+    ws = (R + 2) * nx;  // component stride of the shell sums
+    p = 0;
+    w = p + 2 * vs;
+    red = w + 4 * ws;  // (2, kRed)
     total = red + 2 * kRed;
   }
 };
@@ -71,14 +73,14 @@ __device__ __forceinline__ double xld(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid-wide sums of (v0, v1) that also act as a barrier.  The workgroup's sums (fixed order) are published by thread
-// 0 as four tagged words after every thread's earlier stores have completed (s_waitcnt); each wave then polls the
-// words of all workgroups, lane j those of workgroup j, until every tag equals `epoch`, and sums the payloads over
-// the lanes in a fixed order, so every wave of every workgroup obtains the same bits.  Returns false (in every
-// thread of the workgroup) when a wait timed out or another workgroup gave up.
-template <int NT, bool WAIT, typename Extra>
-__device__ bool exchange(const CgpArgs& a, unsigned epoch, double v0, double v1, double* red, double& s0,
-                         double& s1, Extra&& extra_poll) {
+// Grid-wide sums of (v0, v1) that also act as a barrier, in two halves.  xpub: the workgroup's sums (fixed order)
+// are published by thread 0 as four tagged words after every thread's earlier stores have completed (WAIT:
+// s_waitcnt).  xsum: each wave polls the words of all workgroups, lane j those of workgroup j, until every tag
+// equals `epoch`, and sums the payloads over the lanes in a fixed order, so every wave of every workgroup obtains
+// the same bits.  Work placed between the halves overlaps the exchange.  xsum returns false (in every thread of the
+// workgroup) when a wait timed out or another workgroup gave up.
+template <int NT, bool WAIT>
+__device__ void xpub(const CgpArgs& a, unsigned epoch, double v0, double v1, double* red) {
   constexpr int NW = NT / 64;
   v0 = wave_sum(v0);
   v1 = wave_sum(v1);
@@ -104,10 +106,43 @@ __device__ bool exchange(const CgpArgs& a, unsigned epoch, double v0, double v1,
     __hip_atomic_store(mine + 2, tag | (b1 >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(mine + 3, tag | (b1 & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  uint64_t w[4] = {0, 0, 0, 0};
-  const bool mine_lane = lane < a.nwg;
+}
+
+// Polls until `poll` holds in every lane of every wave of the workgroup (each lane checks the tags of its own
+// words).  Bounded: past kSpinLimit polls, or once another workgroup gave up, the abort flag is set and every thread
+// of the workgroup returns false.
+template <int NT, typename Poll>
+__device__ bool wait_words(const CgpArgs& a, double* red, Poll&& poll) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   bool ok = true;
   for (long spins = 0;; ++spins) {
+    if (__all(poll())) break;
+    if (spins > kSpinLimit || __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+      ok = false;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  if (!ok && lane == 0) __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // every wave of the workgroup takes the same decision
+  if (lane == 0) red[2 * kCgpMaxWaves + wv] = ok ? 1.0 : 0.0;
+  __syncthreads();
+  bool all_ok = true;
+  for (int i = 0; i < NW; ++i) all_ok = all_ok && red[2 * kCgpMaxWaves + i] != 0.0;
+  __syncthreads();
+  return all_ok;
+}
+
+template <int NT, typename Extra>
+__device__ bool xsum(const CgpArgs& a, unsigned epoch, double* red, double& s0, double& s1, Extra&& extra_poll) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t* slot = a.slots + (size_t)(epoch & 1u) * kCgpMaxWg * 4;
+  uint64_t w[4] = {0, 0, 0, 0};
+  // the first wave polls the words (lane j those of workgroup j) and passes the sums on through LDS: one wave's
+  // polling per workgroup keeps the traffic on the shared words low
+  const bool mine_lane = wv == 0 && lane < a.nwg;
+  const bool ok = wait_words<NT>(a, red, [&]() -> bool {
     bool have = extra_poll();
     if (mine_lane) {
 #pragma unroll
@@ -116,28 +151,32 @@ __device__ bool exchange(const CgpArgs& a, unsigned epoch, double v0, double v1,
         have = have && (unsigned)(w[k] >> 32) == epoch;
       }
     }
-    if (__all(have)) break;
-    if (spins > kSpinLimit || __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-      ok = false;
-      break;
+    return have;
+  });
+  if (wv == 0) {
+    double d0 = 0.0, d1 = 0.0;
+    if (mine_lane) {
+      d0 = __longlong_as_double((long long)(((w[0] & 0xffffffffull) << 32) | (w[1] & 0xffffffffull)));
+      d1 = __longlong_as_double((long long)(((w[2] & 0xffffffffull) << 32) | (w[3] & 0xffffffffull)));
     }
-    __builtin_amdgcn_s_sleep(1);
+    d0 = wave_sum(d0);
+    d1 = wave_sum(d1);
+    if (lane == 0) {
+      red[CgpLds::kRed - 2] = d0;
+      red[CgpLds::kRed - 1] = d1;
+    }
   }
-  if (!ok && lane == 0) __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  double d0 = 0.0, d1 = 0.0;
-  if (mine_lane) {
-    d0 = __longlong_as_double((long long)(((w[0] & 0xffffffffull) << 32) | (w[1] & 0xffffffffull)));
-    d1 = __longlong_as_double((long long)(((w[2] & 0xffffffffull) << 32) | (w[3] & 0xffffffffull)));
-  }
-  s0 = wave_sum(d0);
-  s1 = wave_sum(d1);
-  // every wave of the workgroup takes the same decision
-  if (lane == 0) red[2 * kCgpMaxWaves + wv] = ok ? 1.0 : 0.0;
   __syncthreads();
-  bool all_ok = true;
-  for (int i = 0; i < NW; ++i) all_ok = all_ok && red[2 * kCgpMaxWaves + i] != 0.0;
-  __syncthreads();
-  return all_ok;
+  s0 = red[CgpLds::kRed - 2];
+  s1 = red[CgpLds::kRed - 1];
+  return ok;
+}
+
+template <int NT, bool WAIT, typename Extra>
+__device__ bool exchange(const CgpArgs& a, unsigned epoch, double v0, double v1, double* red, double& s0,
+                         double& s1, Extra&& extra_poll) {
+  xpub<NT, WAIT>(a, epoch, v0, v1, red);
+  return xsum<NT>(a, epoch, red, s0, s1, extra_poll);
 }
 
 __device__ __forceinline__ bool no_extra() { return true; }
@@ -165,18 +204,80 @@ __device__ __forceinline__ unsigned slot_bits(int gx, int gy, int nx, int ny) {
 
 // shell sums (S0, S1, TS0, TS1) of the site at LDS row lp (of p), column gx: slots leaving the box read the site
 // itself with weight zero, so the loop has no branches
-__device__ __forceinline__ void shell_site(const double* pL, int nx, int lp, int gx, unsigned vb, const double* tvk,
-                                           double w[4]) {
-  const double vx = pL[(lp * nx + gx) * 2], vy = pL[(lp * nx + gx) * 2 + 1];
+__device__ __forceinline__ void shell_site(const double* pL, int vs, int nx, int lp, int gx, unsigned vb,
+                                           const double (&tvk)[8], double w[4]) {
+  const double vx = pL[lp * nx + gx], vy = pL[vs + lp * nx + gx];
   w[0] = w[1] = w[2] = w[3] = 0.0;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const bool in = (vb >> k) & 1u;
     const int q = in ? (lp + off2(k, 1)) * nx + gx + off2(k, 0) : lp * nx + gx;
     const double dl = (in ? 1.0 : 0.0) *
-                      (nd2<double>(k, 0) * (vx - pL[2 * q]) + nd2<double>(k, 1) * (vy - pL[2 * q + 1]));
+                      (nd2<double>(k, 0) * (vx - pL[q]) + nd2<double>(k, 1) * (vy - pL[vs + q]));
     w[shell2(k)] += dl;
     w[2 + shell2(k)] += tvk[k] * dl;
+  }
+}
+
+// Shell sums of v (LDS rows of the band + two halo rows each side) on the band's own particles and on the side-row
+// site of this thread (row y0 - 1 or y1, its Tv in registers), written to wL.  All sums are formed before the stores,
+// so the LDS reads of both kinds of site overlap.  The caller synchronises before wL is read.
+template <int NT, int PPT>
+__device__ __forceinline__ void band_shell(const double* vL, double* wL, const CgpLds& L, int nx, int nown,
+                                           const unsigned (&vb)[PPT], const double (&tv)[PPT][8], bool hown,
+                                           int hls, int hx, unsigned hvb, const double (&tvh)[8]) {
+  double w[PPT][4], wh[4];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int o = threadIdx.x + j * NT;
+    if (o < nown) shell_site(vL, L.vs, nx, o / nx + 2, o % nx, vb[j], tv[j], w[j]);
+  }
+  if (hown) shell_site(vL, L.vs, nx, hls + 1, hx, hvb, tvh, wh);
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int o = threadIdx.x + j * NT;
+    if (o < nown)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) wL[c * L.ws + nx + o] = w[j][c];
+  }
+  if (hown)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) wL[c * L.ws + hls * nx + hx] = wh[c];
+}
+
+// av = mask (K v) on the band's own particles from v (registers and LDS) and the shell sums in wL
+template <int NT, int PPT>
+__device__ __forceinline__ void band_apply(const double* vL, const double* wL, const CgpLds& L, int nx, int nown,
+                                           const unsigned (&vb)[PPT], const double (&kn)[PPT][8],
+                                           const double (&tv)[PPT][8], const double (&mk)[PPT][2],
+                                           const double (&v)[PPT][2], double (&av)[PPT][2]) {
+  const int vs = L.vs, ws = L.ws;
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int o = threadIdx.x + j * NT;
+    av[j][0] = av[j][1] = 0.0;
+    if (o < nown) {
+      const int gx = o % nx, lp = o / nx + 2, ls = o / nx + 1;
+      const double vx = v[j][0], vy = v[j][1];
+      const int si = ls * nx + gx;
+      const double wi[4] = {wL[si], wL[ws + si], wL[2 * ws + si], wL[3 * ws + si]};
+      double f0 = 0.0, f1 = 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const bool in = (vb[j] >> k) & 1u;
+        const int c = shell2(k);
+        const int q = in ? (lp + off2(k, 1)) * nx + gx + off2(k, 0) : lp * nx + gx;
+        const int qs = in ? (ls + off2(k, 1)) * nx + gx + off2(k, 0) : ls * nx + gx;
+        const double wq_c = wL[c * ws + qs], wq_t = wL[(2 + c) * ws + qs];
+        const double dl = nd2<double>(k, 0) * (vx - vL[q]) + nd2<double>(k, 1) * (vy - vL[vs + q]);
+        const double F = (in ? 1.0 : 0.0) * (2.0 * kn[j][k] * dl + 0.5 * (wi[2 + c] + wq_t) +
+                                             0.5 * tv[j][k] * (wi[c] + wq_c));
+        f0 += F * nd2<double>(k, 0);
+        f1 += F * nd2<double>(k, 1);
+      }
+      av[j][0] = f0 * mk[j][0];
+      av[j][1] = f1 * mk[j][1];
+    }
   }
 }
 
@@ -188,18 +289,13 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
   const CgpLds L(nx, R);
   double* pL = lds + L.p;
   double* wL = lds + L.w;
-  double* tvL = lds + L.tv;
   double* red = lds + L.red;
   const int y0 = blockIdx.x * R, y1 = min(ny, y0 + R), S = y1 - y0;
   const int nown = S * nx;
   const int tid = threadIdx.x;
   unsigned epoch = 0;
 
-  // Tv of the rows y0 - 1 and y1 into LDS (their shell sums); Kn, Tv of the own particles in registers
-  for (int t = tid; t < 2 * nx * 8; t += NT) {
-    const int gy = t < nx * 8 ? y0 - 1 : y1;
-    tvL[t] = (gy >= 0 && gy < ny) ? a.tv[(size_t)gy * nx * 8 + t % (nx * 8)] : 0.0;
-  }
+  // Kn, Tv of the own particles in registers
   double kn[PPT][8], tv[PPT][8];
   double xr[PPT][2], rr_[PPT][2], pr[PPT][2], mk[PPT][2];
   unsigned vb[PPT];
@@ -226,7 +322,7 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
       rr += rr_[j][c] * rr_[j][c];
       if (own) {
         xst(a.pbuf + 2 * gp + c, pr[j][c]);
-        pL[(2 * nx + o) * 2 + c] = pr[j][c];
+        pL[c * L.vs + 2 * nx + o] = pr[j][c];
       }
     }
   }
@@ -235,6 +331,9 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
   const int hrow = ht / nx, hx = ht % nx, hls = hrow == 0 ? 0 : S + 1, hgy = y0 - 1 + hls;
   const bool hown = tid < 2 * nx && hgy >= 0 && hgy < ny;
   const unsigned hvb = hown ? slot_bits(hx, hgy, nx, ny) : 0u;
+  double tvh[8];  // Tv of that site
+#pragma unroll
+  for (int k = 0; k < 8; ++k) tvh[k] = hown ? a.tv[((size_t)hgy * nx + hx) * 8 + k] : 0.0;
 
   double s0, s1;
   ++epoch;
@@ -252,59 +351,21 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
       const int gy = y0 - 2 + lr;
       const size_t gp = (size_t)gy * nx + gx;
       const bool in = gy >= 0 && gy < ny;
-      pL[(lr * nx + gx) * 2] = in ? xld(a.pbuf + 2 * gp) : 0.0;
-      pL[(lr * nx + gx) * 2 + 1] = in ? xld(a.pbuf + 2 * gp + 1) : 0.0;
+      pL[lr * nx + gx] = in ? xld(a.pbuf + 2 * gp) : 0.0;
+      pL[L.vs + lr * nx + gx] = in ? xld(a.pbuf + 2 * gp + 1) : 0.0;
     }
     __syncthreads();
   }
   while (!done) {
     // shell sums of p: the own particles by their threads (Tv in registers), rows y0 - 1 and y1 from LDS
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-      const int o = tid + j * NT;
-      if (o < nown) {
-        double w[4];
-        shell_site(pL, nx, o / nx + 2, o % nx, vb[j], tv[j], w);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) wL[(nx + o) * 4 + c] = w[c];
-      }
-    }
-    if (hown) {
-      double w[4];
-      shell_site(pL, nx, hls + 1, hx, hvb, tvL + (size_t)(hrow * nx + hx) * 8, w);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) wL[(hls * nx + hx) * 4 + c] = w[c];
-    }
+    band_shell<NT, PPT>(pL, wL, L, nx, nown, vb, tv, hown, hls, hx, hvb, tvh);
     __syncthreads();
     // A p on the own particles (masked), partial p.Ap
     double ap[PPT][2], pap = 0.0;
+    band_apply<NT, PPT>(pL, wL, L, nx, nown, vb, kn, tv, mk, pr, ap);
 #pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-      const int o = tid + j * NT;
-      ap[j][0] = ap[j][1] = 0.0;
-      if (o < nown) {
-        const int gx = o % nx, lp = o / nx + 2, ls = o / nx + 1;
-        const double vx = pr[j][0], vy = pr[j][1];
-        const double* wi = wL + (size_t)(ls * nx + gx) * 4;
-        double f0 = 0.0, f1 = 0.0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const bool in = (vb[j] >> k) & 1u;
-          const int c = shell2(k);
-          const int q = in ? (lp + off2(k, 1)) * nx + gx + off2(k, 0) : lp * nx + gx;
-          const int qs = in ? (ls + off2(k, 1)) * nx + gx + off2(k, 0) : ls * nx + gx;
-          const double* wq = wL + (size_t)qs * 4;
-          const double dl = nd2<double>(k, 0) * (vx - pL[2 * q]) + nd2<double>(k, 1) * (vy - pL[2 * q + 1]);
-          const double F = (in ? 1.0 : 0.0) * (2.0 * kn[j][k] * dl + 0.5 * (wi[2 + c] + wq[2 + c]) +
-                                               0.5 * tv[j][k] * (wi[c] + wq[c]));
-          f0 += F * nd2<double>(k, 0);
-          f1 += F * nd2<double>(k, 1);
-        }
-        ap[j][0] = f0 * mk[j][0];
-        ap[j][1] = f1 * mk[j][1];
-        pap += pr[j][0] * ap[j][0] + pr[j][1] * ap[j][1];
-      }
-    }
+    for (int j = 0; j < PPT; ++j)
+      if (tid + j * NT < nown) pap += pr[j][0] * ap[j][0] + pr[j][1] * ap[j][1];
     ++epoch;
     if (!exchange<NT, false>(a, epoch, pap, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1,
                              [] { return no_extra(); })) {
@@ -380,7 +441,7 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
           pr[j][c] = fma(beta, pr[j][c], rr_[j][c]);
-          pL[(2 * nx + o) * 2 + c] = pr[j][c];
+          pL[c * L.vs + 2 * nx + o] = pr[j][c];
         }
     }
 #pragma unroll
@@ -391,7 +452,7 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
       const int lr = hr < 2 ? hr : S + hr;
       const int gy = y0 - 2 + lr;
       if (gy < 0 || gy >= ny) continue;
-      double* ph = pL + (lr * nx + (rem >> 1)) * 2 + (rem & 1);
+      double* ph = pL + (rem & 1) * L.vs + lr * nx + (rem >> 1);
       *ph = fma(beta, *ph, from_tagged(hw[i][0], hw[i][1]));
     }
     __syncthreads();
@@ -412,10 +473,317 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
   }
 }
 
-// published words and the abort flag start at zero (no tag of a launch is zero)
-__global__ void cgp_zero_kernel(uint64_t* slots, int* abort_flag) {
-  for (int i = threadIdx.x; i < 2 * kCgpMaxWg * 4; i += blockDim.x) slots[i] = 0;
-  if (threadIdx.x == 0) *abort_flag = 0;
+// Pipelined CG (Ghysels and Vanroose, "Hiding global synchronization latency in the preconditioned Conjugate
+// Gradient algorithm", Parallel Computing 40 (2014), Alg. 4 without preconditioner): the same Krylov iterates as the
+// classic CG in exact arithmetic with one grid-wide reduction per iteration, (r.r, w.r) with w = A r, whose exchange
+// overlaps the product q = A w:
+//   gamma = r.r, delta = w.r (published);  q = A w;  (sums arrive)
+//   beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old)   (first iteration: 0, gamma / delta)
+//   z = q + beta z, s = w + beta s, p = r + beta p, x += alpha p, r -= alpha s, w -= alpha z.
+// The stopping test ||r|| <= tol ||b|| is taken on the recursively updated r before each update and then confirmed
+// on the true residual mask (b - A x) (one more product and exchange), which the recurrences can drift from.  w is
+// published each iteration as tagged words (buffered by tag parity) and the neighbours' two halo rows are polled
+// before the product.  delta - beta gamma / alpha_old equals p.Ap.  Near the attainable residual that difference
+// cancels as the recurrences drift from the quantities they stand for.  Where it is <= 0 the recurrence quantities are
+// replaced by their definitions (residual replacement, Cools et al., SIAM J. Sci. Comput. 40 (2018)):
+// r = mask (b - A x), w = A r, s = A p, z = A s, and the iteration continues with the same search direction (no
+// update counted); so does a true residual that misses the tolerance the recursive one met.  If p.Ap <= 0 right after
+// a replacement the iteration restarts from x with the true residual (p = s = z = 0); p.Ap <= 0 on the first update
+// after a (re)start, where it is the exact r.Ar, stops with NOT_SPD.
+template <int NT, int PPT>
+__global__ __launch_bounds__(NT) void cg2pp_kernel(CgpArgs a) {
+  constexpr int HMAX = (8 * 128 + NT - 1) / NT;  // halo values per thread (lattices up to 128 wide)
+  extern __shared__ double lds[];
+  const int nx = a.nx, ny = a.ny, R = a.rows;
+  const CgpLds L(nx, R);
+  double* vL = lds + L.p;
+  double* wL = lds + L.w;
+  double* red = lds + L.red;
+  const int y0 = blockIdx.x * R, y1 = min(ny, y0 + R), S = y1 - y0;
+  const int nown = S * nx;
+  const int tid = threadIdx.x;
+  const size_t nbuf = (size_t)nx * ny * 4;  // words of one parity buffer
+  unsigned epoch = 0;
+
+  // halo rows outside the lattice hold zeros (never read through a bond)
+  for (int t = tid; t < 8 * nx; t += NT) {
+    const int c = t / (4 * nx), hr = (t % (4 * nx)) / nx, lr = hr < 2 ? hr : S + hr, gy = y0 - 2 + lr;
+    if (gy < 0 || gy >= ny) vL[c * L.vs + lr * nx + t % nx] = 0.0;
+  }
+  double kn[PPT][8], tv[PPT][8], mk[PPT][2];
+  double x[PPT][2], r[PPT][2], w[PPT][2], p[PPT][2], sv[PPT][2], z[PPT][2], bm[PPT][2];
+  unsigned vb[PPT];
+  double bb = 0.0, rr = 0.0;
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int o = tid + j * NT;
+    const bool own = o < nown;
+    const size_t gp = (size_t)y0 * nx + (own ? o : 0);
+    vb[j] = own ? slot_bits(o % nx, y0 + o / nx, nx, ny) : 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      kn[j][k] = own ? a.kn[gp * 8 + k] : 0.0;
+      tv[j][k] = own ? a.tv[gp * 8 + k] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const double bv = own ? a.b[2 * gp + c] : 0.0;
+      mk[j][c] = (own && a.mask[2 * gp + c]) ? 1.0 : 0.0;
+      bm[j][c] = bv * mk[j][c];
+      r[j][c] = bm[j][c];
+      x[j][c] = w[j][c] = p[j][c] = sv[j][c] = z[j][c] = 0.0;
+      bb += bv * bv;
+      rr += r[j][c] * r[j][c];
+    }
+  }
+  // the side-row sites (shell sums only) go to the threads after the own particles when the workgroup has enough
+  // of them, so both kinds run side by side
+  const int hoff = NT >= R * nx + 2 * nx ? R * nx : 0;
+  const int hid = tid - hoff;
+  const int ht = hid >= 0 && hid < 2 * nx ? hid : 0;
+  const int hrow = ht / nx, hx = ht % nx, hls = hrow == 0 ? 0 : S + 1, hgy = y0 - 1 + hls;
+  const bool hown = hid >= 0 && hid < 2 * nx && hgy >= 0 && hgy < ny;
+  const unsigned hvb = hown ? slot_bits(hx, hgy, nx, ny) : 0u;
+  double tvh[8];  // Tv of that site
+#pragma unroll
+  for (int k = 0; k < 8; ++k) tvh[k] = hown ? a.tv[((size_t)hgy * nx + hx) * 8 + k] : 0.0;
+
+  // a band vector published as tagged words (buffer tag & 1), the neighbours' halo rows polled into hw and both
+  // placed in the LDS rows
+  auto publish = [&](unsigned tag, const double (&v)[PPT][2]) {
+    uint64_t* buf = a.rwords + (size_t)(tag & 1u) * nbuf;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int o = tid + j * NT;
+      if (o < nown) {
+        const size_t gp = (size_t)y0 * nx + o;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) put_tagged(buf + (gp * 2 + c) * 2, tag, v[j][c]);
+      }
+    }
+  };
+  uint64_t hw[HMAX][2];
+  auto halo_poll = [&](unsigned tag) -> bool {
+    const uint64_t* buf = a.rwords + (size_t)(tag & 1u) * nbuf;
+    bool have = true;
+#pragma unroll
+    for (int i = 0; i < HMAX; ++i) {
+      const int h = tid + i * NT;
+      if (h >= 8 * nx) break;
+      const int hr = h / (2 * nx), rem = h - hr * 2 * nx;
+      const int gy = y0 - 2 + (hr < 2 ? hr : S + hr);
+      if (gy < 0 || gy >= ny) continue;
+      const uint64_t* src = buf + (((size_t)gy * nx + (rem >> 1)) * 2 + (rem & 1)) * 2;
+      hw[i][0] = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      hw[i][1] = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      have = have && (unsigned)(hw[i][0] >> 32) == tag && (unsigned)(hw[i][1] >> 32) == tag;
+    }
+    return have;
+  };
+  auto place = [&](const double (&v)[PPT][2]) {
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int o = tid + j * NT;
+      if (o < nown) {
+        vL[2 * nx + o] = v[j][0];
+        vL[L.vs + 2 * nx + o] = v[j][1];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < HMAX; ++i) {
+      const int h = tid + i * NT;
+      if (h >= 8 * nx) break;
+      const int hr = h / (2 * nx), rem = h - hr * 2 * nx;
+      const int lr = hr < 2 ? hr : S + hr;
+      const int gy = y0 - 2 + lr;
+      if (gy < 0 || gy >= ny) continue;
+      vL[(rem & 1) * L.vs + lr * nx + (rem >> 1)] = from_tagged(hw[i][0], hw[i][1]);
+    }
+  };
+
+  // |b|^2 and |r_0|^2
+  double s0, s1;
+  ++epoch;
+  const bool ok = exchange<NT, false>(a, epoch, bb, rr, red + CgpLds::kRed * (epoch & 1u), s0, s1,
+                                      [] { return no_extra(); });
+  const double bn = sqrt(s0);
+  int it = 0, status = ok ? CGP_CONV : CGP_TIMEOUT;
+  double rlast = s1;
+  bool done = !ok || bn == 0.0 || sqrt(s1) <= a.tol * bn;
+
+  // Each pass of the loop multiplies one band vector: v published under a new tag, the neighbours' halo rows
+  // polled, shell sums, product.  The mode says which vector and what the product is for: an iteration (q = A w,
+  // with the reduction of gamma, delta overlapping it), the true residual that confirms convergence, or a step of
+  // a replacement (A x, then w = A r, s = A p, z = A s).  One product site keeps the kernel's registers in bounds.
+  enum : int { M_STEP, M_CONFIRM, M_AX, M_AR, M_AP, M_AS };
+  int mode = M_AR;    // w_0 = A r_0 first
+  bool fresh = true;  // the next update is the first of a (re)start: beta = 0 and p.Ap = delta = r.Ar
+  bool exact = true;  // r, w, s, z are the true quantities (start, replacement, restart), not recurrences
+  double gamma_old = 1.0, alpha_old = 1.0;
+  unsigned tag = 0;
+  while (!done) {
+    double* red_e = red + CgpLds::kRed * (epoch & 1u);
+    if (mode == M_STEP) {
+      double pg = 0.0, pd = 0.0;
+#pragma unroll
+      for (int j = 0; j < PPT; ++j)
+        if (tid + j * NT < nown) {
+          pg += r[j][0] * r[j][0] + r[j][1] * r[j][1];
+          pd += w[j][0] * r[j][0] + w[j][1] * r[j][1];
+        }
+      ++epoch;
+      red_e = red + CgpLds::kRed * (epoch & 1u);
+      xpub<NT, false>(a, epoch, pg, pd, red_e);
+    }
+    double v[PPT][2], out[PPT][2];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        v[j][c] = mode == M_STEP ? w[j][c]
+                  : mode == M_AR ? r[j][c]
+                  : mode == M_AP ? p[j][c]
+                  : mode == M_AS ? sv[j][c]
+                                 : x[j][c];
+    publish(++tag, v);
+    if (!wait_words<NT>(a, red_e, [&] { return halo_poll(tag); })) {
+      status = CGP_TIMEOUT;
+      break;
+    }
+    place(v);
+    __syncthreads();
+    band_shell<NT, PPT>(vL, wL, L, nx, nown, vb, tv, hown, hls, hx, hvb, tvh);
+    __syncthreads();
+    band_apply<NT, PPT>(vL, wL, L, nx, nown, vb, kn, tv, mk, v, out);
+    if (mode == M_AR || mode == M_AP || mode == M_AS) {
+#pragma unroll
+      for (int j = 0; j < PPT; ++j)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          if (mode == M_AR) w[j][c] = out[j][c];
+          if (mode == M_AP) sv[j][c] = out[j][c];
+          if (mode == M_AS) z[j][c] = out[j][c];
+        }
+      // after w: with a search direction, its products too (a restart has none)
+      if (mode == M_AR && !fresh) {
+        mode = M_AP;
+      } else if (mode == M_AP) {
+        mode = M_AS;
+      } else {
+        mode = M_STEP;
+        exact = true;
+      }
+      continue;
+    }
+    if (mode == M_AX || mode == M_CONFIRM) {
+      double tr = 0.0;
+#pragma unroll
+      for (int j = 0; j < PPT; ++j)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          r[j][c] = bm[j][c] - out[j][c];
+          tr += r[j][c] * r[j][c];
+        }
+      if (mode == M_CONFIRM) {
+        // the true residual decides
+        ++epoch;
+        if (!exchange<NT, false>(a, epoch, tr, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1,
+                                 [] { return no_extra(); })) {
+          status = CGP_TIMEOUT;
+          break;
+        }
+        rlast = s0;
+        if (sqrt(s0) <= a.tol * bn) break;
+        // missed (the recursive residual drifted below the true one): replacement, the iteration goes on along the
+        // same search direction (a restart from x would meet the tolerance sooner with a worse error: one
+        // steepest-descent-like step takes the residual norm under it while the smooth error stays)
+      }
+      mode = M_AR;
+      continue;
+    }
+    // M_STEP: q = out
+    if (!xsum<NT>(a, epoch, red_e, s0, s1, [] { return no_extra(); })) {
+      status = CGP_TIMEOUT;
+      break;
+    }
+    const double gamma = s0, delta = s1;
+    rlast = gamma;
+    const double rnorm = sqrt(gamma);
+    if (!(rnorm <= 1e300) || delta != delta) {
+      status = CGP_NONFINITE;
+      break;
+    }
+    if (rnorm <= a.tol * bn) {
+      // a recursive r is confirmed by the true one; if that misses the tolerance the quantities are replaced
+      if (exact) break;
+      mode = M_CONFIRM;
+      continue;
+    }
+    if (it >= a.maxit) {
+      status = CGP_MAXIT;
+      break;
+    }
+    const double beta = fresh ? 0.0 : gamma / gamma_old;
+    const double pap = fresh ? delta : delta - beta * gamma / alpha_old;
+    if (!(pap > 0.0)) {
+      if (pap != pap || fresh) {  // non-finite, or K not positive definite on r
+        status = pap == pap ? CGP_NOTSPD : CGP_NONFINITE;
+        break;
+      }
+      // replacement; where the true quantities give no descent either, a restart from x
+      if (exact) {
+        fresh = true;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j)
+#pragma unroll
+          for (int c = 0; c < 2; ++c) p[j][c] = sv[j][c] = z[j][c] = 0.0;
+      }
+      mode = M_AX;
+      continue;
+    }
+    const double alpha = gamma / pap;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        z[j][c] = fma(beta, z[j][c], out[j][c]);
+        sv[j][c] = fma(beta, sv[j][c], w[j][c]);
+        p[j][c] = fma(beta, p[j][c], r[j][c]);
+        x[j][c] = fma(alpha, p[j][c], x[j][c]);
+        r[j][c] = fma(-alpha, sv[j][c], r[j][c]);
+        w[j][c] = fma(-alpha, z[j][c], w[j][c]);
+      }
+    gamma_old = gamma;
+    alpha_old = alpha;
+    fresh = false;
+    exact = false;
+    ++it;
+  }
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const int o = tid + j * NT;
+    if (o < nown) {
+      const size_t gp = (size_t)y0 * nx + o;
+      a.x[2 * gp] = x[j][0];
+      a.x[2 * gp + 1] = x[j][1];
+    }
+  }
+  if (blockIdx.x == 0 && tid == 0) {
+    a.out[0] = it;
+    a.out[1] = status;
+    *a.relres = bn > 0.0 ? sqrt(rlast) / bn : 0.0;
+  }
+}
+
+// published words, tagged vector words and the abort flag start at zero (no tag of a launch is zero), so no word
+// left by an earlier launch on the same workspace can match a tag
+__global__ void cgp_zero_kernel(uint64_t* slots, uint64_t* rwords, int64_t nrw, int* abort_flag) {
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = i0; i < nrw; i += stride) rwords[i] = 0;
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < 2 * kCgpMaxWg * 4; i += blockDim.x) slots[i] = 0;
+    if (threadIdx.x == 0) *abort_flag = 0;
+  }
 }
 
 // workspace: p_0 (N, 2), tagged r words (2, N, 2, 2), published words, abort flag, result words
@@ -441,41 +809,34 @@ struct CgpWork {
 };
 
 // rows per workgroup (about `want` workgroups) and the kernel: 256 threads with 1, 2 or 4 particles each (larger
-// workgroups run out of registers: x, r, p, Kn and Tv of the own particles live there); false when the rows of one
-// workgroup do not fit its threads (the two side rows need 2 nx <= 256) or its LDS
+// workgroups run out of registers: the vectors, Kn and Tv of the own particles live there); false when the rows of
+// one workgroup do not fit its threads (the two side rows need 2 nx <= 256) or its LDS.  The pipelined kernel
+// defaults to the most workgroups (its reduction is hidden behind the product, so smaller bands pay).
 using CgpKernel = void (*)(CgpArgs);
-bool cgp_shape(int nx, int ny, int ncu, size_t max_lds, int want, int& rows, int& nwg, int& nt, CgpKernel& fn) {
-  want = std::min({want > 0 ? want : kCgpAutoWg, kCgpMaxWg, ncu, ny});
+bool cgp_shape(bool pipelined, int nx, int ny, int ncu, size_t max_lds, int want, int& rows, int& nwg, int& nt,
+               CgpKernel& fn) {
+  want = std::min({want > 0 ? want : (pipelined ? kCgpMaxWg : kCgpAutoWg), kCgpMaxWg, ncu, ny});
   rows = (ny + want - 1) / want;
   nwg = (ny + rows - 1) / rows;
   const int per = rows * nx;
   nt = 256;
   if (per <= 256)
-    fn = cg2p_kernel<256, 1>;
+    fn = pipelined ? cg2pp_kernel<256, 1> : cg2p_kernel<256, 1>;
   else if (per <= 512)
-    fn = cg2p_kernel<256, 2>;
+    fn = pipelined ? cg2pp_kernel<256, 2> : cg2p_kernel<256, 2>;
   else if (per <= 1024)
-    fn = cg2p_kernel<256, 4>;
+    fn = pipelined ? cg2pp_kernel<256, 4> : cg2p_kernel<256, 4>;
   else
     return false;
   return 2 * nx <= nt && sizeof(double) * (size_t)CgpLds(nx, rows).total <= max_lds;
 }
 
-}  // namespace
-
-extern "C" {
-
-size_t lpm_cg2_persistent_workspace(int32_t nx, int32_t ny) {
-  if (nx < 1 || ny < 1) return 0;
-  return CgpWork((int64_t)nx * ny).total;
-}
-
-int lpm_cg2_persistent(int32_t nx, int32_t ny, const double* kn, const double* tv, const uint8_t* mask,
-                       const double* b, double* x, double tol, int32_t maxit, int32_t workgroups, void* work,
-                       int32_t* iters, double* relres, int32_t* conv, void* stream) {
+int cg2_persistent(bool pipelined, const char* name, int32_t nx, int32_t ny, const double* kn, const double* tv,
+                   const uint8_t* mask, const double* b, double* x, double tol, int32_t maxit, int32_t workgroups,
+                   void* work, int32_t* iters, double* relres, int32_t* conv, void* stream) {
   if (nx < 1 || ny < 1 || !kn || !tv || !mask || !b || !x || !work || !iters || !relres || !conv || maxit < 1 ||
       !(tol >= 0.0))
-    return fail(LPM_E_BADARG, "lpm_cg2_persistent: bad arguments");
+    return fail(LPM_E_BADARG, "%s: bad arguments", name);
   hipStream_t st = (hipStream_t)stream;
   int dev = 0, ncu = 0, max_lds = 0, coop = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -484,9 +845,9 @@ int lpm_cg2_persistent(int32_t nx, int32_t ny, const double* kn, const double* t
   HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
   int rows = 0, nwg = 0, nt = 0;
   CgpKernel kfn = nullptr;
-  if (!coop || workgroups < 0 || !cgp_shape(nx, ny, ncu, (size_t)max_lds, workgroups, rows, nwg, nt, kfn))
-    return fail(LPM_E_UNSUPPORTED, "lpm_cg2_persistent: a %d x %d lattice does not fit one cooperative launch", nx,
-                ny);
+  if (!coop || workgroups < 0 ||
+      !cgp_shape(pipelined, nx, ny, ncu, (size_t)max_lds, workgroups, rows, nwg, nt, kfn))
+    return fail(LPM_E_UNSUPPORTED, "%s: a %d x %d lattice does not fit one cooperative launch", name, nx, ny);
   const int64_t N = (int64_t)nx * ny;
   const CgpWork W(N);
   char* w = (char*)work;
@@ -513,9 +874,10 @@ int lpm_cg2_persistent(int32_t nx, int32_t ny, const double* kn, const double* t
   (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);  // a no-op where not needed
   int per_cu = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nt, lds));
-  if (per_cu * ncu < nwg)
-    return fail(LPM_E_UNSUPPORTED, "lpm_cg2_persistent: %d workgroups cannot be resident together", nwg);
-  hipLaunchKernelGGL(cgp_zero_kernel, dim3(1), dim3(256), 0, st, a.slots, a.abort_flag);
+  if (per_cu * ncu < nwg) return fail(LPM_E_UNSUPPORTED, "%s: %d workgroups cannot be resident together", name, nwg);
+  const int64_t nrw = 8 * N;  // tagged words: 2 parity buffers x N x 2 components x 2 words
+  const unsigned zb = (unsigned)std::min<int64_t>(1024, (nrw + 255) / 256);
+  hipLaunchKernelGGL(cgp_zero_kernel, dim3(zb), dim3(256), 0, st, a.slots, a.rwords, nrw, a.abort_flag);
   LAUNCHCHK();
   void* params[] = {&a};
   HIPCHK(hipLaunchCooperativeKernel(fn, dim3(nwg), dim3(nt), params, lds, st));
@@ -525,11 +887,33 @@ int lpm_cg2_persistent(int32_t nx, int32_t ny, const double* kn, const double* t
   HIPCHK(hipStreamSynchronize(st));
   *iters = out[0];
   *conv = out[1] == CGP_CONV ? 1 : 0;
-  if (out[1] == CGP_NOTSPD) return fail(LPM_E_NOT_SPD, "lpm_cg2_persistent: p.Ap <= 0 at iteration %d", out[0] + 1);
-  if (out[1] == CGP_NONFINITE) return fail(LPM_E_NONFINITE, "lpm_cg2_persistent: non-finite value at iteration %d",
-                                           out[0] + 1);
-  if (out[1] == CGP_TIMEOUT) return fail(LPM_E_HIP, "lpm_cg2_persistent: grid barrier timed out");
+  if (out[1] == CGP_NOTSPD) return fail(LPM_E_NOT_SPD, "%s: p.Ap <= 0 at iteration %d", name, out[0] + 1);
+  if (out[1] == CGP_NONFINITE) return fail(LPM_E_NONFINITE, "%s: non-finite value at iteration %d", name, out[0] + 1);
+  if (out[1] == CGP_TIMEOUT) return fail(LPM_E_HIP, "%s: grid barrier timed out", name);
   return LPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t lpm_cg2_persistent_workspace(int32_t nx, int32_t ny) {
+  if (nx < 1 || ny < 1) return 0;
+  return CgpWork((int64_t)nx * ny).total;
+}
+
+int lpm_cg2_persistent(int32_t nx, int32_t ny, const double* kn, const double* tv, const uint8_t* mask,
+                       const double* b, double* x, double tol, int32_t maxit, int32_t workgroups, void* work,
+                       int32_t* iters, double* relres, int32_t* conv, void* stream) {
+  return cg2_persistent(false, "lpm_cg2_persistent", nx, ny, kn, tv, mask, b, x, tol, maxit, workgroups, work, iters,
+                        relres, conv, stream);
+}
+
+int lpm_cg2_pipelined(int32_t nx, int32_t ny, const double* kn, const double* tv, const uint8_t* mask,
+                      const double* b, double* x, double tol, int32_t maxit, int32_t workgroups, void* work,
+                      int32_t* iters, double* relres, int32_t* conv, void* stream) {
+  return cg2_persistent(true, "lpm_cg2_pipelined", nx, ny, kn, tv, mask, b, x, tol, maxit, workgroups, work, iters,
+                        relres, conv, stream);
 }
 
 }  // extern "C"

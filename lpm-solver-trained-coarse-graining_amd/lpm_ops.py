@@ -365,12 +365,14 @@ class Cg2Persistent:
     """Plain FP64 CG of one 2D square-lattice system in one cooperative launch (cg2plain.cg_mf,
     timing/t2d/cg2plain.py:11-49).  Kn, Tv (N, 8) directed FP64, free (N, 2) per-DOF mask (nonzero = free);
     solve(b) returns x (N, 2) from a zero start and SolveInfo.  The workspace is allocated once per instance;
-    workgroups = 0 picks the launch shape automatically."""
+    workgroups = 0 picks the launch shape automatically.  pipelined=True runs the pipelined form of the same CG
+    (one grid-wide reduction per iteration, overlapped with the product; lpm_cg2_pipelined): the same iterates in
+    exact arithmetic, rounding differs."""
 
-    def __init__(self, dims, Kn, Tv, free, workgroups=0):
+    def __init__(self, dims, Kn, Tv, free, workgroups=0, pipelined=False):
         nx, ny = int(dims[0]), int(dims[1])
         N = nx * ny
-        self.dims, self.workgroups = (nx, ny), int(workgroups)
+        self.dims, self.workgroups, self.pipelined = (nx, ny), int(workgroups), bool(pipelined)
         dev = Kn.device
         self.kn = Kn.to(torch.float64).reshape(N, 8).contiguous()
         self.tv = Tv.to(torch.float64).reshape(N, 8).contiguous()
@@ -384,9 +386,10 @@ class Cg2Persistent:
         x = out if out is not None else torch.empty((N, 2), device=be.device, dtype=torch.float64)
         it, conv = ctypes.c_int32(0), ctypes.c_int32(0)
         rel = ctypes.c_double(0.0)
-        L.check(L.lib().lpm_cg2_persistent(nx, ny, _ptr(self.kn), _ptr(self.tv), _ptr(self.mask), _ptr(be), _ptr(x),
-                                           float(tol), int(maxit), self.workgroups, _ptr(self.work), ctypes.byref(it),
-                                           ctypes.byref(rel), ctypes.byref(conv), L.stream_of(be)))
+        fn = L.lib().lpm_cg2_pipelined if self.pipelined else L.lib().lpm_cg2_persistent
+        L.check(fn(nx, ny, _ptr(self.kn), _ptr(self.tv), _ptr(self.mask), _ptr(be), _ptr(x), float(tol), int(maxit),
+                   self.workgroups, _ptr(self.work), ctypes.byref(it), ctypes.byref(rel), ctypes.byref(conv),
+                   L.stream_of(be)))
         return x, SolveInfo(np.array([[it.value]]), np.array([[rel.value]]), np.array([[conv.value]]))
 
 

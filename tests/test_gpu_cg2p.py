@@ -1,6 +1,6 @@
-"""Single-launch persistent FP64 CG of one 2D system (lpm_cg2_persistent) against the oracle's plain CG
-(timing/t2d/cg2plain.py:11-49 restated in oracle/lpm_ref.cg_plain): solutions and iteration counts over launch
-shapes (one row per workgroup, several rows, a ragged last workgroup, 1, 2 and 4 particles per thread, a single
+"""Single-launch persistent FP64 CG of one 2D system (lpm_cg2_persistent, and its pipelined form lpm_cg2_pipelined)
+against the oracle's plain CG (timing/t2d/cg2plain.py:11-49 restated in oracle/lpm_ref.cg_plain): solutions and
+iteration counts over launch shapes (one row per workgroup, several rows, a ragged last workgroup, 1, 2 and 4 particles per thread, a single
 workgroup), the per-DOF roller + pinned-node mask of the plain 2D benchmark, a zero right-hand side, the maxit stop,
 a matrix that is not positive definite and a lattice too wide for one workgroup's LDS."""
 import numpy as np
@@ -28,9 +28,21 @@ def _problem(dims, seed):
     return lat, Kn, Tv, m, b
 
 
-def _solver(dims, Kn, Tv, m, dev, workgroups=0):
+def _solver(dims, Kn, Tv, m, dev, workgroups=0, pipelined=False):
     t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
-    return ops.Cg2Persistent(dims, t(Kn), t(Tv), torch.as_tensor(m, device=dev), workgroups=workgroups)
+    return ops.Cg2Persistent(dims, t(Kn), t(Tv), torch.as_tensor(m, device=dev), workgroups=workgroups,
+                             pipelined=pipelined)
+
+
+def _its_close(a, b, pipelined):
+    """The classic form follows the oracle's iterations to rounding.  The pipelined form's recurrences round
+    differently and drift from the quantities they stand for; its count stays within a few percent on the square
+    and wide lattices, while on a 9-wide strip at rel-res 1e-10 the drift (replacements and restarts) doubled it
+    (733 against 384), so the bound there is 2x."""
+    return (b - 3 <= a <= 2 * b + 3) if pipelined else abs(a - b) <= 2
+
+
+PIPE = pytest.mark.parametrize("pipelined", [False, True], ids=["classic", "pipelined"])
 
 
 def rel(a, b):
@@ -43,59 +55,93 @@ SHAPES = [((16, 12), 0), ((37, 23), 7), ((37, 23), 1), ((120, 9), 2), ((40, 30),
           ((33, 40), 40)]
 
 
+@PIPE
 @pytest.mark.parametrize("dims,workgroups", SHAPES)
-def test_persistent_cg_matches_oracle(dev, dims, workgroups):
-    print("cg2p", dims, workgroups, flush=True)
+def test_persistent_cg_matches_oracle(dev, dims, workgroups, pipelined):
+    """Classic form at rel-res 1e-12: the oracle's iterations (+-2) and solution (1e-9).  Pipelined form at 1e-10
+    (its recurrences lose accuracy near the attainable residual): iterations as _its_close, the true residual
+    within the tolerance (its stopping test confirms it) and the solution within 1e-6 (the displacement bar) of the
+    oracle's 1e-12 solution."""
+    print("cg2p", dims, workgroups, "pipelined" if pipelined else "classic", flush=True)
     lat, Kn, Tv, m, b = _problem(dims, 3 + dims[0])
-    x, info = _solver(dims, Kn, Tv, m, dev, workgroups).solve(torch.as_tensor(b, device=dev), tol=1e-12)
-    xr, it, rr = R.cg_plain(lat, b, Kn, Tv, m, tol=1e-12)
-    assert info.conv.all() and info.relmax <= 1e-12
-    assert abs(info.it - it) <= 2, (info.it, it)
-    assert rel(x.cpu().numpy(), xr) < 1e-9
+    tol = 1e-10 if pipelined else 1e-12
+    x, info = _solver(dims, Kn, Tv, m, dev, workgroups, pipelined).solve(torch.as_tensor(b, device=dev), tol=tol)
+    xr, it, rr = R.cg_plain(lat, b, Kn, Tv, m, tol=tol)
+    print("  its %d (oracle %d)" % (info.it, it), flush=True)
+    assert info.conv.all() and info.relmax <= tol
+    assert _its_close(info.it, it, pipelined), (info.it, it)
+    xe = xr if not pipelined else R.cg_plain(lat, b, Kn, Tv, m, tol=1e-12)[0]
+    assert rel(x.cpu().numpy(), xe) < (1e-6 if pipelined else 1e-9)
+    # the true residual of the returned x against the tolerance the recursive residual met
+    res = R.apply(lat, x.cpu().numpy(), Kn, Tv) * m - b
+    assert np.linalg.norm(res) <= (1.01 * tol if pipelined else 1e-10) * np.linalg.norm(b)
 
 
-def test_persistent_cg_matches_batched_pcg(dev):
+@PIPE
+def test_persistent_cg_matches_batched_pcg(dev, pipelined):
     """Same iterations and solution as the launch-per-iteration plain CG of lpm_pcg_solve."""
+    print("cg2p vs batched", pipelined, flush=True)
     dims = (48, 40)
     lat, Kn, Tv, m, b = _problem(dims, 9)
     t = lambda a: torch.as_tensor(a, device=dev, dtype=torch.float64)
     free = torch.as_tensor(m.astype(np.uint8), device=dev)
-    x1, i1 = _solver(dims, Kn, Tv, m, dev).solve(t(b), tol=1e-10)
+    x1, i1 = _solver(dims, Kn, Tv, m, dev, pipelined=pipelined).solve(t(b), tol=1e-10)
     x2, i2 = ops.pcg(dims, 1.0, t(b), t(Kn), t(Tv), minv=None, free=free, tol=1e-10, maxit=100000)
-    assert abs(i1.it - i2.it) <= 1
-    assert rel(x1.cpu().numpy(), x2.reshape(-1, 2).cpu().numpy()) < 1e-9
+    assert _its_close(i1.it, i2.it, pipelined) if pipelined else abs(i1.it - i2.it) <= 1
+    assert rel(x1.cpu().numpy(), x2.reshape(-1, 2).cpu().numpy()) < (1e-7 if pipelined else 1e-9)
 
 
-def test_persistent_cg_zero_rhs_and_maxit(dev):
+@PIPE
+def test_persistent_cg_zero_rhs_and_maxit(dev, pipelined):
+    print("cg2p zero rhs / maxit", pipelined, flush=True)
     dims = (24, 20)
     lat, Kn, Tv, m, b = _problem(dims, 5)
-    s = _solver(dims, Kn, Tv, m, dev)
+    s = _solver(dims, Kn, Tv, m, dev, pipelined=pipelined)
     x, info = s.solve(torch.zeros((lat.N, 2), device=dev, dtype=torch.float64), tol=1e-10)
     assert info.it == 0 and info.conv.all() and info.relmax == 0.0 and float(x.abs().max()) == 0.0
     # stopped after 5 iterations: the oracle's iterate after 5 iterations, not converged
     x, info = s.solve(torch.as_tensor(b, device=dev), tol=1e-12, maxit=5)
     xr, it, rr = R.cg_plain(lat, b, Kn, Tv, m, tol=1e-12, maxit=5)
     assert info.it == 5 and it == 5 and not info.conv.any()
-    assert rel(x.cpu().numpy(), xr) < 1e-12
+    assert rel(x.cpu().numpy(), xr) < (1e-10 if pipelined else 1e-12)
     assert abs(info.relmax - rr) <= 1e-9 * rr
     # the same instance solves again (barrier counter and flags reset per launch)
     x2, info2 = s.solve(torch.as_tensor(b, device=dev), tol=1e-12, maxit=5)
     assert info2.it == 5 and torch.equal(x, x2)
 
 
-def test_persistent_cg_not_spd_raises(dev):
+@PIPE
+def test_persistent_cg_not_spd_raises(dev, pipelined):
     """K = -K_spd: p.Ap < 0 at the first iteration; the solve stops with FloatingPointError (LPM_E_NOT_SPD)."""
     dims = (20, 16)
     lat, Kn, Tv, m, b = _problem(dims, 7)
     with pytest.raises(FloatingPointError):
-        _solver(dims, -Kn, -Tv, m, dev).solve(torch.as_tensor(b, device=dev), tol=1e-10)
+        _solver(dims, -Kn, -Tv, m, dev, pipelined=pipelined).solve(torch.as_tensor(b, device=dev), tol=1e-10)
 
 
-def test_persistent_cg_too_wide_is_refused(dev):
+@PIPE
+def test_persistent_cg_repeated_solves_same_workspace(dev, pipelined):
+    """Solves of different right-hand sides one after another on one workspace each match their own oracle solve
+    (tagged words of an earlier launch are cleared, so no halo value of an earlier solve is taken)."""
+    print("cg2p repeated", pipelined, flush=True)
+    dims = (40, 36)
+    lat, Kn, Tv, m, b = _problem(dims, 11)
+    s = _solver(dims, Kn, Tv, m, dev, workgroups=12, pipelined=pipelined)
+    rng = np.random.default_rng(12)
+    for k in range(3):
+        bk = b * (k + 1) + rng.standard_normal(b.shape) * m * 1e-4
+        x, info = s.solve(torch.as_tensor(bk, device=dev), tol=1e-11)
+        xr, it, rr = R.cg_plain(lat, bk, Kn, Tv, m, tol=1e-11)
+        assert info.conv.all() and rel(x.cpu().numpy(), xr) < 1e-8, k
+
+
+@PIPE
+def test_persistent_cg_too_wide_is_refused(dev, pipelined):
     """One workgroup cannot hold the LDS rows of a 1000-wide lattice: NotImplementedError, nothing launched."""
     dims = (1000, 4)
     lat = R.Box(dims)
     z = torch.zeros((lat.N, 8), device=dev, dtype=torch.float64)
-    s = ops.Cg2Persistent(dims, z + 1.0, z, torch.ones((lat.N, 2), device=dev, dtype=torch.uint8))
+    s = ops.Cg2Persistent(dims, z + 1.0, z, torch.ones((lat.N, 2), device=dev, dtype=torch.uint8),
+                          pipelined=pipelined)
     with pytest.raises(NotImplementedError):
         s.solve(torch.ones((lat.N, 2), device=dev, dtype=torch.float64))

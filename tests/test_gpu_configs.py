@@ -51,10 +51,10 @@ def rel(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
 
 
-@pytest.mark.parametrize("path", ["batched", "persistent"])
+@pytest.mark.parametrize("path", ["batched", "persistent", "pipelined"])
 def test_c1_plain2d_128_matches_superlu(dev, path):
-    """path: the batched launch-per-iteration CG (lpm_pcg_solve) or the single-launch persistent CG
-    (lpm_cg2_persistent)."""
+    """path: the batched launch-per-iteration CG (lpm_pcg_solve), the single-launch persistent CG
+    (lpm_cg2_persistent) or its pipelined form (lpm_cg2_pipelined)."""
     print("C1 plain2d 128", path, flush=True)
     W = workloads.plain2d(128, device=dev)
     dims = W["dims"]
@@ -62,7 +62,8 @@ def test_c1_plain2d_128_matches_superlu(dev, path):
     if path == "batched":
         x, info = ops.pcg(dims, 1.0, b, W["Kn"], W["Tv"], minv=None, free=W["mask"], tol=1e-8, maxit=200000)
     else:
-        x, info = ops.Cg2Persistent(dims, W["Kn"], W["Tv"], W["mask"]).solve(b, tol=1e-8, maxit=200000)
+        x, info = ops.Cg2Persistent(dims, W["Kn"], W["Tv"], W["mask"],
+                                    pipelined=path == "pipelined").solve(b, tol=1e-8, maxit=200000)
     assert info.conv.all() and info.relmax <= 1e-8
     ug = W["up_np"] + x.reshape(-1, 2).cpu().numpy()
     lat = R.Box(dims)
@@ -84,6 +85,14 @@ def test_c1_plain2d_128_matches_superlu(dev, path):
     ec, ecs = rel(ug, uc), rel(R.stress(lat, ug, Kn, Tv), R.stress(lat, uc, Kn, Tv))
     print("C1: %d its (oracle CG %d); vs the oracle's CG: u %.2e, stress %.2e; vs SuperLU: u %.2e, stress %.2e, "
           "reaction %.2e" % (info.it, it, ec, ecs, eu, es, abs(Fg - Fo) / abs(Fo)), flush=True)
+    if path == "pipelined":
+        # the pipelined recurrences drift from the CG iterates on this lattice (replacements keep its true residual
+        # under the tolerance); at the same residual its error is about 4x the classic form's, so it is held to the
+        # bars a rel-res 1e-8 solve meets against the exact solution, not to the reference's iterate
+        res = R.apply(lat, ug, Kn, Tv) * mask
+        assert np.linalg.norm(res.ravel()) <= 1e-8 * np.linalg.norm(bo) * 1.0001
+        assert eu < 1e-5 and es < 1e-4 and abs(Fg - Fo) <= 1e-6 * abs(Fo) and info.it <= 1.1 * it
+        return
     # against the reference's solver at the same tolerance (the parity bar) ...
     assert ec < 1e-6 and ecs < 1e-5 and abs(info.it - it) <= max(5, it // 100)
     # ... and against the exact solution, which a solve stopped at rel-res 1e-8 reaches to about 1e-6 on this
