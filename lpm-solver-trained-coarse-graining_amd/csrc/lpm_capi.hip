@@ -371,8 +371,17 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
 
   const int check = std::max(1, o->check);
   std::vector<int> host_ist((size_t)nsys * NINT);
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  // the captured check block, destroyed on every way out of this function
+  struct GraphKeep {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    ~GraphKeep() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+    }
+  } gk;
+  hipGraphExec_t& gexec = gk.exec;
+  hipGraph_t& graph = gk.graph;
   bool capturing = false;
   auto body = [&](hipStream_t sq) -> int {
     KuCall c{p, bvec, kn, tv, bc, mask, bm, nullptr, bvec, ap, bvec, partial, ist + I_ACTIVE};
@@ -434,18 +443,10 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
       if (notspd < 0 && host_ist[(size_t)s * NINT + I_NOTSPD]) notspd = s;
     }
     if (notspd >= 0) {
-      if (gexec) {
-        (void)hipGraphExecDestroy(gexec);
-        (void)hipGraphDestroy(graph);
-      }
       return fail(LPM_E_NOT_SPD, "(P)CG: system %d is not positive definite (p.Ap <= 0 or r.z < 0 at iteration %d)",
                   notspd, host_ist[(size_t)notspd * NINT + I_IT]);
     }
     if (!any) break;
-  }
-  if (gexec) {
-    (void)hipGraphExecDestroy(gexec);
-    (void)hipGraphDestroy(graph);
   }
   if (sd.nt > 0) {
     // levels never met: the final iterate, iteration -1
@@ -1253,9 +1254,9 @@ int lpm_serve_step(void* server, int64_t budget, int32_t use_graph, void* stream
     HIPCHK(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(serve_kernel, grid, block, 0, s->cap, s->d_slots, s->max_slots, (long long)budget);
     HIPCHK(hipStreamEndCapture(s->cap, &gr));
+    s->graph_keep[s->ngraph++] = gr;  // destroyed with the server
     hipGraphExec_t ex;
     HIPCHK(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
-    s->graph_keep[s->ngraph++] = gr;
     it = s->graphs.emplace(budget, ex).first;
   }
   HIPCHK(hipGraphLaunch(it->second, st));
