@@ -169,9 +169,10 @@ int lpm_pcg_snapshots(const lpm_grid* g, int32_t nb, int32_t nl, int32_t dtype, 
 /* Plain FP64 CG of one 2D square-lattice system in one cooperative launch (cg2plain.cg_mf,
  * timing/t2d/cg2plain.py:11-49): kn, tv (nx*ny, 8) directed FP64, mask (N, 2) per-DOF bytes (1 = free), b (N, 2);
  * x (N, 2) receives the solution from a zero start; stops when ||r|| <= tol ||b|| (0 iterations when r0 = mask b
- * already does) or after maxit iterations (conv = 0).  workgroups: 0 = automatic (about 48), else about that many (at
+ * already does) or after maxit iterations (conv = 0).  workgroups: 0 = automatic (about 64), else about that many (at
  * most 64), each owning ceil(ny / workgroups) rows; LPM_E_UNSUPPORTED when those rows do not fit one workgroup
- * (1024 particles, nx <= 128, 8 nx (6 rows + 32) + 768 bytes of LDS) or the workgroups cannot all be resident.  LPM_E_NOT_SPD when p.Ap <= 0, LPM_E_NONFINITE on NaN / Inf,
+ * (1024 particles, nx <= 128, 8 ((6 rows + 16) nx + 100) bytes of LDS) or the workgroups cannot all be resident.
+ * LPM_E_NOT_SPD when p.Ap <= 0, LPM_E_NONFINITE on NaN / Inf,
  * LPM_E_HIP when a grid-barrier wait times out.  work: lpm_cg2_persistent_workspace(nx, ny) bytes.  Returns after
  * the solve (synchronises the stream). */
 size_t lpm_cg2_persistent_workspace(int32_t nx, int32_t ny);
