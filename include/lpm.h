@@ -230,6 +230,10 @@ int lpm_serve_admit(void* server, int32_t nx, int32_t ny, double h, const float*
                     const uint8_t* mask, const double* up, double tol, double inner_tol, int32_t maxit, void* stream,
                     int32_t* slot);
 int lpm_serve_step(void* server, int64_t budget, int32_t use_graph, void* stream);
+/* Tuning switch (process-wide, no counterpart in the reference): 1 (default) runs the inner iterations' shell sums and
+ * product as one banded pass through LDS, 0 as two passes through device memory; same operator, the partial sums of
+ * p.Ap follow a different order. */
+void lpm_serve_set_fused(int32_t fused);
 int lpm_serve_poll(void* server, int32_t* state, int32_t* iters, int32_t* outer, double* relres, void* stream);
 int lpm_serve_result(void* server, int32_t slot, double* u, void* stream);
 int lpm_serve_release(void* server, int32_t slot);

@@ -67,6 +67,7 @@ SIGS = {
     "lpm_serve_admit": (I32, [P, I32, I32, ctypes.c_double, P, P, P, P, ctypes.c_double, ctypes.c_double, I32, P,
                               ctypes.POINTER(I32)]),
     "lpm_serve_step": (I32, [P, ctypes.c_int64, I32, P]),
+    "lpm_serve_set_fused": (None, [I32]),
     "lpm_serve_poll": (I32, [P, P, P, P, P, P]),
     "lpm_serve_result": (I32, [P, I32, P, P]),
     "lpm_serve_release": (I32, [P, I32]),

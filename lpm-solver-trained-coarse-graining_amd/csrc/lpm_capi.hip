@@ -101,6 +101,7 @@ inline int vec_blocks(int64_t N) { return (int)std::min<int64_t>(1024, std::max<
 // K u (plain and masked-dot forms) to the lean kernel.  It is opt-in: at 256^3 it is faster alone (443 vs 477 us)
 // but slower inside the PCG iteration (471 vs 444 us), where the vector kernels between launches evict the caches.
 int g_tile_wide = 0, g_zc = 0, g_lean = 0;
+int g_serve_fused = 1;  // serving: fused band product of the inner iterations (0: two passes)
 
 // compute units of the current device (cached per device; 256 when the runtime cannot tell)
 int cu_count() {
@@ -1237,27 +1238,32 @@ int lpm_serve_admit(void* server, int32_t nx, int32_t ny, double h, const float*
   return LPM_OK;
 }
 
+void lpm_serve_set_fused(int32_t fused) { g_serve_fused = fused != 0; }
+
 int lpm_serve_step(void* server, int64_t budget, int32_t use_graph, void* stream) {
   Server* s = (Server*)server;
   if (!s || budget < 1) return fail(LPM_E_BADARG, "lpm_serve_step: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((unsigned)s->max_slots), block((unsigned)s->threads);
   if (!use_graph) {
-    hipLaunchKernelGGL(serve_kernel, grid, block, 0, st, s->d_slots, s->max_slots, (long long)budget);
+    hipLaunchKernelGGL(serve_kernel, grid, block, 0, st, s->d_slots, s->max_slots, (long long)budget,
+                       g_serve_fused);
     LAUNCHCHK();
     return LPM_OK;
   }
-  auto it = s->graphs.find(budget);
+  const long long key = 2 * (long long)budget + g_serve_fused;
+  auto it = s->graphs.find(key);
   if (it == s->graphs.end()) {
     if (s->ngraph >= 64) return fail(LPM_E_BADARG, "lpm_serve_step: too many distinct budgets");
     hipGraph_t gr;
     HIPCHK(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(serve_kernel, grid, block, 0, s->cap, s->d_slots, s->max_slots, (long long)budget);
+    hipLaunchKernelGGL(serve_kernel, grid, block, 0, s->cap, s->d_slots, s->max_slots, (long long)budget,
+                       g_serve_fused);
     HIPCHK(hipStreamEndCapture(s->cap, &gr));
     s->graph_keep[s->ngraph++] = gr;  // destroyed with the server
     hipGraphExec_t ex;
     HIPCHK(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
-    it = s->graphs.emplace(budget, ex).first;
+    it = s->graphs.emplace(key, ex).first;
   }
   HIPCHK(hipGraphLaunch(it->second, st));
   return LPM_OK;

@@ -6,7 +6,8 @@
 //   INIT   b = -mask * K u_p (FP64), ||b||, 2x2 block-Jacobi inverse M (free DOFs only);
 //   OUTER  r64 = mask * (b - K x) in FP64; stop when ||r64|| <= tol ||b||; else start an FP32 inner PCG on
 //          K d = r64 with the relative tolerance min(1/2, max(inner_tol, tol / (2 rel)));
-//   INNER  FP32 PCG iterations; on inner convergence x += d (FP64) and back to OUTER;
+//   INNER  FP32 PCG iterations (shell sums and product in one banded pass through LDS); on inner convergence
+//          x += d (FP64) and back to OUTER;
 //   DONE   u = u_p + x written over u_p.
 // The workgroup runs until its slot is done or the launch's work budget (site-iterations) is spent, then leaves
 // its state in the descriptor; the next launch resumes it.  All control values are workgroup-uniform (computed
@@ -49,6 +50,13 @@ struct SlotDesc {
 namespace srv {
 
 constexpr int NT_MAX = 1024;  // threads per slot workgroup: 256, 512 or 1024 (blockDim.x), fixed per server
+// Fused shell-sum + product pass of the inner iteration: bands of B rows with B + 2 <= BAND_SPT blockDim.x / nx, so
+// every thread owns at most BAND_SPT shell sites of a band; LDS holds p of the band rows + 2 each side and the shell
+// sums of the band rows + 1 each side.  Lattices wider than BAND_NX_MAX take the two-pass path.
+constexpr int BAND_SPT = 3;
+constexpr int BAND_NX_MAX = BAND_SPT * NT_MAX / 3;
+constexpr int BAND_W_MAX = BAND_SPT * NT_MAX;                // shell sites of a band
+constexpr int BAND_P_MAX = BAND_W_MAX + 2 * BAND_NX_MAX;     // p sites of a band
 
 // directed slot geometry of the 2D square lattice
 __device__ __forceinline__ int nbr2(int x, int y, int nx, int ny, int k) {
@@ -251,8 +259,95 @@ __device__ void phase_outer(SlotDesc& s, double* red) {
   s.state = SLOT_INNER;
 }
 
-// one inner PCG iteration (FP32)
-__device__ void inner_iteration(SlotDesc& s, float* red) {
+// ap = mask (K p) and the partial p.ap in one pass over bands of B rows (the operator of shell_pass + ku_site, same
+// arithmetic): p of rows y0 - 2 .. y0 + nb + 1 into LDS, shell sums of rows y0 - 1 .. y0 + nb into LDS by thread t
+// for shell sites t + j blockDim.x, which keep their Tv in registers, then the product of the band rows by the
+// threads that formed their shell sums.  Every array of the slot is read once per iteration except the halo rows.
+__device__ float band_product(const SlotDesc& s, const float* __restrict__ pv, float* __restrict__ apv,
+                              Vec<float, 2>* pL, Vec<float, 4>* wL, int B) {
+  const int nx = s.nx, ny = s.ny, nt = blockDim.x;
+  const float* __restrict__ knp = s.kn;
+  const float* __restrict__ tvp = s.tv;
+  const uint8_t* __restrict__ mk = s.mask;
+  float pap = 0.f;
+  for (int y0 = 0; y0 < ny; y0 += B) {
+    const int nb = min(B, ny - y0);
+    for (int t = threadIdx.x; t < (nb + 4) * nx; t += nt) {
+      const int r = t / nx, gy = y0 - 2 + r;
+      Vec<float, 2> v;
+      v.v[0] = v.v[1] = 0.f;
+      if (gy >= 0 && gy < ny) v = vld<float, 2>(pv + 2 * ((size_t)gy * nx + (t - r * nx)));
+      pL[t] = v;
+    }
+    __syncthreads();
+    float tvr[BAND_SPT][8];
+#pragma unroll
+    for (int j = 0; j < BAND_SPT; ++j) {
+      const int i = threadIdx.x + j * nt;  // shell site: row i / nx of rows y0 - 1 .., LDS p row i / nx + 1
+      const int r = i / nx, x = i - r * nx, gy = y0 - 1 + r;
+      if (i < (nb + 2) * nx && gy >= 0 && gy < ny) {
+        const int p = gy * nx + x;
+        const Vec<float, 4> t0 = vld<float, 4>(tvp + 8 * (size_t)p), t1 = vld<float, 4>(tvp + 8 * (size_t)p + 4);
+        tvr[j][0] = t0.v[0], tvr[j][1] = t0.v[1], tvr[j][2] = t0.v[2], tvr[j][3] = t0.v[3];
+        tvr[j][4] = t1.v[0], tvr[j][5] = t1.v[1], tvr[j][6] = t1.v[2], tvr[j][7] = t1.v[3];
+        const Vec<float, 2> ui = pL[i + nx];
+        Vec<float, 4> a;
+        a.v[0] = a.v[1] = a.v[2] = a.v[3] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (nbr2(x, gy, nx, ny, k) < 0) continue;
+          const Vec<float, 2> uq = pL[i + nx + off2(k, 1) * nx + off2(k, 0)];
+          const float dl = nd2<float>(k, 0) * (ui.v[0] - uq.v[0]) + nd2<float>(k, 1) * (ui.v[1] - uq.v[1]);
+          a.v[shell2(k)] += dl;
+          a.v[2 + shell2(k)] += tvr[j][k] * dl;
+        }
+        wL[i] = a;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < BAND_SPT; ++j) {
+      const int i = threadIdx.x + j * nt;
+      const int r = i / nx, x = i - r * nx, gy = y0 - 1 + r;
+      if (i < (nb + 2) * nx && r >= 1 && r <= nb) {
+        const int p = gy * nx + x;
+        const Vec<float, 4> k0 = vld<float, 4>(knp + 8 * (size_t)p), k1 = vld<float, 4>(knp + 8 * (size_t)p + 4);
+        const float knk[8] = {k0.v[0], k0.v[1], k0.v[2], k0.v[3], k1.v[0], k1.v[1], k1.v[2], k1.v[3]};
+        const Vec<float, 2> ui = pL[i + nx];
+        const Vec<float, 4> wi = wL[i];
+        float f0 = 0.f, f1 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (nbr2(x, gy, nx, ny, k) < 0) continue;
+          const int c = shell2(k), dq = off2(k, 1) * nx + off2(k, 0);
+          const Vec<float, 2> uq = pL[i + nx + dq];
+          const Vec<float, 4> wq = wL[i + dq];
+          const float dl = nd2<float>(k, 0) * (ui.v[0] - uq.v[0]) + nd2<float>(k, 1) * (ui.v[1] - uq.v[1]);
+          const float F = 2.f * knk[k] * dl + 0.5f * (wi.v[2 + c] + wq.v[2 + c]) + 0.5f * tvr[j][k] * (wi.v[c] + wq.v[c]);
+          f0 += F * nd2<float>(k, 0);
+          f1 += F * nd2<float>(k, 1);
+        }
+        Vec<float, 2> o;
+        o.v[0] = mk[2 * p] ? f0 : 0.f;
+        o.v[1] = mk[2 * p + 1] ? f1 : 0.f;
+        *(Vec<float, 2>*)(apv + 2 * (size_t)p) = o;
+        pap += ui.v[0] * o.v[0] + ui.v[1] * o.v[1];
+      }
+    }
+    __syncthreads();
+  }
+  return pap;
+}
+
+// rows per band of the fused pass for this slot and workgroup size (0: the lattice is too wide, two passes)
+__device__ __forceinline__ int band_rows(int nx) {
+  if (nx > BAND_NX_MAX) return 0;
+  const int b = BAND_SPT * (int)blockDim.x / nx - 2;
+  return b >= 1 ? b : 0;
+}
+
+// one inner PCG iteration (FP32); fused = 0 forces the two-pass product
+__device__ void inner_iteration(SlotDesc& s, float* red, Vec<float, 2>* pL, Vec<float, 4>* wL, int fused) {
   const int N = s.nx * s.ny;
   float* __restrict__ pv = s.p;
   float* __restrict__ apv = s.ap;
@@ -261,18 +356,23 @@ __device__ void inner_iteration(SlotDesc& s, float* red) {
   float* __restrict__ dv = s.d;
   const float* __restrict__ mi = s.minv;
   const uint8_t* __restrict__ mk = s.mask;
-  shell_pass<float>(s, pv, s.w32);
-  __syncthreads();
   float pap = 0.f, unused = 0.f;
-  for (int p = threadIdx.x; p < N; p += blockDim.x) {
-    float f0, f1;
-    ku_site<float>(s, pv, s.w32, p, f0, f1);
-    Vec<float, 2> o;
-    o.v[0] = mk[2 * p] ? f0 : 0.f;
-    o.v[1] = mk[2 * p + 1] ? f1 : 0.f;
-    *(Vec<float, 2>*)(apv + 2 * p) = o;
-    const Vec<float, 2> pp = vld<float, 2>(pv + 2 * p);
-    pap += pp.v[0] * o.v[0] + pp.v[1] * o.v[1];
+  const int B = fused ? band_rows(s.nx) : 0;
+  if (B > 0) {
+    pap = band_product(s, pv, apv, pL, wL, B);
+  } else {
+    shell_pass<float>(s, pv, s.w32);
+    __syncthreads();
+    for (int p = threadIdx.x; p < N; p += blockDim.x) {
+      float f0, f1;
+      ku_site<float>(s, pv, s.w32, p, f0, f1);
+      Vec<float, 2> o;
+      o.v[0] = mk[2 * p] ? f0 : 0.f;
+      o.v[1] = mk[2 * p + 1] ? f1 : 0.f;
+      *(Vec<float, 2>*)(apv + 2 * p) = o;
+      const Vec<float, 2> pp = vld<float, 2>(pv + 2 * p);
+      pap += pp.v[0] * o.v[0] + pp.v[1] * o.v[1];
+    }
   }
   wg_sum2(pap, unused, red);
   if (!(pap > 0.f)) {
@@ -326,9 +426,11 @@ __device__ void inner_iteration(SlotDesc& s, float* red) {
 // site-iterations.  Slot descriptors live in
 // device memory; the workgroup keeps a private copy (identical in all threads) and thread 0 writes it back.
 __global__ __launch_bounds__(srv::NT_MAX) void serve_kernel(SlotDesc* __restrict__ slots, int nslots,
-                                                             long long budget) {
+                                                             long long budget, int fused) {
   __shared__ double red64[2 * srv::NT_MAX / 64];
   __shared__ float red32[2 * srv::NT_MAX / 64];
+  __shared__ srv::Vec<float, 2> band_p[srv::BAND_P_MAX];
+  __shared__ srv::Vec<float, 4> band_w[srv::BAND_W_MAX];
   const int id = blockIdx.x;
   if (id >= nslots) return;
   SlotDesc s = slots[id];
@@ -343,7 +445,7 @@ __global__ __launch_bounds__(srv::NT_MAX) void serve_kernel(SlotDesc* __restrict
       srv::phase_outer(s, red64);
       work += 3 * N;
     } else if (s.state == SLOT_INNER) {
-      srv::inner_iteration(s, red32);
+      srv::inner_iteration(s, red32, band_p, band_w, fused);
       work += N;
     } else {
       break;

@@ -38,7 +38,7 @@ def oracle_u(r):
 
 def solve_all(dev, reqs, **kw):
     srv = serve.Server(device=dev, **{k: v for k, v in kw.items() if k in ("max_slots", "arena_bytes", "budget",
-                                                                           "use_graph")})
+                                                                           "use_graph", "threads")})
     arr = kw.get("arrivals")
     stats = serve.run_stream(srv, reqs, arr)
     srv.close()
@@ -96,3 +96,25 @@ def test_zero_load_request(dev):
     r.up = torch.zeros_like(r.up)
     stats, us = solve_all(dev, [r], max_slots=2)
     assert stats["failed"] == 0 and float(us[0].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("threads", [256, 1024])
+@pytest.mark.parametrize("fused", [1, 0], ids=["fused", "two_pass"])
+def test_banded_and_two_pass_products(dev, threads, fused):
+    """The inner iterations' product as one banded pass through LDS (bands of 3 threads / nx - 2 rows: 72-wide
+    requests take 9 bands at 256 threads and 2 at 1024, 32-wide ones 2 and 1) and as two passes through device
+    memory: each against SuperLU, alone and in a mix."""
+    import _lpmlib as L
+    print("serve threads", threads, "fused", fused, flush=True)
+    L.lib().lpm_serve_set_fused(fused)
+    try:
+        reqs = [serve.coarse_request(144, 2, 0.3, 4.0, 500 + i, dev) for i in range(2)] + \
+               [serve.coarse_request(96, 3, 0.25, 4.0, 600 + i, dev) for i in range(2)]
+        stats, mixed = solve_all(dev, reqs, max_slots=3, budget=20000, threads=threads)
+        assert stats["failed"] == 0 and stats["max_relres"] <= 1e-8
+        for r, u in zip(reqs, mixed):
+            assert rel(u, oracle_u(r)) < 1e-6
+        _, alone = solve_all(dev, reqs[:1], max_slots=1, budget=10 ** 9, use_graph=False, threads=threads)
+        assert torch.equal(alone[0], mixed[0])
+    finally:
+        L.lib().lpm_serve_set_fused(1)
