@@ -411,7 +411,7 @@ def bench_plain2d(a):
           "plain 2D LPM 128^2 (32768 DOF), disks phi 0.30 contrast 10, unpreconditioned CG to rel-res %g" % a.tol,
           {"latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
            "solve": {"cg_iters": info.it, "relres": float(info.relmax), "phase_fraction": P["phase_fraction"],
-                     "path": "lpm_cg2_persistent (one cooperative launch, 64 workgroups of 2 rows)"},
+                     "path": "lpm_cg2_persistent (one cooperative launch, 64 workgroups of 2 rows, 512 threads)"},
            "pipelined_cg": {"solves_per_s": len(lat_p) / sum(lat_p), "iters": info_p.it,
                             "relres_true": float(info_p.relmax), "max_rel_diff": diff_p,
                             "note": "lpm_cg2_pipelined (opt-in): one grid-wide reduction per iteration overlapped with "

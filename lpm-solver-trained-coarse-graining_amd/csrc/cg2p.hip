@@ -327,9 +327,12 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
     }
   }
   // the sites of rows y0 - 1 and y1 whose shell sums this workgroup forms: thread t < 2 nx, row h = t / nx
-  const int ht = tid < 2 * nx ? tid : 0;
+  // (threads after the own particles when the workgroup has enough of them, so both kinds run side by side)
+  const int hoff = NT >= R * nx + 2 * nx ? R * nx : 0;
+  const int hid = tid - hoff;
+  const int ht = hid >= 0 && hid < 2 * nx ? hid : 0;
   const int hrow = ht / nx, hx = ht % nx, hls = hrow == 0 ? 0 : S + 1, hgy = y0 - 1 + hls;
-  const bool hown = tid < 2 * nx && hgy >= 0 && hgy < ny;
+  const bool hown = hid >= 0 && hid < 2 * nx && hgy >= 0 && hgy < ny;
   const unsigned hvb = hown ? slot_bits(hx, hgy, nx, ny) : 0u;
   double tvh[8];  // Tv of that site
 #pragma unroll
@@ -808,10 +811,12 @@ struct CgpWork {
   }
 };
 
-// rows per workgroup (about `want` workgroups) and the kernel: 256 threads with 1, 2 or 4 particles each (larger
-// workgroups run out of registers: the vectors, Kn and Tv of the own particles live there); false when the rows of
-// one workgroup do not fit its threads (the two side rows need 2 nx <= 256) or its LDS.  The pipelined kernel
-// defaults to the most workgroups (its reduction is hidden behind the product, so smaller bands pay).
+// rows per workgroup (about `want` workgroups) and the kernel: 256 threads with 1, 2 or 4 particles each, or for the
+// classic form with up to 256 own particles 512 threads, the side-row sites on threads of their own (9.06 against
+// 9.45 us per iteration at 128^2; larger workgroups run out of registers: the vectors, Kn and Tv of the own
+// particles live there); false when the rows of one workgroup do not fit its threads (the two side rows need
+// 2 nx <= 256) or its LDS.  The pipelined kernel defaults to the most workgroups too (its reduction is hidden behind
+// the product, so smaller bands pay).
 using CgpKernel = void (*)(CgpArgs);
 bool cgp_shape(bool pipelined, int nx, int ny, int ncu, size_t max_lds, int want, int& rows, int& nwg, int& nt,
                CgpKernel& fn) {
@@ -820,6 +825,11 @@ bool cgp_shape(bool pipelined, int nx, int ny, int ncu, size_t max_lds, int want
   nwg = (ny + rows - 1) / rows;
   const int per = rows * nx;
   nt = 256;
+  if (!pipelined && per <= 256 && per + 2 * nx <= 512) {
+    nt = 512;
+    fn = cg2p_kernel<512, 1>;
+    return sizeof(double) * (size_t)CgpLds(nx, rows).total <= max_lds;
+  }
   if (per <= 256)
     fn = pipelined ? cg2pp_kernel<256, 1> : cg2p_kernel<256, 1>;
   else if (per <= 512)
