@@ -41,7 +41,8 @@ struct CgpArgs {
   double* x;            // (N, 2) out
   double* pbuf;         // (N, 2) p_0 (exchanged once)
   uint64_t* rwords;     // (2, N, 2, 2) r by iteration parity, each double as two words (epoch << 32) | half
-  uint64_t* slots;      // (2, kCgpMaxWg, 4) published partial sums: (epoch << 32) | half of a double (zero at launch)
+  uint64_t* slots;      // (2, 2, 2, kCgpMaxWg) published partial sums by epoch parity, value and half: (epoch << 32)
+                        // | half of the double (zero at launch)
   int* abort_flag;      // set by a workgroup whose exchange wait timed out (zero at launch)
   double tol;
   int maxit;
@@ -73,17 +74,29 @@ __device__ __forceinline__ double xld(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid-wide sums of (v0, v1) that also act as a barrier, in two halves.  xpub: the workgroup's sums (fixed order)
-// are published by thread 0 as four tagged words after every thread's earlier stores have completed (WAIT:
-// s_waitcnt).  xsum: each wave polls the words of all workgroups, lane j those of workgroup j, until every tag
-// equals `epoch`, and sums the payloads over the lanes in a fixed order, so every wave of every workgroup obtains
-// the same bits.  Work placed between the halves overlaps the exchange.  xsum returns false (in every thread of the
-// workgroup) when a wait timed out or another workgroup gave up.
-template <int NT, bool WAIT>
+// a double as two tagged words and back
+__device__ __forceinline__ void put_tagged(uint64_t* w, unsigned epoch, double v) {
+  const uint64_t tag = (uint64_t)epoch << 32, b = (uint64_t)__double_as_longlong(v);
+  __hip_atomic_store(w, tag | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(w + 1, tag | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double from_tagged(uint64_t hi, uint64_t lo) {
+  return __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+}
+
+// Grid-wide sums of NV values (v0, v1) that also act as a barrier, in two halves.  xpub: the workgroup's sums
+// (fixed order) are published by thread 0, each as two words (epoch << 32) | half, after every thread's earlier
+// stores have completed (WAIT: s_waitcnt).  xsum: the first wave polls the words of all workgroups, lane j those of
+// workgroup j, until every tag equals `epoch`, and sums the payloads over the lanes in a fixed order, so every
+// workgroup obtains the same bits.  The words of one half of one value lie side by side for all workgroups, so one
+// wave load reads them (a poll round requests 2 NV runs of 512 bytes, not a line per workgroup and word).  Work
+// placed between the halves overlaps the exchange.  xsum returns false (in every thread of the workgroup) when a
+// wait timed out or another workgroup gave up.
+template <int NT, int NV, bool WAIT>
 __device__ void xpub(const CgpArgs& a, unsigned epoch, double v0, double v1, double* red) {
   constexpr int NW = NT / 64;
   v0 = wave_sum(v0);
-  v1 = wave_sum(v1);
+  if constexpr (NV > 1) v1 = wave_sum(v1);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) {
     red[2 * wv] = v0;
@@ -91,20 +104,21 @@ __device__ void xpub(const CgpArgs& a, unsigned epoch, double v0, double v1, dou
   }
   if constexpr (WAIT) __builtin_amdgcn_s_waitcnt(0);  // this wave's stores of untagged exchanged data completed
   __syncthreads();
-  uint64_t* slot = a.slots + (size_t)(epoch & 1u) * kCgpMaxWg * 4;
+  uint64_t* slot = a.slots + (size_t)(epoch & 1u) * 4 * kCgpMaxWg + blockIdx.x;
   if (threadIdx.x == 0) {
-    double t0 = 0.0, t1 = 0.0;
+    double t[2] = {0.0, 0.0};
     for (int i = 0; i < NW; ++i) {
-      t0 += red[2 * i];
-      t1 += red[2 * i + 1];
+      t[0] += red[2 * i];
+      t[1] += red[2 * i + 1];
     }
     const uint64_t tag = (uint64_t)epoch << 32;
-    const uint64_t b0 = (uint64_t)__double_as_longlong(t0), b1 = (uint64_t)__double_as_longlong(t1);
-    uint64_t* mine = slot + (size_t)blockIdx.x * 4;
-    __hip_atomic_store(mine + 0, tag | (b0 >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(mine + 1, tag | (b0 & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(mine + 2, tag | (b1 >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(mine + 3, tag | (b1 & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const uint64_t b = (uint64_t)__double_as_longlong(t[k]);
+      __hip_atomic_store(slot + (2 * k) * kCgpMaxWg, tag | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(slot + (2 * k + 1) * kCgpMaxWg, tag | (b & 0xffffffffull), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -118,7 +132,10 @@ __device__ bool wait_words(const CgpArgs& a, double* red, Poll&& poll) {
   bool ok = true;
   for (long spins = 0;; ++spins) {
     if (__all(poll())) break;
-    if (spins > kSpinLimit || __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+    // the abort flag is another round trip to memory: read it only every 64th poll, so a poll round waits for
+    // one load latency, not two
+    if ((spins & 63) == 63 &&
+        (spins > kSpinLimit || __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
       ok = false;
       break;
     }
@@ -134,10 +151,10 @@ __device__ bool wait_words(const CgpArgs& a, double* red, Poll&& poll) {
   return all_ok;
 }
 
-template <int NT, typename Extra>
+template <int NT, int NV, typename Extra>
 __device__ bool xsum(const CgpArgs& a, unsigned epoch, double* red, double& s0, double& s1, Extra&& extra_poll) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const uint64_t* slot = a.slots + (size_t)(epoch & 1u) * kCgpMaxWg * 4;
+  const uint64_t* slot = a.slots + (size_t)(epoch & 1u) * 4 * kCgpMaxWg;
   uint64_t w[4] = {0, 0, 0, 0};
   // the first wave polls the words (lane j those of workgroup j) and passes the sums on through LDS: one wave's
   // polling per workgroup keeps the traffic on the shared words low
@@ -146,21 +163,18 @@ __device__ bool xsum(const CgpArgs& a, unsigned epoch, double* red, double& s0, 
     bool have = extra_poll();
     if (mine_lane) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        w[k] = __hip_atomic_load(slot + (size_t)lane * 4 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int k = 0; k < 2 * NV; ++k) {
+        w[k] = __hip_atomic_load(slot + k * kCgpMaxWg + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         have = have && (unsigned)(w[k] >> 32) == epoch;
       }
     }
     return have;
   });
   if (wv == 0) {
-    double d0 = 0.0, d1 = 0.0;
-    if (mine_lane) {
-      d0 = __longlong_as_double((long long)(((w[0] & 0xffffffffull) << 32) | (w[1] & 0xffffffffull)));
-      d1 = __longlong_as_double((long long)(((w[2] & 0xffffffffull) << 32) | (w[3] & 0xffffffffull)));
-    }
+    double d0 = mine_lane ? from_tagged(w[0], w[1]) : 0.0;
+    double d1 = mine_lane && NV > 1 ? from_tagged(w[2], w[3]) : 0.0;
     d0 = wave_sum(d0);
-    d1 = wave_sum(d1);
+    if constexpr (NV > 1) d1 = wave_sum(d1);
     if (lane == 0) {
       red[CgpLds::kRed - 2] = d0;
       red[CgpLds::kRed - 1] = d1;
@@ -172,24 +186,14 @@ __device__ bool xsum(const CgpArgs& a, unsigned epoch, double* red, double& s0, 
   return ok;
 }
 
-template <int NT, bool WAIT, typename Extra>
+template <int NT, int NV, bool WAIT, typename Extra>
 __device__ bool exchange(const CgpArgs& a, unsigned epoch, double v0, double v1, double* red, double& s0,
                          double& s1, Extra&& extra_poll) {
-  xpub<NT, WAIT>(a, epoch, v0, v1, red);
-  return xsum<NT>(a, epoch, red, s0, s1, extra_poll);
+  xpub<NT, NV, WAIT>(a, epoch, v0, v1, red);
+  return xsum<NT, NV>(a, epoch, red, s0, s1, extra_poll);
 }
 
 __device__ __forceinline__ bool no_extra() { return true; }
-
-// a double as two tagged words and back
-__device__ __forceinline__ void put_tagged(uint64_t* w, unsigned epoch, double v) {
-  const uint64_t tag = (uint64_t)epoch << 32, b = (uint64_t)__double_as_longlong(v);
-  __hip_atomic_store(w, tag | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(w + 1, tag | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double from_tagged(uint64_t hi, uint64_t lo) {
-  return __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
-}
 
 // one site's slots that stay in the box, as bits
 __device__ __forceinline__ unsigned slot_bits(int gx, int gy, int nx, int ny) {
@@ -340,7 +344,7 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
 
   double s0, s1;
   ++epoch;
-  bool ok = exchange<NT, true>(a, epoch, bb, rr, red + CgpLds::kRed * (epoch & 1u), s0, s1, [] { return no_extra(); });
+  bool ok = exchange<NT, 2, true>(a, epoch, bb, rr, red + CgpLds::kRed * (epoch & 1u), s0, s1, [] { return no_extra(); });
   const double bn = sqrt(s0);
   rr = s1;
   int it = 0, status = ok ? CGP_CONV : CGP_TIMEOUT;
@@ -370,7 +374,7 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
     for (int j = 0; j < PPT; ++j)
       if (tid + j * NT < nown) pap += pr[j][0] * ap[j][0] + pr[j][1] * ap[j][1];
     ++epoch;
-    if (!exchange<NT, false>(a, epoch, pap, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1,
+    if (!exchange<NT, 1, false>(a, epoch, pap, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1,
                              [] { return no_extra(); })) {
       status = CGP_TIMEOUT;
       break;
@@ -418,7 +422,7 @@ __global__ __launch_bounds__(NT) void cg2p_kernel(CgpArgs a) {
       return have;
     };
     ++epoch;
-    if (!exchange<NT, false>(a, epoch, rn, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1, halo_poll)) {
+    if (!exchange<NT, 1, false>(a, epoch, rn, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1, halo_poll)) {
       status = CGP_TIMEOUT;
       break;
     }
@@ -607,7 +611,7 @@ __global__ __launch_bounds__(NT) void cg2pp_kernel(CgpArgs a) {
   // |b|^2 and |r_0|^2
   double s0, s1;
   ++epoch;
-  const bool ok = exchange<NT, false>(a, epoch, bb, rr, red + CgpLds::kRed * (epoch & 1u), s0, s1,
+  const bool ok = exchange<NT, 2, false>(a, epoch, bb, rr, red + CgpLds::kRed * (epoch & 1u), s0, s1,
                                       [] { return no_extra(); });
   const double bn = sqrt(s0);
   int it = 0, status = ok ? CGP_CONV : CGP_TIMEOUT;
@@ -636,7 +640,7 @@ __global__ __launch_bounds__(NT) void cg2pp_kernel(CgpArgs a) {
         }
       ++epoch;
       red_e = red + CgpLds::kRed * (epoch & 1u);
-      xpub<NT, false>(a, epoch, pg, pd, red_e);
+      xpub<NT, 2, false>(a, epoch, pg, pd, red_e);
     }
     double v[PPT][2], out[PPT][2];
 #pragma unroll
@@ -690,7 +694,7 @@ __global__ __launch_bounds__(NT) void cg2pp_kernel(CgpArgs a) {
       if (mode == M_CONFIRM) {
         // the true residual decides
         ++epoch;
-        if (!exchange<NT, false>(a, epoch, tr, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1,
+        if (!exchange<NT, 1, false>(a, epoch, tr, 0.0, red + CgpLds::kRed * (epoch & 1u), s0, s1,
                                  [] { return no_extra(); })) {
           status = CGP_TIMEOUT;
           break;
@@ -705,7 +709,7 @@ __global__ __launch_bounds__(NT) void cg2pp_kernel(CgpArgs a) {
       continue;
     }
     // M_STEP: q = out
-    if (!xsum<NT>(a, epoch, red_e, s0, s1, [] { return no_extra(); })) {
+    if (!xsum<NT, 2>(a, epoch, red_e, s0, s1, [] { return no_extra(); })) {
       status = CGP_TIMEOUT;
       break;
     }
@@ -784,7 +788,7 @@ __global__ void cgp_zero_kernel(uint64_t* slots, uint64_t* rwords, int64_t nrw, 
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = i0; i < nrw; i += stride) rwords[i] = 0;
   if (blockIdx.x == 0) {
-    for (int i = threadIdx.x; i < 2 * kCgpMaxWg * 4; i += blockDim.x) slots[i] = 0;
+    for (int i = threadIdx.x; i < 2 * 4 * kCgpMaxWg; i += blockDim.x) slots[i] = 0;
     if (threadIdx.x == 0) *abort_flag = 0;
   }
 }
@@ -800,7 +804,7 @@ struct CgpWork {
     rwords = o;
     o += al(sizeof(uint64_t) * 8 * N);
     slots = o;
-    o += al(sizeof(uint64_t) * 2 * kCgpMaxWg * 4);
+    o += al(sizeof(uint64_t) * 2 * 4 * kCgpMaxWg);
     flag = o;
     o += 256;
     out = o;
