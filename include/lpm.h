@@ -80,6 +80,11 @@ void lpm_set_ku_tiling(int32_t wide, int32_t zc);
  * own-bond terms summed as the shell sums form, 64-wide tiles); 0 (default): the general kernel.  Process-wide; no
  * counterpart in the reference (tuning / comparison knob). */
 void lpm_set_ku_kernel(int32_t lean);
+/* box = 1: every 3D K u runs the box kernel (one block per 16 x 4 x 4 sites, no z march); 0: never; -1 (default):
+ * automatic, the box kernel where the marching kernel's launch would have fewer blocks than the device has CUs
+ * (small lattices with few systems) and no tiling override is set.  Process-wide; no counterpart in the reference
+ * (tuning knob). */
+void lpm_set_ku_box(int32_t box);
 void lpm_profile_ku_read(double* total_ms, int64_t* launches);
 
 /* Bond words from solid flags (nbatch, N) bytes, nonzero = solid: bit k of p set iff slot k stays in the box and

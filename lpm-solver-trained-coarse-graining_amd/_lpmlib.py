@@ -68,6 +68,7 @@ SIGS = {
                               ctypes.POINTER(I32)]),
     "lpm_serve_step": (I32, [P, ctypes.c_int64, I32, P]),
     "lpm_serve_set_fused": (None, [I32]),
+    "lpm_set_ku_box": (None, [I32]),
     "lpm_serve_poll": (I32, [P, P, P, P, P, P]),
     "lpm_serve_result": (I32, [P, I32, P, P]),
     "lpm_serve_release": (I32, [P, I32]),

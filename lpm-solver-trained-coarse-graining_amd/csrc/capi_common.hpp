@@ -5,6 +5,7 @@
 
 #include "../../include/lpm.h"
 #include "ku2d.hpp"
+#include "ku3b.hpp"
 #include "ku3d.hpp"
 
 namespace lpmc {
@@ -24,10 +25,11 @@ int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
     if (e_ != hipSuccess) return lpmc::fail(LPM_E_HIP, "kernel launch: %s", hipGetErrorString(e_)); \
   } while (0)
 
-// K u launch geometry: grid (tiles x z chunks, nl, nb), block = tx * ty threads, zc planes per block
+// K u launch geometry: grid (tiles x z chunks, nl, nb), block = tx * ty threads, zc planes per block; box = 1: the
+// 3D box kernel (ku3b.hpp), tiles of kBoxX x kBoxY x kBoxZ sites, nzc = z tiles, zc = kBoxZ
 struct KuGeom {
   dim3 grid, block;
-  int ntx, nty, nzc, zc, tx, ty;
+  int ntx, nty, nzc, zc, tx, ty, box;
 };
 
 template <typename T, int MODE, int EPI, bool BONDS>
@@ -38,6 +40,8 @@ constexpr int kLeanBlocksPerCU = 3;
 template <int EPI>
 int launch_ku3l(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const lpm::Ku3Args<float, float>& a0,
                 hipStream_t st);
+template <typename T, int MODE, int EPI, bool BONDS>
+int launch_ku3b(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const lpm::Ku3Args<T, T>& a0, hipStream_t st);
 template <typename T, int MODE, int EPI, bool BONDS>
 int launch_ku2(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const lpm::Ku2Args<T, T>& a0, hipStream_t st);
 

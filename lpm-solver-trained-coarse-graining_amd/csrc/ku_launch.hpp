@@ -46,6 +46,26 @@ int launch_ku3(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const lpm::K
 }
 
 template <typename T, int MODE, int EPI, bool BONDS>
+int launch_ku3b(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const lpm::Ku3Args<T, T>& a0, hipStream_t st) {
+  lpm::Ku3Args<T, T> a = a0;
+  a.nx = g->nx;
+  a.ny = g->ny;
+  a.nz = g->nz;
+  a.nl = nl;
+  a.zc = kg.zc;
+  a.ntx = kg.ntx;
+  a.nty = kg.nty;
+  a.nzc = kg.nzc;
+  constexpr int NT = lpm::kBoxX * lpm::kBoxY * lpm::kBoxZ;
+  if (!kg.box || kg.block.x != (unsigned)NT || kg.tx != lpm::kBoxX || kg.ty != lpm::kBoxY || kg.zc != lpm::kBoxZ ||
+      kg.grid.x != (unsigned)(kg.ntx * kg.nty * kg.nzc))
+    return fail(LPM_E_BADARG, "K u launch: geometry is not the box kernel's");
+  hipLaunchKernelGGL((lpm::ku3b_kernel<T, T, MODE, EPI, BONDS>), kg.grid, kg.block, 0, st, a);
+  LAUNCHCHK();
+  return LPM_OK;
+}
+
+template <typename T, int MODE, int EPI, bool BONDS>
 int launch_ku2(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const lpm::Ku2Args<T, T>& a0, hipStream_t st) {
   lpm::Ku2Args<T, T> a = a0;
   a.nx = g->nx;
@@ -83,7 +103,25 @@ int launch_ku2(const lpm_grid* g, int nb, int nl, const KuGeom& kg, const lpm::K
                                                                             const lpm::Ku3Args<T, T>&, hipStream_t); \
   template int lpmc::launch_ku3<T, lpm::COEF_PACKED, lpm::EPI_RESID, B>(const lpm_grid*, int, int,              \
                                                                           const lpmc::KuGeom&,                     \
-                                                                          const lpm::Ku3Args<T, T>&, hipStream_t);
+                                                                          const lpm::Ku3Args<T, T>&, hipStream_t); \
+  template int lpmc::launch_ku3b<T, lpm::COEF_DIRECTED, lpm::EPI_PLAIN, B>(const lpm_grid*, int, int,                \
+                                                            const lpmc::KuGeom&,                                     \
+                                                            const lpm::Ku3Args<T, T>&, hipStream_t);                 \
+  template int lpmc::launch_ku3b<T, lpm::COEF_DIRECTED, lpm::EPI_MASKDOT, B>(const lpm_grid*, int, int,              \
+                                                            const lpmc::KuGeom&,                                     \
+                                                            const lpm::Ku3Args<T, T>&, hipStream_t);                 \
+  template int lpmc::launch_ku3b<T, lpm::COEF_DIRECTED, lpm::EPI_RESID, B>(const lpm_grid*, int, int,                \
+                                                            const lpmc::KuGeom&,                                     \
+                                                            const lpm::Ku3Args<T, T>&, hipStream_t);                 \
+  template int lpmc::launch_ku3b<T, lpm::COEF_PACKED, lpm::EPI_PLAIN, B>(const lpm_grid*, int, int,                  \
+                                                            const lpmc::KuGeom&,                                     \
+                                                            const lpm::Ku3Args<T, T>&, hipStream_t);                 \
+  template int lpmc::launch_ku3b<T, lpm::COEF_PACKED, lpm::EPI_MASKDOT, B>(const lpm_grid*, int, int,                \
+                                                            const lpmc::KuGeom&,                                     \
+                                                            const lpm::Ku3Args<T, T>&, hipStream_t);                 \
+  template int lpmc::launch_ku3b<T, lpm::COEF_PACKED, lpm::EPI_RESID, B>(const lpm_grid*, int, int,                  \
+                                                            const lpmc::KuGeom&,                                     \
+                                                            const lpm::Ku3Args<T, T>&, hipStream_t);
 
 #define LPM_INSTANTIATE_KU2(T, B)                                                                                  \
   template int lpmc::launch_ku2<T, lpm::COEF_DIRECTED, lpm::EPI_PLAIN, B>(const lpm_grid*, int, int,            \

@@ -101,6 +101,9 @@ inline int vec_blocks(int64_t N) { return (int)std::min<int64_t>(1024, std::max<
 // K u (plain and masked-dot forms) to the lean kernel.  It is opt-in: at 256^3 it is faster alone (443 vs 477 us)
 // but slower inside the PCG iteration (471 vs 444 us), where the vector kernels between launches evict the caches.
 int g_tile_wide = 0, g_zc = 0, g_lean = 0;
+// 3D box kernel (ku3b.hpp): -1 automatic (launches the marching kernel would run with fewer blocks than CUs), 0
+// never, 1 always
+int g_box = -1;
 int g_serve_fused = 1;  // serving: fused band product of the inner iterations (0: two passes)
 
 // compute units of the current device (cached per device; 256 when the runtime cannot tell)
@@ -118,10 +121,23 @@ int cu_count() {
 
 // tile shapes: 3D float 64x4 or 32x8, double 32x4 or 16x8; 2D float 32x8, double 32x4
 template <int DIM, typename T>
-KuGeom ku_geom(const lpm_grid* g, int nb, int nl, bool bonds, int lean_switch = -1) {
+KuGeom ku_geom(const lpm_grid* g, int nb, int nl, bool bonds, int lean_switch = -1, int box_switch = -2) {
   const int use_lean = lean_switch < 0 ? g_lean : lean_switch;
+  const int use_box = box_switch < -1 ? g_box : box_switch;
   KuGeom k{};
   if constexpr (DIM == 3) {
+    if (use_box == 1) {
+      k.box = 1;
+      k.tx = lpm::kBoxX;
+      k.ty = lpm::kBoxY;
+      k.zc = lpm::kBoxZ;
+      k.ntx = (g->nx + lpm::kBoxX - 1) / lpm::kBoxX;
+      k.nty = (g->ny + lpm::kBoxY - 1) / lpm::kBoxY;
+      k.nzc = (g->nz + lpm::kBoxZ - 1) / lpm::kBoxZ;
+      k.grid = dim3(k.ntx * k.nty * k.nzc, nl, nb);
+      k.block = dim3(lpm::kBoxX * lpm::kBoxY * lpm::kBoxZ);
+      return k;
+    }
     const bool wide = g_tile_wide ? g_tile_wide > 0 : g->nx >= 48;
     if (sizeof(T) == 4) {
       k.tx = wide ? 64 : 32;
@@ -152,6 +168,12 @@ KuGeom ku_geom(const lpm_grid* g, int nb, int nl, bool bonds, int lean_switch = 
     }
     k.nzc = (g->nz + k.zc - 1) / k.zc;
     k.grid = dim3(k.ntx * k.nty * k.nzc, nl, nb);
+    // automatic: a marching launch with fewer blocks than CUs (small lattices, few systems) leaves the chip idle
+    // while each block walks its planes; the box kernel spreads the same sites over about 16 times more blocks.
+    // Measured kernel durations (FP32 packed): 24^3 x 16 systems 25.7 -> 14.6 us, 64^3 27.0 -> 20.8 us; with a
+    // full marching launch the box kernel is not faster (48^3 x 9: 44.6 against 47.5 us; 128^3: 61 against 116).
+    if (use_box < 0 && g_tile_wide == 0 && g_zc == 0 && (int64_t)k.grid.x * nl * nb < cu_count())
+      return ku_geom<DIM, T>(g, nb, nl, bonds, lean_switch, 1);
   } else {
     k.tx = 32;
     k.ty = sizeof(T) == 4 ? 8 : 4;
@@ -173,8 +195,10 @@ KuGeom ku_geom(const lpm_grid* g, int nb, int nl) {
 template <int DIM, typename T>
 int ku_blocks_max(const lpm_grid* g, int nsys) {
   unsigned m = 0;
-  for (int lean = 0; lean < 2; ++lean)
-    for (int bonds = 0; bonds < 2; ++bonds) m = std::max(m, ku_geom<DIM, T>(g, 1, nsys, bonds != 0, lean).grid.x);
+  for (int box = 0; box < 2; ++box)
+    for (int lean = 0; lean < 2; ++lean)
+      for (int bonds = 0; bonds < 2; ++bonds)
+        m = std::max(m, ku_geom<DIM, T>(g, 1, nsys, bonds != 0, lean, box).grid.x);
   return (int)m;
 }
 
@@ -222,6 +246,13 @@ int ku_call(const lpm_grid* g, int nb, int nl, int layout, const KuGeom& kg, con
     a.act_stride = NINT;
     a.bonds = g->bonds;
     a.bb = Bcast{g->bb.sb, g->bb.sl};
+    if (kg.box) {
+      if (g->bonds)
+        return layout == LPM_COEF_PACKED ? launch_ku3b<T, COEF_PACKED, EPI, true>(g, nb, nl, kg, a, st)
+                                         : launch_ku3b<T, COEF_DIRECTED, EPI, true>(g, nb, nl, kg, a, st);
+      return layout == LPM_COEF_PACKED ? launch_ku3b<T, COEF_PACKED, EPI, false>(g, nb, nl, kg, a, st)
+                                       : launch_ku3b<T, COEF_DIRECTED, EPI, false>(g, nb, nl, kg, a, st);
+    }
     if (g->bonds)
       return layout == LPM_COEF_PACKED ? launch_ku3<T, COEF_PACKED, EPI, true>(g, nb, nl, kg, a, st)
                                        : launch_ku3<T, COEF_DIRECTED, EPI, true>(g, nb, nl, kg, a, st);
@@ -488,6 +519,8 @@ void lpm_set_ku_tiling(int32_t wide, int32_t zc) {
 }
 
 void lpm_set_ku_kernel(int32_t lean) { g_lean = lean != 0; }
+
+void lpm_set_ku_box(int32_t box) { g_box = box < 0 ? -1 : (box > 0 ? 1 : 0); }
 
 void lpm_profile_ku(int32_t enable) {
   g_prof.on = enable != 0;

@@ -309,6 +309,47 @@ def test_apply_tilings_and_z_chunks(dev, wide, zc, dt):
         L.lib().lpm_set_ku_tiling(0, 0)
 
 
+_BOX_DIMS = [(1, 1, 1), (3, 2, 1), (17, 5, 9), (33, 7, 6), (48, 9, 13)]
+
+
+@pytest.mark.parametrize("dims", _BOX_DIMS)
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_box_kernel_matches_oracle_and_marching_kernel(dev, dims, dt):
+    """The box K u kernel (one block per 16 x 4 x 4 sites: partial boxes on every axis, single sites) and the
+    marching kernel, each forced, give the oracle's K u, packed and directed; inside the FP64 PCG (masked-dot form)
+    and the refined solver (FP64 residual form) the box kernel reaches the oracle's PCG solution."""
+    print("box", dims, dt, flush=True)
+    lat, Kn, Tv, u = case(dims, seed=23, two_phase=True)
+    ref = R.apply(lat, u, Kn, Tv)
+    t = lambda a: torch.as_tensor(a, device=dev, dtype=dt)
+    kp, tp, _ = ops.pack(dims, t(Kn), t(Tv))
+    try:
+        outs = {}
+        for box in (1, 0):
+            L.lib().lpm_set_ku_box(box)
+            outs[box] = ops.apply(dims, 1.0, t(u), kp, tp, layout=L.PACKED)
+            assert rel(outs[box], ref) < TOL[dt], box
+            assert rel(ops.apply(dims, 1.0, t(u), t(Kn), t(Tv)), ref) < TOL[dt], box
+        assert rel(outs[1], outs[0].double().cpu().numpy()) < TOL[dt]
+        L.lib().lpm_set_ku_box(1)
+        if dt == torch.float64 and min(dims) >= 2:
+            fixed, up, _ = R.prescribed3("tb_tz", lat.x, lat.ijk, dims, 1)
+            b = -R.apply(lat, up * fixed[:, None], Kn, Tv) * (~fixed)[:, None]
+            xr, _, _ = R.pcg(lat, b, Kn, Tv, R.precond(lat, Kn, Tv, ~fixed), ~fixed, tol=1e-12, maxit=20000)
+            free = torch.as_tensor(~fixed, device=dev)
+            minv = ops.block_jacobi(dims, 1.0, kp, tp, free=free, layout=L.PACKED)
+            x, info = ops.pcg(dims, 1.0, t(b), kp, tp, minv=minv, free=free, tol=1e-12, maxit=20000,
+                              layout=L.PACKED)
+            assert info.conv.all() and rel(x, xr) < 1e-9
+            k32, t32, _ = ops.pack(dims, t(Kn), t(Tv), out_dtype=torch.float32)
+            m32 = ops.block_jacobi(dims, 1.0, kp, tp, free=free, layout=L.PACKED, out_dtype=torch.float32)
+            x, info = ops.pcg_refined(dims, 1.0, t(b), kp, tp, k32, t32, minv32=m32, free=free, tol=1e-10,
+                                      maxit=20000, inner_tol=1e-4, layout64=L.PACKED, layout32=L.PACKED)
+            assert info.conv.all() and rel(x, xr) < 1e-8
+    finally:
+        L.lib().lpm_set_ku_box(-1)
+
+
 _LEAN_DIMS = [(64, 4, 1), (64, 1, 3), (128, 5, 7), (130, 9, 13), (64, 8, 20), (96, 6, 15)]
 _LEAN_REF = {}
 

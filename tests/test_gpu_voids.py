@@ -225,6 +225,25 @@ def test_void_tilings_and_z_chunks(dev, wide, zc):
         L.lib().lpm_set_ku_tiling(0, 0)
 
 
+def test_void_box_kernel(dev):
+    """The box K u kernel with bond words (forced on), both value types and layouts, against the oracle."""
+    print("void box", flush=True)
+    dims = (37, 11, 14)
+    full, lat, solid, Kf, Tf, u = case(dims, seed=19, frac=0.25)
+    ref = R.apply(lat, u[lat.sites], Kf[lat.sites], Tf[lat.sites])
+    sites = torch.as_tensor(lat.sites, device=dev)
+    w = ops.bond_words(dims, torch.as_tensor(solid, device=dev))
+    try:
+        L.lib().lpm_set_ku_box(1)
+        for dt in (torch.float32, torch.float64):
+            t = lambda a: torch.as_tensor(a, device=dev, dtype=dt)
+            kp, tp, _ = ops.pack(dims, t(Kf), t(Tf))
+            assert rel(ops.apply(dims, 1.0, t(u), kp, tp, layout=L.PACKED, bonds=w)[sites], ref) < TOL[dt]
+            assert rel(ops.apply(dims, 1.0, t(u), t(Kf), t(Tf), bonds=w)[sites], ref) < TOL[dt]
+    finally:
+        L.lib().lpm_set_ku_box(-1)
+
+
 # ------------------------------------------------------------------------------------------------ torch layer
 import torch_lpm3d as T3  # noqa: E402
 import lin3d  # noqa: E402
