@@ -17,7 +17,10 @@ __device__ __forceinline__ T nan_of() {
 }
 
 // per-system scalars (type T) and integer state
-enum CgScal : int { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_BN, S_TOL, S_REL, NSCAL };
+// S_RZN: r.z of the newest residual, written where it is summed (cg_fin_init, cg_pupdate_fused); the fused update
+// copies it to S_RZ, the r.z of the iteration's own residual, so no kernel reads a scalar another block of the same
+// launch writes.
+enum CgScal : int { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_BN, S_TOL, S_REL, S_RZN, NSCAL };
 // I_XUPD: the x += alpha p of the current iteration is still to be applied (set by cg_fin_alpha, consumed by
 // cg_pupdate, which folds the x update into its pass over p).
 // I_NOTSPD: the system stopped because K (or M) is not positive definite on it: p.Ap <= 0 for a nonzero search
@@ -127,6 +130,17 @@ __device__ __forceinline__ T sum_partials(const T* __restrict__ pp, int nblk, in
   for (int j = threadIdx.x; j < nblk; j += blockDim.x) acc += pp[(int64_t)j * stride + c];
   return block_sum(acc, red);
 }
+// the same sum in every thread of the block
+template <typename T>
+__device__ __forceinline__ T sum_partials_all(const T* __restrict__ pp, int nblk, int stride, int c, T* red,
+                                              T* slot) {
+  const T v = sum_partials(pp, nblk, stride, c, red);
+  if (threadIdx.x == 0) *slot = v;
+  __syncthreads();
+  const T r = *slot;
+  __syncthreads();
+  return r;
+}
 
 // after cg_init: rz, rr, bn; a system with b = 0 is done at iteration 0 (coarse3d/lin3d.py:479-480).  In snapshot
 // mode the levels the initial residual meets are recorded with iteration 0 and the system is done once every level
@@ -144,6 +158,7 @@ __global__ __launch_bounds__(256) void cg_fin_init_kernel(T* partial, int nblk, 
     int* is = ist + (int64_t)s * NINT;
     const T bn = sqrt(bb);
     sc[S_RZ] = rz;
+    sc[S_RZN] = rz;
     sc[S_RR] = rr;
     sc[S_BN] = bn;
     sc[S_ALPHA] = T(0);
@@ -358,6 +373,169 @@ __global__ __launch_bounds__(256) void cg_pupdate_kernel(CgVec<T> v, SnapDev<T> 
         for (int a = 0; a < D; ++a) xv[u].c[a] += alpha * pv[u].c[a];
         st_site<D>(x_, q * D, xv[u]);
         for (int k = k0; k < k1; ++k) st_site<D>(sd.snaps + (s * sd.nt + k) * v.N * D, q * D, xv[u]);
+        if (active) {
+          for (int a = 0; a < D; ++a) pv[u].c[a] = zv[u].c[a] + beta * pv[u].c[a];
+          st_site<D>(p_, q * D, pv[u]);
+        }
+      }
+    }
+  }
+}
+
+// ---- the iteration in three launches (K p, update, p update) instead of five: the scalar steps cg_fin_alpha and
+// cg_fin_beta are folded into the vector kernels that consume them.  Every block of a fused kernel sums its
+// system's partials of the previous launch itself, in the same fixed order, so all blocks take the same scalars and
+// decisions; block 0 records them for later launches and the host.  A block never reads a value another block of
+// the same launch writes: I_ACTIVE is read before block 0 may clear it (and a block that reads it cleared has
+// p.Ap <= 0 itself), the iteration count is advanced by the update and read by the p update, and the r.z of the new
+// residual goes to S_RZN, which the next update copies to S_RZ.  Not used with snapshot levels (cg_fin_beta
+// records those).
+
+// cg_fin_alpha + cg_update: r -= alpha mask Ap, z = M r; partials (r.z, r.r) into v.partial (the K p partials are
+// kpart, knblk per system).
+template <int D, typename T>
+__global__ __launch_bounds__(256) void cg_update_fused_kernel(CgVec<T> v, const T* __restrict__ kpart, int knblk) {
+  __shared__ T red[4];
+  __shared__ T bc;
+  const int l = blockIdx.y, b = blockIdx.z;
+  const int64_t s = (int64_t)b * v.nl + l;
+  int* is = v.ist + s * NINT;
+  T* sc = v.scal + s * NSCAL;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!is[I_ACTIVE]) {
+    if (lead) is[I_XUPD] = 0;
+    return;
+  }
+  const T pap = sum_partials_all(kpart + s * knblk, knblk, 1, 0, red, &bc);
+  const T rz = sc[S_RZN];
+  // an active system has r.z > 0, so p != 0 and p.Ap > 0 for SPD K; p.Ap <= 0: K is not positive definite on the
+  // free DOFs, the system stops without an update (a NaN p.Ap is left to the non-finite check of the residual)
+  const bool spd = !(pap <= T(0));
+  if (lead) {
+    sc[S_PAP] = pap;
+    sc[S_RZ] = rz;
+    sc[S_ALPHA] = spd ? rz / pap : T(0);
+    is[I_XUPD] = spd ? 1 : 0;
+    if (spd) {
+      is[I_IT] = is[I_IT] + 1;
+    } else {
+      is[I_NOTSPD] = 1;
+      is[I_ACTIVE] = 0;
+    }
+  }
+  if (!spd) return;
+  constexpr int NS = D == 3 ? 6 : 3;
+  const T alpha = rz / pap;
+  T* __restrict__ r_ = v.r + s * v.N * D;
+  T* __restrict__ z_ = v.z ? v.z + s * v.N * D : nullptr;
+  const T* __restrict__ ap_ = v.ap + s * v.N * D;
+  const T* __restrict__ mi_ = v.minv ? v.minv + v.bminv.at(b, l) : nullptr;
+  const uint8_t* __restrict__ mk_ = v.mask ? v.mask + v.bm.at(b, l) : nullptr;
+  T rzs = T(0), rr = T(0);
+  const int64_t step = (int64_t)gridDim.x * blockDim.x * VU;
+  for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x * VU + threadIdx.x; p0 < v.N; p0 += step) {
+    Site<D, T> rv[VU], av[VU];
+    T m[VU][3];
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+      const int64_t p = p0 + (int64_t)u * blockDim.x;
+      if (p < v.N) {
+        rv[u] = ld_site<D>(r_, p * D);
+        av[u] = ld_site<D>(ap_, p * D);
+        for (int a = 0; a < D; ++a) m[u][a] = mk_ ? (mk_[D * p + a] ? T(1) : T(0)) : T(1);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+      const int64_t p = p0 + (int64_t)u * blockDim.x;
+      if (p < v.N) {
+        T r[3], z[3];
+        for (int a = 0; a < D; ++a) {
+          r[a] = rv[u].c[a] - alpha * (m[u][a] * av[u].c[a]);
+          rv[u].c[a] = r[a];
+        }
+        st_site<D>(r_, p * D, rv[u]);
+        if (mi_) {
+          apply_minv<D, T>(mi_ + NS * p, r, z);
+          Site<D, T> zv;
+          for (int a = 0; a < D; ++a) zv.c[a] = z[a] = m[u][a] * z[a];
+          st_site<D>(z_, p * D, zv);
+        } else {
+          for (int a = 0; a < D; ++a) z[a] = r[a];
+        }
+        for (int a = 0; a < D; ++a) {
+          rzs += r[a] * z[a];
+          rr += r[a] * r[a];
+        }
+      }
+    }
+  }
+  const T s0 = block_sum(rzs, red);
+  const T s1 = block_sum(rr, red);
+  if (threadIdx.x == 0) {
+    T* pp = v.partial + (s * gridDim.x + blockIdx.x) * 3;
+    pp[0] = s0;
+    pp[1] = s1;
+  }
+}
+
+// cg_fin_beta + cg_pupdate: convergence / failure decision and beta from the update's partials (upart, unblk per
+// system), x += alpha p, and p = z + beta p while the system goes on.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void cg_pupdate_fused_kernel(CgVec<T> v, const T* __restrict__ upart, int unblk) {
+  __shared__ T red[4];
+  __shared__ T bc;
+  const int l = blockIdx.y, b = blockIdx.z;
+  const int64_t s = (int64_t)b * v.nl + l;
+  int* is = v.ist + s * NINT;
+  T* sc = v.scal + s * NSCAL;
+  if (!is[I_XUPD]) return;
+  const T* pp = upart + s * unblk * 3;
+  const T rzn = sum_partials_all(pp, unblk, 3, 0, red, &bc);
+  const T rr = sum_partials_all(pp, unblk, 3, 1, red, &bc);
+  const int it = is[I_IT];
+  const T bn = sc[S_BN], rn = sqrt(rr);
+  // the decisions of cg_fin_beta, in its order
+  const bool met = rn <= sc[S_TOL] * bn;
+  const bool bad = !met && (!is_finite(rn) || !is_finite(rzn));
+  const bool indef = !met && !bad && rzn < T(0);
+  const bool cap = !met && !bad && !indef && it >= is[I_MAXIT];
+  const bool active = !met && !bad && !indef && !cap;
+  const T rz = sc[S_RZ];
+  const T beta = rz != T(0) ? rzn / rz : T(0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_RR] = rr;
+    sc[S_REL] = bad ? nan_of<T>() : rn / bn;
+    if (met) is[I_CONV] = 1;
+    if (indef) is[I_NOTSPD] = 1;
+    if (!active) is[I_ACTIVE] = 0;
+    if (active) {
+      sc[S_BETA] = beta;
+      sc[S_RZN] = rzn;
+    }
+  }
+  const T alpha = sc[S_ALPHA];
+  T* __restrict__ x_ = v.x + s * v.N * D;
+  T* __restrict__ p_ = v.p + s * v.N * D;
+  const T* __restrict__ z_ = (v.z ? v.z : v.r) + s * v.N * D;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x * VU;
+  for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x * VU + threadIdx.x; p0 < v.N; p0 += step) {
+    Site<D, T> pv[VU], xv[VU], zv[VU];
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+      const int64_t q = p0 + (int64_t)u * blockDim.x;
+      if (q < v.N) {
+        pv[u] = ld_site<D>(p_, q * D);
+        xv[u] = ld_site<D>(x_, q * D);
+        if (active) zv[u] = ld_site<D>(z_, q * D);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+      const int64_t q = p0 + (int64_t)u * blockDim.x;
+      if (q < v.N) {
+        for (int a = 0; a < D; ++a) xv[u].c[a] += alpha * pv[u].c[a];
+        st_site<D>(x_, q * D, xv[u]);
         if (active) {
           for (int a = 0; a < D; ++a) pv[u].c[a] = zv[u].c[a] + beta * pv[u].c[a];
           st_site<D>(p_, q * D, pv[u]);

@@ -312,7 +312,7 @@ __global__ void cg_settol_kernel(T* scal, const double* tol, int nsys) {
 }
 
 struct PcgLayout {
-  size_t r, z, p, ap, partial, scal, ist, tols, snapit, total;
+  size_t r, z, p, ap, partial, partial2, scal, ist, tols, snapit, total;
   int maxblk;
 };
 constexpr int kMaxSnapLevels = 64;  // tolerances of one snapshot solve
@@ -336,6 +336,8 @@ PcgLayout pcg_layout(const lpm_grid* g, int nsys) {
   o += vec;
   L.partial = o;
   o += align256(sizeof(T) * (size_t)nsys * L.maxblk * 3);
+  L.partial2 = o;  // the update's partials while the K p partials are still read (fused iteration)
+  o += align256(sizeof(T) * (size_t)nsys * vblk * 3);
   L.scal = o;
   o += align256(sizeof(T) * (size_t)nsys * NSCAL);
   L.ist = o;
@@ -364,6 +366,7 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
   T* p = (T*)(work + L.p);
   T* ap = (T*)(work + L.ap);
   T* partial = (T*)(work + L.partial);
+  T* partial2 = (T*)(work + L.partial2);
   T* scal = (T*)(work + L.scal);
   int* ist = (int*)(work + L.ist);
   const Bcast bvec{(int64_t)nl * N * D, N * D};
@@ -433,10 +436,20 @@ int pcg_core(const lpm_grid* g, int nb, int nl, int layout, const void* kn, cons
       HIPCHK(hipEventRecord(g_prof.ev[g_prof.used + 1], sq));
       g_prof.used += 2;
     }
-    hipLaunchKernelGGL((cg_fin_alpha_kernel<T>), dim3(nsys), dim3(256), 0, sq, partial, (int)kg.grid.x, scal, ist);
-    hipLaunchKernelGGL((cg_update_kernel<D, T>), vgrid, dim3(256), 0, sq, v);
-    hipLaunchKernelGGL((cg_fin_beta_kernel<T>), dim3(nsys), dim3(256), 0, sq, partial, vblk, scal, ist, sd);
-    hipLaunchKernelGGL((cg_pupdate_kernel<D, T>), vgrid, dim3(256), 0, sq, v, sd);
+    if (sd.nt > 0) {
+      hipLaunchKernelGGL((cg_fin_alpha_kernel<T>), dim3(nsys), dim3(256), 0, sq, partial, (int)kg.grid.x, scal,
+                         ist);
+      hipLaunchKernelGGL((cg_update_kernel<D, T>), vgrid, dim3(256), 0, sq, v);
+      hipLaunchKernelGGL((cg_fin_beta_kernel<T>), dim3(nsys), dim3(256), 0, sq, partial, vblk, scal, ist, sd);
+      hipLaunchKernelGGL((cg_pupdate_kernel<D, T>), vgrid, dim3(256), 0, sq, v, sd);
+    } else {
+      // the scalar steps folded into the vector kernels (three launches per iteration)
+      CgVec<T> vf = v;
+      vf.partial = partial2;
+      hipLaunchKernelGGL((cg_update_fused_kernel<D, T>), vgrid, dim3(256), 0, sq, vf, (const T*)partial,
+                         (int)kg.grid.x);
+      hipLaunchKernelGGL((cg_pupdate_fused_kernel<D, T>), vgrid, dim3(256), 0, sq, vf, (const T*)partial2, vblk);
+    }
     LAUNCHCHK();
     return LPM_OK;
   };
