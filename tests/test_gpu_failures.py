@@ -1,4 +1,4 @@
-"""Failure paths of the solve path on the GPU (SURVEY-listed conventions of the reference):
+"""Failure paths of the solve path on the GPU (the reference's conventions):
 
 * a coefficient set whose K is not positive definite raises FloatingPointError, as the reference's Cholesky does
   (coarse2d/torch_lpm.py:20-25); the batched PCG detects it on the device (r.z < 0 or p.Ap <= 0) and the host maps

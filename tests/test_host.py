@@ -33,9 +33,9 @@ def test_abi_exports_every_declared_symbol():
 
 
 def test_integration_binding_matches_the_library_structs(monkeypatch):
-    """The ctypes binding INTEGRATION.md gives maintainers, executed as written: its Grid / Bcast have the size and
-    field offsets of the library's own binding (lpm_grid is 48 bytes with the bond-word pointer and its strides),
-    and its lpm_apply signature is the one the library is driven through."""
+    """The ctypes binding the integration guide gives maintainers, executed as written: its Grid / Bcast have the
+    size and field offsets of the library's own binding (lpm_grid is 48 bytes with the bond-word pointer and its
+    strides), and its lpm_apply signature is the one the library is driven through."""
     import _lpmlib as L
     txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     blocks = re.findall(r"```python\n(.*?)```", txt, re.S)
