@@ -1086,8 +1086,8 @@ def main():
         probe(a.probe, a.probe_reps)
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every box of this build has one MI355X and the solve path runs on a "
-              "single device (run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: the solve path runs on a single MI355X (run with --gpus 1); sharding over "
+              "several GPUs is a design, not an implementation, of this library", file=sys.stderr)
         sys.exit(2)
     if a.config != "cube256":
         {"plain2d": bench_plain2d, "kubc3d": bench_kubc3d, "serve": bench_serve, "train3d": bench_train3d,

@@ -32,7 +32,7 @@ enum {
   LPM_E_HIP = 2,         /* HIP runtime error */
   LPM_E_NOT_SPD = 3,     /* preconditioner block or curvature not positive */
   LPM_E_NONFINITE = 4,   /* NaN / Inf met in the iteration */
-  LPM_E_UNSUPPORTED = 5, /* valid request this build does not implement */
+  LPM_E_UNSUPPORTED = 5, /* valid request the library does not implement */
   LPM_E_FULL = 6         /* serving: no free slot or arena space (nothing changed) */
 };
 enum { LPM_F32 = 0, LPM_F64 = 1 };
