@@ -406,6 +406,29 @@ __global__ __launch_bounds__(256) void cg_update_fused_kernel(CgVec<T> v, const 
     if (lead) is[I_XUPD] = 0;
     return;
   }
+  constexpr int NS = D == 3 ? 6 : 3;
+  T* __restrict__ r_ = v.r + s * v.N * D;
+  T* __restrict__ z_ = v.z ? v.z + s * v.N * D : nullptr;
+  const T* __restrict__ ap_ = v.ap + s * v.N * D;
+  const T* __restrict__ mi_ = v.minv ? v.minv + v.bminv.at(b, l) : nullptr;
+  const uint8_t* __restrict__ mk_ = v.mask ? v.mask + v.bm.at(b, l) : nullptr;
+  Site<D, T> rv[VU], av[VU];
+  T m[VU][3];
+  auto load = [&](int64_t p0) {
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+      const int64_t p = p0 + (int64_t)u * blockDim.x;
+      if (p < v.N) {
+        rv[u] = ld_site<D>(r_, p * D);
+        av[u] = ld_site<D>(ap_, p * D);
+        for (int a = 0; a < D; ++a) m[u][a] = mk_ ? (mk_[D * p + a] ? T(1) : T(0)) : T(1);
+      }
+    }
+  };
+  const int64_t step = (int64_t)gridDim.x * blockDim.x * VU;
+  const int64_t first = (int64_t)blockIdx.x * blockDim.x * VU + threadIdx.x;
+  // the first group's loads are in flight while the block sums the K p partials
+  load(first);
   const T pap = sum_partials_all(kpart + s * knblk, knblk, 1, 0, red, &bc);
   const T rz = sc[S_RZN];
   // an active system has r.z > 0, so p != 0 and p.Ap > 0 for SPD K; p.Ap <= 0: K is not positive definite on the
@@ -424,27 +447,10 @@ __global__ __launch_bounds__(256) void cg_update_fused_kernel(CgVec<T> v, const 
     }
   }
   if (!spd) return;
-  constexpr int NS = D == 3 ? 6 : 3;
   const T alpha = rz / pap;
-  T* __restrict__ r_ = v.r + s * v.N * D;
-  T* __restrict__ z_ = v.z ? v.z + s * v.N * D : nullptr;
-  const T* __restrict__ ap_ = v.ap + s * v.N * D;
-  const T* __restrict__ mi_ = v.minv ? v.minv + v.bminv.at(b, l) : nullptr;
-  const uint8_t* __restrict__ mk_ = v.mask ? v.mask + v.bm.at(b, l) : nullptr;
   T rzs = T(0), rr = T(0);
-  const int64_t step = (int64_t)gridDim.x * blockDim.x * VU;
-  for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x * VU + threadIdx.x; p0 < v.N; p0 += step) {
-    Site<D, T> rv[VU], av[VU];
-    T m[VU][3];
-#pragma unroll
-    for (int u = 0; u < VU; ++u) {
-      const int64_t p = p0 + (int64_t)u * blockDim.x;
-      if (p < v.N) {
-        rv[u] = ld_site<D>(r_, p * D);
-        av[u] = ld_site<D>(ap_, p * D);
-        for (int a = 0; a < D; ++a) m[u][a] = mk_ ? (mk_[D * p + a] ? T(1) : T(0)) : T(1);
-      }
-    }
+  for (int64_t p0 = first; p0 < v.N; p0 += step) {
+    if (p0 != first) load(p0);
 #pragma unroll
     for (int u = 0; u < VU; ++u) {
       const int64_t p = p0 + (int64_t)u * blockDim.x;
@@ -490,6 +496,24 @@ __global__ __launch_bounds__(256) void cg_pupdate_fused_kernel(CgVec<T> v, const
   int* is = v.ist + s * NINT;
   T* sc = v.scal + s * NSCAL;
   if (!is[I_XUPD]) return;
+  T* __restrict__ x_ = v.x + s * v.N * D;
+  T* __restrict__ p_ = v.p + s * v.N * D;
+  const T* __restrict__ z_ = (v.z ? v.z : v.r) + s * v.N * D;
+  Site<D, T> pv[VU], xv[VU], zv[VU];
+  auto load_px = [&](int64_t p0) {
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+      const int64_t q = p0 + (int64_t)u * blockDim.x;
+      if (q < v.N) {
+        pv[u] = ld_site<D>(p_, q * D);
+        xv[u] = ld_site<D>(x_, q * D);
+      }
+    }
+  };
+  const int64_t step = (int64_t)gridDim.x * blockDim.x * VU;
+  const int64_t first = (int64_t)blockIdx.x * blockDim.x * VU + threadIdx.x;
+  // p and x of the first group are in flight while the block sums the update's partials
+  load_px(first);
   const T* pp = upart + s * unblk * 3;
   const T rzn = sum_partials_all(pp, unblk, 3, 0, red, &bc);
   const T rr = sum_partials_all(pp, unblk, 3, 1, red, &bc);
@@ -515,20 +539,12 @@ __global__ __launch_bounds__(256) void cg_pupdate_fused_kernel(CgVec<T> v, const
     }
   }
   const T alpha = sc[S_ALPHA];
-  T* __restrict__ x_ = v.x + s * v.N * D;
-  T* __restrict__ p_ = v.p + s * v.N * D;
-  const T* __restrict__ z_ = (v.z ? v.z : v.r) + s * v.N * D;
-  const int64_t step = (int64_t)gridDim.x * blockDim.x * VU;
-  for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x * VU + threadIdx.x; p0 < v.N; p0 += step) {
-    Site<D, T> pv[VU], xv[VU], zv[VU];
+  for (int64_t p0 = first; p0 < v.N; p0 += step) {
+    if (p0 != first) load_px(p0);
 #pragma unroll
     for (int u = 0; u < VU; ++u) {
       const int64_t q = p0 + (int64_t)u * blockDim.x;
-      if (q < v.N) {
-        pv[u] = ld_site<D>(p_, q * D);
-        xv[u] = ld_site<D>(x_, q * D);
-        if (active) zv[u] = ld_site<D>(z_, q * D);
-      }
+      if (q < v.N && active) zv[u] = ld_site<D>(z_, q * D);
     }
 #pragma unroll
     for (int u = 0; u < VU; ++u) {
